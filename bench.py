@@ -1,0 +1,262 @@
+"""Headline benchmark: umappp graph + layout hot path on one MI355X.
+
+Workload (config C3 of the reference's headline): 1,000,000 observations x 50
+dims float32 from a 100-component Gaussian mixture (seed 3), exact kNN k = 15
+(int32 indices, float32 distances) precomputed on the GPU before timing,
+out_dim = 2, 200 epochs, negative_sample_rate = 5, min_dist 0.1 / spread 1.
+
+One step = the whole hot path from a resident kNN graph: fuzzy similarities
+(K1), fuzzy union (K2), epoch schedule (K3), setting the initial embedding and
+200 layout epochs (K4).  value = edge-updates / s over the step, where an
+edge-update is one firing of a stored edge (attractive move plus its negative
+samples) and the count is the kernel's exact tally.
+
+Extra JSON fields: roofline of the layout epoch kernel (HIP-event launch times
+on the library's stream), cpu_baseline (the oracle's conflict-scheduled
+parallel layout and graph build on the host cores, timed on a bounded sample),
+and a per-stage breakdown.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "edge-updates/s, 1M pts k=15 200 ep: batched/autograd vs plain GPU at ref kNN-recall"
+HBM_PEAK_GBS = 8000.0
+
+CONFIGS = {
+    "c3": dict(n=1_000_000, dim=50, ncomp=100, seed=3, k=15, epochs=200,
+               label="1M x 50 f32, 100-component mixture, exact kNN k=15, D=2, 200 epochs, nsr 5"),
+    "c2": dict(n=100_000, dim=50, ncomp=30, seed=2, k=15, epochs=200,
+               label="100k x 50 f32, 30-component mixture, exact kNN k=15, D=2, 200 epochs, nsr 5"),
+}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_mixture(seed, n, dim, ncomp):
+    rng = np.random.default_rng(seed)
+    means = rng.normal(0.0, 5.0, size=(ncomp, dim))
+    sds = rng.uniform(0.5, 2.0, size=(ncomp, dim))
+    lab = rng.integers(0, ncomp, size=n)
+    x = means[lab] + rng.normal(size=(n, dim)) * sds[lab]
+    return x.astype(np.float32)
+
+
+def knn_torch(x_np, k, dev):
+    """Exact kNN (self excluded) on the GPU: chunked ||q||^2 + ||x||^2 - 2 q.x
+    screening for k + 8 candidates, then exact distances of the candidates
+    recomputed and sorted.  Input preparation, not part of the timed path."""
+    import torch
+
+    x = torch.from_numpy(x_np).to(dev)
+    n = x.shape[0]
+    sq = (x * x).sum(1)
+    kk = k + 8
+    idx_out = torch.empty((n, k), dtype=torch.int32, device=dev)
+    dist_out = torch.empty((n, k), dtype=torch.float32, device=dev)
+    q = 4096
+    for s in range(0, n, q):
+        e = min(n, s + q)
+        d = sq[s:e, None] + sq[None, :] - 2.0 * (x[s:e] @ x.T)
+        d[torch.arange(e - s, device=dev), torch.arange(s, e, device=dev)] = float("inf")
+        cand = torch.topk(d, kk, dim=1, largest=False).indices
+        diff = x[cand] - x[s:e, None, :]
+        exact = torch.sqrt((diff.double() ** 2).sum(-1)).float()
+        order = torch.argsort(exact, dim=1, stable=True)[:, :k]
+        idx_out[s:e] = torch.gather(cand, 1, order).int()
+        dist_out[s:e] = torch.gather(exact, 1, order)
+        if (s // q) % 50 == 0:
+            log(f"knn {e}/{n}")
+    torch.cuda.synchronize()
+    return idx_out, dist_out
+
+
+def read_traffic(cfg_key):
+    p = ROOT / "profiles" / f"layout_epoch_{cfg_key}_traffic.json"
+    if p.exists():
+        try:
+            return json.loads(p.read_text())
+        except Exception:
+            return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU layout sample time")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import umappp_amd as umx
+
+    umx.lib()
+    dev = torch.device("cuda:0")
+    cfg = CONFIGS[args.config]
+    n, k, T = cfg["n"], cfg["k"], cfg["epochs"]
+    nsr = 5.0
+    a, b = umx.find_ab(1.0, 0.1)
+
+    log(f"generating {cfg['label']}")
+    x = make_mixture(cfg["seed"], n, cfg["dim"], cfg["ncomp"])
+    t0 = time.perf_counter()
+    kidx, kdist = knn_torch(x, k, dev)
+    log(f"knn done in {time.perf_counter() - t0:.1f}s")
+    rp = torch.arange(n + 1, dtype=torch.int64, device=dev) * k
+    rng = np.random.default_rng(cfg["seed"] + 100)
+    emb0 = torch.from_numpy(rng.uniform(-10, 10, size=(n, 2)).astype(np.float32)).to(dev)
+    emb = torch.empty_like(emb0)
+    torch.cuda.synchronize()
+
+    def step(timing=False):
+        g = umx.Graph((rp.data_ptr(), kidx.data_ptr(), kdist.data_ptr()), device_ptrs=True, n=n,
+                      dtype=np.float32)
+        t_a = time.perf_counter()
+        g.neighbor_similarities(1.0, 1.0, 1e-3, True)
+        t_b = time.perf_counter()
+        g.combine(1.0)
+        t_c = time.perf_counter()
+        lay = umx.Layout(graph=g, num_epochs=T, negative_sample_rate=nsr)
+        t_d = time.perf_counter()
+        lay.set_params(2, a, b, 1.0, 1.0, 1234567890)
+        lay.set_embedding(device_ptr=emb0.data_ptr())
+        lay.set_timing(timing)
+        lay.run(T)
+        t_e = time.perf_counter()
+        lay.get_embedding(device_ptr=emb.data_ptr())
+        return lay, dict(similarities=t_b - t_a, combine=t_c - t_b, epochs=t_d - t_c, layout=t_e - t_d)
+
+    for w in range(args.warmup):
+        log(f"warmup {w + 1}/{args.warmup}")
+        step()
+    torch.cuda.synchronize()
+    stage = {}
+    kernel_ms, launches = 0.0, 0
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        lay, st = step(timing=True)
+        for key, v in st.items():
+            stage[key] = stage.get(key, 0.0) + v
+        ms, nl = lay.timing()
+        kernel_ms += ms
+        launches += nl
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    ms_per_step = 1000.0 * wall / args.steps
+    log(f"{args.steps} steps in {wall:.3f}s ({ms_per_step:.1f} ms/step)")
+
+    tal = lay.tallies()
+    U = int(tal[:, 0].sum())
+    S = int(tal[:, 1].sum())
+    _, E, _, _ = lay.info()
+    value = U / (ms_per_step / 1000.0)
+
+    # roofline of the layout epoch kernel: algorithmic bytes per launch
+    D = 2
+    F = tal[:, 0].astype(np.float64)
+    Se = tal[:, 1].astype(np.float64)
+    bytes_per_epoch = 4.0 * E + F * 20.0 + F * 2 * 4 * D + F * 2 * 4 * D + Se * 4 * D
+    avg_bytes = float(bytes_per_epoch.mean())
+    avg_launch_s = (kernel_ms / 1000.0) / max(1, launches)
+    achieved = avg_bytes / avg_launch_s / 1e9
+    traffic = read_traffic(args.config)
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": (traffic or {}).get("bytes_per_launch"),
+                "kernel": "k_layout_rows<float,2>", "avg_launch_us": round(avg_launch_s * 1e6, 2),
+                "algorithmic_bytes_per_launch": round(avg_bytes)}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(x if False else None, kidx, kdist, n, k, T, nsr, a, b, args.cpu_seconds, U)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "edge-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"plain UMAP, {cfg['label']}", "n": n, "k": k, "epochs": T, "num_dim": 2,
+                   "stored_edges": int(E), "edge_updates": U, "negative_draws": S, "a": a, "b": b},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "stages_ms_per_step": {key: round(1000 * v / args.steps, 3) for key, v in stage.items()},
+        "layout_kernel_ms_per_step": round(kernel_ms / args.steps, 3),
+    }
+    if cpu and cpu.get("value"):
+        out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(_x, kidx, kdist, n, k, T, nsr, a, b, target_s, U_gpu):
+    """The reference CPU path (the oracle's restatement) on the host cores:
+    graph build on the full input, then the conflict-scheduled parallel
+    layout (optimize_layout_parallel) for as many epochs as fit ~target_s."""
+    import oracle
+
+    cores = min(os.cpu_count() or 1, 16)
+    log(f"cpu baseline on {cores} threads")
+    rp = np.arange(n + 1, dtype=np.int64) * k
+    idx = kidx.cpu().numpy().reshape(-1).copy()
+    dist = kdist.cpu().numpy().reshape(-1).copy()
+    t0 = time.perf_counter()
+    w, _, _ = oracle.neighbor_similarities(rp, dist, nthreads=cores)
+    t1 = time.perf_counter()
+    crp, cidx, cw = oracle.combine_neighbor_sets(rp, idx, w, 1.0)
+    t2 = time.perf_counter()
+    ep = oracle.similarities_to_epochs(crp, cidx, cw, T, nsr)
+    t3 = time.perf_counter()
+    graph_s = t3 - t0
+    log(f"cpu graph build {graph_s:.2f}s (similarities {t1 - t0:.2f}, union {t2 - t1:.2f}, epochs {t3 - t2:.2f})")
+    lay = oracle.Layout(ep, T, nsr, rng_kind=oracle.RNG_MT, seed=1234567890, dtype=np.float32)
+    emb = np.random.default_rng(7).uniform(-10, 10, size=(n, 2)).astype(np.float32)
+    done, spent = 0, 0.0
+    # epoch 0 never fires; run one epoch at a time until the sample budget is used
+    while done < T and spent < target_s:
+        ts = time.perf_counter()
+        lay.run(emb, a, b, epoch_limit=done + 1, nthreads=cores)
+        spent += time.perf_counter() - ts
+        done += 1
+    _, _, _, tal = lay.state()
+    u_sample = int(tal[:, 0].sum())
+    layout_rate = u_sample / spent if spent > 0 else 0.0
+    # whole-path estimate at the GPU's edge-update count
+    est_total_s = graph_s + (U_gpu / layout_rate if layout_rate else float("inf"))
+    return {"value": round(U_gpu / est_total_s, 1), "unit": "edge-updates/s", "cores": cores, "kind": "port",
+            "sample": f"oracle graph build on the full input ({graph_s:.2f}s) + parallel layout epochs 0..{done - 1} "
+                      f"({u_sample} edge-updates in {spent:.2f}s, {layout_rate:.3g}/s), "
+                      f"extrapolated to the GPU's {U_gpu} edge-updates",
+            "graph_s": round(graph_s, 3), "layout_rate": round(layout_rate, 1)}
+
+
+if __name__ == "__main__":
+    main()

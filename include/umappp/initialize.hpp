@@ -1,0 +1,80 @@
+// Drop-in for umappp/initialize.hpp (reference :27-119): similarities, fuzzy
+// union and schedule are built on the GPU; the initial coordinates, (a, b)
+// and the epoch count are host steps as in the reference.
+#ifndef UMAPPP_AMD_INITIALIZE_HPP
+#define UMAPPP_AMD_INITIALIZE_HPP
+
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <optional>
+#include <stdexcept>
+
+#include "NeighborList.hpp"
+#include "Options.hpp"
+#include "Status.hpp"
+#include "detail_csr.hpp"
+#include "find_ab.hpp"
+
+namespace umappp {
+
+// 500 epochs up to 10,000 observations, else 200 + ceil(300 * 10^4 / N)
+// (reference initialize.hpp:27-47).
+template <typename Index_>
+int choose_num_epochs(const std::optional<int> num_epochs, const Index_ size) {
+    if (num_epochs.has_value()) return *num_epochs;
+    constexpr double limit = 10000;
+    if (static_cast<double>(size) <= limit) return 500;
+    return 200 + static_cast<int>(std::ceil(300.0 * limit / static_cast<double>(size)));
+}
+
+// Uniform draws in [-scale, scale) from std::mt19937_64(seed): 53 random
+// bits per draw scaled to [0, 1).  Same role as the reference's random_init
+// (spectral_init.hpp:209-225); its aarand mapping is not reproduced.
+template <typename Index_, typename Float_>
+void random_init(const Index_ num_obs, const std::size_t num_dim, Float_* const vals,
+                 const RngEngine::result_type seed, const double scale) {
+    RngEngine rng(seed);
+    const Float_ mult = static_cast<Float_>(scale * 2);
+    const Float_ shift = static_cast<Float_>(scale);
+    const std::size_t total = num_dim * static_cast<std::size_t>(num_obs);
+    for (std::size_t i = 0; i < total; ++i) {
+        const double u = static_cast<double>(rng() >> 11) * 0x1.0p-53;
+        vals[i] = static_cast<Float_>(u) * mult - shift;
+    }
+}
+
+template <typename Index_, typename Float_>
+Status<Index_, Float_> initialize(NeighborList<Index_, Float_> x, const std::size_t num_dim, Float_* const embedding,
+                                  Options options) {
+    auto c = detail::to_csr(x);
+    const int64_t n = static_cast<int64_t>(x.size());
+    umx_graph* g = nullptr;
+    detail::check(umx_graph_create(detail::dtype_of<Float_>(), n, c.row_ptr.data(), c.idx.data(), c.val.data(), 0, &g));
+    std::unique_ptr<umx_graph, void (*)(umx_graph*)> graph(g, &umx_graph_destroy);
+    detail::check(umx_graph_neighbor_similarities(g, options.local_connectivity, options.bandwidth, 1e-3, 1));
+    detail::check(umx_graph_combine(g, options.mix_ratio));
+
+    bool use_random = options.initialize_method == InitializeMethod::RANDOM;
+    if (options.initialize_method == InitializeMethod::SPECTRAL) {
+        throw std::runtime_error("spectral initialisation is not available in this build; "
+                                 "use InitializeMethod::RANDOM or InitializeMethod::NONE");
+    }
+    if (use_random) random_init<Index_, Float_>(static_cast<Index_>(n), num_dim, embedding, options.initialize_seed,
+                                                options.initialize_random_scale);
+
+    if (!options.a.has_value() || !options.b.has_value()) {
+        const auto found = find_ab<double>(options.spread, options.min_dist);
+        options.a = found.first;
+        options.b = found.second;
+    }
+    options.num_epochs = choose_num_epochs<Index_>(options.num_epochs, static_cast<Index_>(n));
+
+    umx_layout* L = nullptr;
+    detail::check(umx_layout_create(g, *options.num_epochs, options.negative_sample_rate, &L));
+    return Status<Index_, Float_>(L, std::move(options), num_dim);
+}
+
+}  // namespace umappp
+
+#endif

@@ -1,0 +1,165 @@
+/*
+ * umappp_amd.h — C ABI of the MI355X (gfx950) implementation of umappp's
+ * graph-and-layout hot path.
+ *
+ * The reference (libscran/umappp 3.3.2) is a header-only C++ template library
+ * with no ABI of its own; each entry point below replaces one reference
+ * function or method, cited as file:line under include/umappp/ of the
+ * reference.  The C++ drop-in headers under include/umappp/ of this
+ * repository (namespace umappp: Options, initialize(), Status) are written on
+ * top of this ABI.
+ *
+ * Conventions
+ *  - Every function returns UMX_OK (0) or a negative umx_status; the message
+ *    of the last failure on the calling thread is umx_last_error().
+ *  - dtype selects the caller's Float_: UMX_F32 (float) or UMX_F64 (double).
+ *    Neighbour indices are int32 (the reference's Index_ = int); row offsets
+ *    are int64 (the reference's size_t cumulative_num_edges).
+ *  - Pointers are host pointers unless an argument named 'device_ptrs' (or a
+ *    function name ending in _device) says they live in GPU memory.
+ *  - A handle owns its GPU memory and one HIP stream; handles are not
+ *    thread-safe, separate handles are independent.  Calls block until their
+ *    results are in the caller's memory, like the reference's calls.
+ */
+#ifndef UMAPPP_AMD_H
+#define UMAPPP_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum umx_dtype { UMX_F32 = 0, UMX_F64 = 1 };
+
+enum umx_status {
+    UMX_OK = 0,
+    UMX_ERR_ARGUMENT = -1,  /* bad size, option or pointer */
+    UMX_ERR_GRAPH = -2,     /* neighbour index out of range, self or duplicate */
+    UMX_ERR_DEVICE = -3,    /* a HIP call failed */
+    UMX_ERR_STATE = -4,     /* call not valid in the handle's current state */
+    UMX_ERR_OVERFLOW = -5,  /* a size or count does not fit its type */
+    UMX_ERR_CAPACITY = -6   /* caller's output buffer too small */
+};
+
+/* Message for the last non-zero status returned on this thread. */
+const char* umx_last_error(void);
+/* ABI version (major * 10000 + minor * 100 + patch). */
+int umx_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Fuzzy similarities                                                  */
+/* ------------------------------------------------------------------ */
+
+/*
+ * neighbor_similarities — replaces umappp::neighbor_similarities
+ * (neighbor_similarities.hpp:46-207) with NeighborSimilaritiesOptions
+ * (:38-44).  The kNN lists are passed as CSR: row i holds entries
+ * [row_ptr[i], row_ptr[i+1]) of 'val', sorted by increasing distance.  'val'
+ * holds distances on entry and fuzzy weights on return (in place, as in the
+ * reference).  The per-row solve runs in double precision whatever 'dtype' is.
+ * use_newton = 0 selects the bisection-only variant (template argument
+ * use_newton_ = false).  rho_out / sigma_out (double[n], may be NULL) receive
+ * each row's rho and final sigma; NaN where the reference sets every weight to
+ * 1 before computing them.  Host pointers.
+ */
+int umx_neighbor_similarities(int dtype, int64_t n, const int64_t* row_ptr, void* val,
+                              double local_connectivity, double bandwidth, double min_k_dist_scale,
+                              int use_newton, double* rho_out, double* sigma_out);
+
+/*
+ * combine_neighbor_sets — replaces umappp::combine_neighbor_sets
+ * (combine_neighbor_sets.hpp:15-100).  Input is the directed weighted graph
+ * (row_ptr, idx, val) with unique, non-self indices per row; output is the
+ * symmetrised graph with every row sorted by index, written to out_row_ptr
+ * (int64[n+1]), out_idx and out_val (capacity 'cap' entries; 2 * nnz always
+ * suffices).  *out_nnz receives the output entry count (also on
+ * UMX_ERR_CAPACITY).  Host pointers.
+ */
+int umx_combine_neighbor_sets(int dtype, int64_t n, const int64_t* row_ptr, const int32_t* idx,
+                              const void* val, double mix_ratio, int64_t cap, int64_t* out_row_ptr,
+                              int32_t* out_idx, void* out_val, int64_t* out_nnz);
+
+/* ------------------------------------------------------------------ */
+/* Graph handle: the kNN graph resident on the GPU                     */
+/* ------------------------------------------------------------------ */
+
+typedef struct umx_graph umx_graph;
+
+/* Copies a CSR kNN graph (distances in 'val') to the GPU.  Replaces the
+ * NeighborList argument of initialize() (initialize.hpp:69-70).
+ * device_ptrs = 1 if row_ptr/idx/val are GPU pointers. */
+int umx_graph_create(int dtype, int64_t n, const int64_t* row_ptr, const int32_t* idx, const void* val,
+                     int device_ptrs, umx_graph** out);
+/* neighbor_similarities on the resident graph (initialize.hpp:71-75). */
+int umx_graph_neighbor_similarities(umx_graph* g, double local_connectivity, double bandwidth,
+                                    double min_k_dist_scale, int use_newton);
+/* combine_neighbor_sets on the resident graph (initialize.hpp:77). */
+int umx_graph_combine(umx_graph* g, double mix_ratio);
+int umx_graph_info(const umx_graph* g, int64_t* n, int64_t* nnz, int* dtype);
+/* Copies the current graph to host memory (row_ptr int64[n+1], idx, val). */
+int umx_graph_export(const umx_graph* g, int64_t* row_ptr, int32_t* idx, void* val);
+/* rho and sigma of the last neighbor_similarities call (double[n] each). */
+int umx_graph_export_rho_sigma(const umx_graph* g, double* rho, double* sigma);
+void umx_graph_destroy(umx_graph* g);
+
+/* ------------------------------------------------------------------ */
+/* Layout handle: EpochData + SGD epochs                               */
+/* ------------------------------------------------------------------ */
+
+typedef struct umx_layout umx_layout;
+
+/* similarities_to_epochs (optimize_layout.hpp:47-92) on a resident graph. */
+int umx_layout_create(const umx_graph* g, int num_epochs, double negative_sample_rate, umx_layout** out);
+/* Same from a symmetric weighted CSR (host or GPU pointers). */
+int umx_layout_create_csr(int dtype, int64_t n, const int64_t* row_ptr, const int32_t* idx, const void* val,
+                          int device_ptrs, int num_epochs, double negative_sample_rate, umx_layout** out);
+/* Sizes and progress: Status::num_observations / num_epochs / epoch
+ * (Status.hpp:63-87). */
+int umx_layout_info(const umx_layout* L, int64_t* n, int64_t* nnz, int* total_epochs, int* current_epoch);
+/* EpochData export (optimize_layout.hpp:27-45): cumulative_num_edges,
+ * edge_targets, epochs_per_sample, epoch_of_next_sample,
+ * epoch_of_next_negative_sample.  Any pointer may be NULL. */
+int umx_layout_export_epochs(const umx_layout* L, int64_t* row_ptr, int32_t* targets, void* epochs_per_sample,
+                             void* epoch_of_next_sample, void* epoch_of_next_negative_sample);
+/* Optimisation parameters (Status.hpp:103-113 passes these to
+ * optimize_layout): embedding dimension, a, b, repulsion_strength (gamma),
+ * learning_rate (initial alpha) and the negative-sampling seed. */
+int umx_layout_set_params(umx_layout* L, int num_dim, double a, double b, double gamma, double learning_rate,
+                          uint64_t seed);
+/* Embedding in / out ("column-major num_dim x n", i.e. emb[i*num_dim + d],
+ * Status.hpp:93).  device_ptr = 1 for a GPU pointer. */
+int umx_layout_set_embedding(umx_layout* L, const void* embedding, int device_ptr);
+int umx_layout_get_embedding(const umx_layout* L, void* embedding, int device_ptr);
+/* Runs epochs [epoch(), epoch_limit) on the resident embedding with the
+ * parallel (Hogwild) kernel — replaces optimize_layout /
+ * optimize_layout_parallel as dispatched by Status::run (Status.hpp:101-128). */
+int umx_layout_run(umx_layout* L, int epoch_limit);
+/* Deterministic one-thread replay of optimize_layout (optimize_layout.hpp:
+ * 116-185) in the reference's edge order and mixed precision.  If 'samples' is
+ * non-NULL, negative samples are taken from it in draw order (count entries);
+ * otherwise from the counter sampler.  For parity tests. */
+int umx_layout_run_serial(umx_layout* L, int epoch_limit, const int32_t* samples, int64_t count);
+/* Per-epoch tallies since creation: out[3*e + {0,1,2}] = firing edges, negative
+ * draws, self-hits of epoch e; cap = number of int64 slots. */
+int umx_layout_tallies(const umx_layout* L, int64_t* out, int64_t cap);
+/* HIP-event timing of the epoch kernels (on the handle's stream). */
+int umx_layout_set_timing(umx_layout* L, int on);
+int umx_layout_timing(const umx_layout* L, double* kernel_ms, int64_t* launches);
+void umx_layout_destroy(umx_layout* L);
+
+/* ------------------------------------------------------------------ */
+/* Host helpers (no GPU work)                                          */
+/* ------------------------------------------------------------------ */
+
+/* find_ab (find_ab.hpp:44-168): a and b of 1 / (1 + a x^(2b)). */
+int umx_find_ab(double spread, double min_dist, double* a, double* b);
+/* choose_num_epochs (initialize.hpp:27-47); num_epochs < 0 means unset. */
+int umx_choose_num_epochs(int num_epochs, int64_t size);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UMAPPP_AMD_H */

@@ -1,0 +1,226 @@
+"""ORACLE — TEST INFRASTRUCTURE ONLY.
+
+ctypes wrappers over ``oracle/liboracle_umappp.so``, the CPU restatement of
+umappp's graph-and-layout hot path (``oracle/umappp_oracle.cpp``).  Only
+``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` import this module; the product never does.
+
+Parity pins (see tests/test_oracle_*.py):
+  neighbor_similarities   reference invariants (neighbor_similarities.cpp:52-228) + root re-solve
+  combine_neighbor_sets   brute-force symmetrisation (combine_neighbor_sets.cpp:47-112)
+  similarities_to_epochs  closed form, Epochs test (optimize_layout.cpp:58-77)
+  Layout (serial)         finite differences of the UMAP losses, closed-form schedule counts,
+                          restart equality (optimize_layout.cpp:89-104)
+  Layout (parallel)       bit-equality with serial (optimize_layout.cpp:106-125)
+  rng kind 0 (mt19937_64 + rejection sampler)  parity unpinned against aarand
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle_umappp.so"
+
+RNG_MT, RNG_COUNTER, RNG_REPLAY = 0, 1, 2
+
+_lib = None
+_vp = C.c_void_p
+
+
+def build() -> Path:
+    subprocess.run(["bash", str(HERE / "build.sh")], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        sig = {
+            "orc_neighbor_similarities": (C.c_int, [C.c_int, C.c_int64, _vp, _vp, C.c_double, C.c_double, C.c_double,
+                                                    C.c_int, _vp, _vp, C.c_int]),
+            "orc_combine_neighbor_sets": (C.c_int64, [C.c_int, C.c_int64, _vp, _vp, _vp, C.c_double, C.c_int64, _vp,
+                                                      _vp, _vp]),
+            "orc_similarities_to_epochs": (C.c_int64, [C.c_int, C.c_int64, _vp, _vp, _vp, C.c_int, C.c_double, _vp,
+                                                       _vp, _vp, _vp, _vp, _vp]),
+            "orc_layout_create": (_vp, [C.c_int, C.c_int64, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int,
+                                        C.c_int, C.c_uint64]),
+            "orc_layout_destroy": (None, [_vp]),
+            "orc_layout_set_replay": (C.c_int, [_vp, _vp, C.c_int64]),
+            "orc_layout_set_record": (C.c_int, [_vp, _vp, C.c_int64]),
+            "orc_layout_recorded": (C.c_int64, [_vp]),
+            "orc_layout_run": (C.c_int, [_vp, C.c_int, _vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                         C.c_int]),
+            "orc_layout_state": (C.c_int, [_vp, _vp, _vp, C.POINTER(C.c_int), _vp, C.c_int64]),
+            "orc_counter_draw": (C.c_uint32, [C.c_uint64, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32]),
+            "orc_mt19937_64_nth": (C.c_uint64, [C.c_uint64, C.c_int64]),
+            "orc_discrete_uniform_stream": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, _vp]),
+            "orc_gen_normal": (None, [C.c_uint64, C.c_int64, _vp]),
+            "orc_gen_uniform01": (None, [C.c_uint64, C.c_int64, _vp]),
+            "orc_knn_bruteforce": (None, [C.c_int64, C.c_int, _vp, C.c_int, C.c_int, _vp, _vp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_vp)
+
+
+def _dt(dtype) -> int:
+    return 0 if np.dtype(dtype) == np.float32 else 1
+
+
+def neighbor_similarities(row_ptr, val, local_connectivity=1.0, bandwidth=1.0, min_k_dist_scale=1e-3,
+                          use_newton=True, nthreads=1):
+    """Returns (weights, rho, sigma); 'val' is not modified."""
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int64)
+    w = np.array(val, copy=True)
+    n = rp.size - 1
+    rho = np.empty(n)
+    sigma = np.empty(n)
+    rc = lib().orc_neighbor_similarities(_dt(w.dtype), n, _p(rp), _p(w), local_connectivity, bandwidth,
+                                         min_k_dist_scale, 1 if use_newton else 0, _p(rho), _p(sigma), nthreads)
+    if rc != 0:
+        raise ValueError(f"oracle similarities failed ({rc})")
+    return w, rho, sigma
+
+
+def combine_neighbor_sets(row_ptr, idx, val, mix_ratio=1.0):
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int64)
+    ix = np.ascontiguousarray(idx, dtype=np.int32)
+    v = np.ascontiguousarray(val)
+    n = rp.size - 1
+    cap = max(1, 2 * int(rp[-1]))
+    orp = np.empty(n + 1, dtype=np.int64)
+    oidx = np.empty(cap, dtype=np.int32)
+    oval = np.empty(cap, dtype=v.dtype)
+    m = lib().orc_combine_neighbor_sets(_dt(v.dtype), n, _p(rp), _p(ix), _p(v), float(mix_ratio), cap, _p(orp),
+                                        _p(oidx), _p(oval))
+    if m < 0:
+        raise ValueError(f"oracle combine failed ({m})")
+    return orp, oidx[:m].copy(), oval[:m].copy()
+
+
+def similarities_to_epochs(row_ptr, idx, val, num_epochs, negative_sample_rate):
+    """Returns dict with row_ptr, targets, eps, next, next_neg, maxed."""
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int64)
+    ix = np.ascontiguousarray(idx, dtype=np.int32)
+    v = np.ascontiguousarray(val)
+    n = rp.size - 1
+    E = int(rp[-1])
+    orp = np.empty(n + 1, dtype=np.int64)
+    tg = np.empty(max(E, 1), dtype=np.int32)
+    eps = np.empty(max(E, 1), dtype=v.dtype)
+    nx = np.empty(max(E, 1), dtype=v.dtype)
+    nn = np.empty(max(E, 1), dtype=v.dtype)
+    mx = np.zeros(1, dtype=v.dtype)
+    m = lib().orc_similarities_to_epochs(_dt(v.dtype), n, _p(rp), _p(ix), _p(v), int(num_epochs),
+                                         float(negative_sample_rate), _p(orp), _p(tg), _p(eps), _p(nx), _p(nn), _p(mx))
+    if m < 0:
+        raise ValueError(f"oracle similarities_to_epochs failed ({m})")
+    return dict(row_ptr=orp, targets=tg[:m].copy(), eps=eps[:m].copy(), next=nx[:m].copy(), next_neg=nn[:m].copy(),
+                maxed=mx[0])
+
+
+class Layout:
+    """EpochData + sampler state (the reference's Status minus Options)."""
+
+    def __init__(self, epochs: dict, num_epochs: int, negative_sample_rate: float, rng_kind=RNG_MT,
+                 seed=1234567890, dtype=None, current_epoch=0):
+        dt = np.dtype(dtype) if dtype is not None else epochs["eps"].dtype
+        self.dtype = dt
+        rp = np.ascontiguousarray(epochs["row_ptr"], dtype=np.int64)
+        tg = np.ascontiguousarray(epochs["targets"], dtype=np.int32)
+        eps = np.ascontiguousarray(epochs["eps"], dtype=dt)
+        nx = np.ascontiguousarray(epochs.get("next", eps), dtype=dt)
+        nn = np.ascontiguousarray(epochs.get("next_neg", eps / dt.type(negative_sample_rate)), dtype=dt)
+        self.n = rp.size - 1
+        self.E = int(rp[-1])
+        self.total_epochs = int(num_epochs)
+        self._keep = (rp, tg, eps, nx, nn)
+        self._h = lib().orc_layout_create(_dt(dt), self.n, _p(rp), _p(tg), _p(eps), _p(nx), _p(nn),
+                                          float(negative_sample_rate), int(num_epochs), int(current_epoch),
+                                          int(rng_kind), C.c_uint64(int(seed) & (2**64 - 1)))
+        self._rec = None
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_layout_destroy(self._h)
+            self._h = None
+
+    def set_replay(self, samples):
+        s = np.ascontiguousarray(samples, dtype=np.int32)
+        self._replay = s
+        lib().orc_layout_set_replay(self._h, _p(s), s.size)
+
+    def record(self, cap):
+        self._rec = np.zeros(int(cap), dtype=np.int32)
+        lib().orc_layout_set_record(self._h, _p(self._rec), self._rec.size)
+
+    def recorded(self):
+        m = lib().orc_layout_recorded(self._h)
+        return self._rec[: min(m, self._rec.size)].copy()
+
+    def run(self, embedding: np.ndarray, a, b, gamma=1.0, alpha0=1.0, epoch_limit=None, nthreads=1):
+        if embedding.dtype != self.dtype or not embedding.flags.c_contiguous:
+            raise TypeError("embedding must be C-contiguous of the layout's dtype")
+        D = embedding.shape[1] if embedding.ndim == 2 else 1
+        lim = self.total_epochs if epoch_limit is None else int(epoch_limit)
+        rc = lib().orc_layout_run(self._h, D, _p(embedding), float(a), float(b), float(gamma), float(alpha0), lim,
+                                  int(nthreads))
+        if rc != 0:
+            raise ValueError(f"oracle layout run failed ({rc})")
+
+    def state(self):
+        nx = np.empty(max(self.E, 1), dtype=self.dtype)
+        nn = np.empty(max(self.E, 1), dtype=self.dtype)
+        cur = C.c_int()
+        tal = np.zeros(3 * self.total_epochs, dtype=np.int64)
+        lib().orc_layout_state(self._h, _p(nx), _p(nn), C.byref(cur), _p(tal), tal.size)
+        return nx[: self.E].copy(), nn[: self.E].copy(), cur.value, tal.reshape(-1, 3)[: cur.value].copy()
+
+
+def counter_draw(seed, epoch, edge, p, n):
+    return int(lib().orc_counter_draw(C.c_uint64(seed), int(epoch), C.c_uint64(edge), int(p), int(n)))
+
+
+def mt19937_64_nth(seed, nth):
+    return int(lib().orc_mt19937_64_nth(C.c_uint64(seed), int(nth)))
+
+
+def discrete_uniform_stream(seed, n, count):
+    out = np.empty(count, dtype=np.int64)
+    lib().orc_discrete_uniform_stream(C.c_uint64(seed), C.c_uint64(n), count, _p(out))
+    return out
+
+
+def gen_normal(seed, count):
+    out = np.empty(int(count))
+    lib().orc_gen_normal(C.c_uint64(int(seed) & (2**64 - 1)), out.size, _p(out))
+    return out
+
+
+def gen_uniform01(seed, count):
+    out = np.empty(int(count))
+    lib().orc_gen_uniform01(C.c_uint64(int(seed) & (2**64 - 1)), out.size, _p(out))
+    return out
+
+
+def knn_bruteforce(x: np.ndarray, k: int, nthreads=8):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n, dim = x.shape
+    idx = np.empty((n, k), dtype=np.int32)
+    dist = np.empty((n, k))
+    lib().orc_knn_bruteforce(n, dim, _p(x), k, nthreads, _p(idx), _p(dist))
+    return idx, dist

@@ -1,0 +1,235 @@
+"""Pins the oracle's layout epochs (CPU only).
+
+* per-edge updates against central finite differences of the UMAP losses
+  (attractive log(1 + a s^b); repulsive -gamma log(1 - 1/(1 + a s^b))), which
+  fixes the sign, the factor 2, the exponent b - 1 and the learning-rate decay;
+* the gradient clip at 4;
+* the schedule against closed forms on dyadic epochs_per_sample values (exact
+  in binary floating point): firing epochs ceil(m * eps), and a total of
+  floor((n_last - eps/nsr) / (eps/nsr)) negative draws per edge;
+* restart equality and serial == parallel (optimize_layout.cpp:89-125);
+* the counter sampler against an independent Python restatement of its
+  definition, and mt19937_64 against the C++ standard's known answer.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle
+from _data import layout_fixture
+
+M64 = (1 << 64) - 1
+
+
+def _mix64(z):
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+    return z ^ (z >> 31)
+
+
+def counter_draw_py(seed, epoch, edge, p, n):
+    ek = _mix64((seed + 0x9E3779B97F4A7C15 * (epoch + 1)) & M64)
+    ej = _mix64(ek ^ ((edge * 0xD1B54A32D192ED03) & M64))
+    r = _mix64((ej + 0x9E3779B97F4A7C15 * (p + 1)) & M64)
+    return (r * n) >> 64
+
+
+def one_edge_layout(eps_val, n_points, extra_rows=0, dtype=np.float64, nsr=5.0, total=10):
+    # row 0 holds a single edge 0 -> 1; other rows are empty
+    rp = np.zeros(n_points + 1, dtype=np.int64)
+    rp[1:] = 1
+    ep = dict(row_ptr=rp, targets=np.array([1], np.int32), eps=np.array([eps_val], dtype))
+    return oracle.Layout(ep, total, nsr, rng_kind=oracle.RNG_REPLAY, dtype=dtype)
+
+
+def fd_grad(f, y, h=1e-6):
+    g = np.zeros_like(y)
+    for d in range(y.size):
+        yp, ym = y.copy(), y.copy()
+        yp[d] += h
+        ym[d] -= h
+        g[d] = (f(yp) - f(ym)) / (2 * h)
+    return g
+
+
+def test_attractive_step_matches_loss_gradient():
+    a, b, T = 1.577, 0.8951, 10
+    y = np.array([[0.3, -0.2, 0.1], [1.1, 0.4, -0.5]], dtype=np.float64)
+    lay = one_edge_layout(1.0, 2, total=T)
+    lay.set_replay(np.zeros(64, np.int32))  # every negative sample is the head itself: skipped
+    emb = y.copy()
+    lay.run(emb, a, b, epoch_limit=2)  # epoch 0 never fires (eps >= 1); epoch 1 fires once
+    alpha = 1.0 * (1.0 - 1.0 / T)
+
+    def loss(y0):
+        s = np.sum((y0 - y[1]) ** 2)
+        return np.log1p(a * s ** b)
+
+    g = fd_grad(loss, y[0].copy())
+    np.testing.assert_allclose(emb[0] - y[0], -alpha * g, rtol=1e-6)
+    np.testing.assert_allclose(emb[1] - y[1], alpha * g, rtol=1e-6)
+
+
+def test_repulsive_step_matches_loss_gradient():
+    a, b, gamma, T = 1.2, 0.9, 0.7, 10
+    y = np.array([[0.0, 0.0], [0.2, 0.1], [4.0, 3.0]], dtype=np.float64)  # s_rep = 25 >> 0.001
+    lay = one_edge_layout(1.0, 3, total=T, nsr=2.0)  # (1 - 0.5) / 0.5 = one draw at epoch 1
+    lay.set_replay(np.array([2], np.int32))
+    emb = y.copy()
+    lay.run(emb, a, b, gamma=gamma, epoch_limit=2)
+    nx, nn, cur, tal = lay.state()
+    assert tal[1].tolist() == [1, 1, 0]
+    alpha = 1.0 - 1.0 / T
+
+    def attr(y0):
+        return np.log1p(a * np.sum((y0 - y[1]) ** 2) ** b)
+
+    y0_after_attr = y[0] - alpha * fd_grad(attr, y[0].copy())
+
+    def rep(y0):
+        s = np.sum((y0 - y[2]) ** 2)
+        return -gamma * np.log(1.0 - 1.0 / (1.0 + a * s ** b))
+
+    expect = y0_after_attr - alpha * fd_grad(rep, y0_after_attr.copy())
+    # the reference regularises the repulsive denominator with 0.001 + s
+    np.testing.assert_allclose(emb[0], expect, rtol=1e-4, atol=1e-7)
+
+
+def test_gradient_clip_at_four():
+    a, b = 1.0, 1.0
+    y = np.array([[0.0, 0.0], [0.01, 0.0]], dtype=np.float64)  # huge coefficient at tiny distance
+    lay = one_edge_layout(1.0, 2, total=4)
+    lay.set_replay(np.zeros(64, np.int32))
+    emb = y.copy()
+    lay.run(emb, a, b, epoch_limit=2)
+    alpha = 1.0 - 1.0 / 4
+    # unclipped step would be 2ab/(1+a s^b) * 0.01 ~ 0.02 < 4: not clipped
+    assert abs(emb[0, 0] - y[0, 0]) < 4 * alpha
+    y2 = np.array([[0.0, 0.0], [1e-3, 0.0]], dtype=np.float64)
+    lay2 = one_edge_layout(1.0, 2, total=4, nsr=2.0)
+    lay2.set_replay(np.array([1], np.int32))  # repulsion from the tail itself at s = tiny
+    emb2 = y2.copy()
+    lay2.run(emb2, 1.0, 1.0, epoch_limit=2)
+    # repulsive coefficient 2b/((0.001+s)(a s + 1)) * dx: with dx ~ -1e-3 after attraction -> clipped?
+    step = emb2[0] - y2[0]
+    assert np.all(np.abs(step) <= 2 * 4 * alpha + 1e-12)
+    y3 = np.array([[0.0, 0.0], [0.0, 1e-9]], dtype=np.float64)
+    lay3 = one_edge_layout(1.0, 2, total=4)
+    lay3.set_replay(np.zeros(64, np.int32))
+    emb3 = y3.copy()
+    lay3.run(emb3, 1.0, 0.5, epoch_limit=2)  # b = 0.5: coefficient ~ s^-0.5 * 1e-9 ... bounded
+    assert np.all(np.isfinite(emb3))
+
+
+def test_clip_value_exact():
+    # Repulsion at distance d with gamma large: coefficient * delta >> 4 -> step = alpha * 4 exactly.
+    y = np.array([[0.0, 0.0], [5.0, 0.0], [0.5, 0.0]], dtype=np.float64)
+    lay = one_edge_layout(1.0, 3, total=4, nsr=2.0)
+    lay.set_replay(np.array([2], np.int32))
+    emb = y.copy()
+    lay.run(emb, 1.0, 1.0, gamma=1000.0, epoch_limit=2)
+    alpha = 1.0 - 1.0 / 4
+    a_, b_ = 1.0, 1.0
+    s = 25.0
+    gc = (-2 * a_ * b_ * s ** b_) / (s * (a_ * s ** b_ + 1.0))
+    x_after = 0.0 + alpha * gc * (0.0 - 5.0)  # attraction, unclipped (|gc*dx| < 4)
+    assert abs(gc * 5.0) < 4
+    assert emb[0, 0] == pytest.approx(x_after - alpha * 4.0, rel=1e-12)
+
+
+@pytest.mark.parametrize("eps_val,nsr,T", [(1.0, 4.0, 20), (1.5, 4.0, 25), (2.5, 2.0, 30), (4.0, 8.0, 33),
+                                           (3.25, 4.0, 50)])
+def test_schedule_closed_form(eps_val, nsr, T):
+    lay = one_edge_layout(eps_val, 2, total=T, nsr=nsr)
+    lay.set_replay(np.zeros(100000, np.int32))
+    emb = np.zeros((2, 2))
+    emb[1, 0] = 1.0
+    lay.run(emb, 1.0, 1.0, epoch_limit=T)
+    nx, nn, cur, tal = lay.state()
+    fires_at = [n for n in range(T) if tal[n, 0]]
+    m = 1
+    expect = []
+    while np.ceil(m * eps_val) < T:
+        expect.append(int(np.ceil(m * eps_val)))
+        m += 1
+    assert fires_at == expect
+    epns = eps_val / nsr
+    n_last = expect[-1]
+    assert int(tal[:, 1].sum()) == int(np.floor((n_last - epns) / epns))
+    assert nx[0] == (len(expect) + 1) * eps_val
+
+
+@pytest.mark.parametrize("nobs,k", [(50, 5), (100, 10), (200, 15)])
+@pytest.mark.parametrize("rng_kind", [oracle.RNG_MT, oracle.RNG_COUNTER])
+def test_restart_equals_single_run(nobs, k, rng_kind):
+    x, rp, idx, w = layout_fixture(nobs, k)
+    ep = oracle.similarities_to_epochs(rp, idx, w, 500, 5.0)
+    e1 = x.copy()
+    l1 = oracle.Layout(ep, 500, 5.0, rng_kind=rng_kind, seed=10)
+    l1.run(e1, 2.0, 1.0, epoch_limit=100)
+    l1.run(e1, 2.0, 1.0)
+    e2 = x.copy()
+    l2 = oracle.Layout(ep, 500, 5.0, rng_kind=rng_kind, seed=10)
+    l2.run(e2, 2.0, 1.0)
+    assert np.array_equal(e1, e2)
+    assert not np.array_equal(e1, x)
+
+
+@pytest.mark.parametrize("nobs,k", list(itertools.product([50, 100, 200], [5, 10, 15])))
+def test_parallel_equals_serial(nobs, k):
+    x, rp, idx, w = layout_fixture(nobs, k)
+    ep = oracle.similarities_to_epochs(rp, idx, w, 500, 5.0)
+    e1 = x.copy()
+    oracle.Layout(ep, 500, 5.0, rng_kind=oracle.RNG_MT, seed=100).run(e1, 2.0, 1.0)
+    e2 = x.copy()
+    oracle.Layout(ep, 500, 5.0, rng_kind=oracle.RNG_MT, seed=100).run(e2, 2.0, 1.0, nthreads=3)
+    assert not np.array_equal(x, e1)
+    assert np.array_equal(e1, e2)
+
+
+def test_parallel_equals_serial_float_many_threads():
+    x, rp, idx, w = layout_fixture(200, 15)
+    ep = oracle.similarities_to_epochs(rp, idx, w.astype(np.float32), 200, 5.0)
+    x = x[:, :2].astype(np.float32).copy()
+    e1 = x.copy()
+    oracle.Layout(ep, 200, 5.0, rng_kind=oracle.RNG_COUNTER, seed=7).run(e1, 1.6, 0.9)
+    e2 = x.copy()
+    oracle.Layout(ep, 200, 5.0, rng_kind=oracle.RNG_COUNTER, seed=7).run(e2, 1.6, 0.9, nthreads=8)
+    assert np.array_equal(e1, e2)
+
+
+def test_counter_sampler_definition():
+    for seed, epoch, edge, p, n in [(0, 0, 0, 0, 10), (1234567890, 7, 123456, 3, 1000000), (2**63 + 5, 199, 2**33, 9, 7),
+                                    (42, 1, 1, 0, 2**31 - 1)]:
+        assert oracle.counter_draw(seed, epoch, edge, p, n) == counter_draw_py(seed, epoch, edge, p, n)
+    draws = np.array([oracle.counter_draw(99, 3, e, p, 17) for e in range(400) for p in range(5)])
+    counts = np.bincount(draws, minlength=17)
+    chi2 = ((counts - draws.size / 17) ** 2 / (draws.size / 17)).sum()
+    assert chi2 < 45  # 16 dof, p ~ 2e-4
+
+
+def test_mt19937_64_known_answer():
+    # The C++ standard requires the 10000th output of a default-constructed
+    # mt19937_64 (seed 5489) to be 9981545732273789042.
+    assert oracle.mt19937_64_nth(5489, 10000) == 9981545732273789042
+
+
+def test_discrete_uniform_range_and_uniformity():
+    s = oracle.discrete_uniform_stream(10, 13, 26000)
+    assert s.min() >= 0 and s.max() < 13
+    counts = np.bincount(s, minlength=13)
+    chi2 = ((counts - 2000) ** 2 / 2000).sum()
+    assert chi2 < 40
+
+
+def test_tallies_count_self_hits():
+    x, rp, idx, w = layout_fixture(50, 5)
+    ep = oracle.similarities_to_epochs(rp, idx, w, 50, 5.0)
+    lay = oracle.Layout(ep, 50, 5.0, rng_kind=oracle.RNG_COUNTER, seed=3)
+    e = x.copy()
+    lay.run(e, 1.5, 0.9)
+    _, _, cur, tal = lay.state()
+    assert cur == 50 and tal[0].tolist() == [0, 0, 0]
+    assert tal[:, 2].sum() > 0  # 50 points, ~1/50 of draws hit the head
+    assert tal[:, 1].sum() > 5 * tal[:, 2].sum()

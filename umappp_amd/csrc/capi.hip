@@ -1,0 +1,581 @@
+// C ABI of libumappp_amd (declared in include/umappp_amd.h).  Handles own
+// their device buffers and one HIP stream; every call converts exceptions to
+// a umx_status plus a thread-local message.
+#include "umappp_amd.h"
+#include "umx_internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+using umx::DevBuf;
+using umx::Error;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+int guard(F&& f) {
+    try {
+        f();
+        return UMX_OK;
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return UMX_ERR_DEVICE;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return UMX_ERR_DEVICE;
+    }
+}
+
+void require(bool ok, int code, const char* msg) {
+    if (!ok) throw Error(code, msg);
+}
+
+hipStream_t make_stream() {
+    hipStream_t s = nullptr;
+    UMX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+}
+
+__global__ void k_check_rowptr(const int64_t* rp, int64_t n, int64_t nnz, int* bad) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i == 0 && rp[0] != 0) atomicOr(bad, 1);
+    if (i < n && rp[i + 1] < rp[i]) atomicOr(bad, 1);
+    if (i == n && rp[n] != nnz) atomicOr(bad, 1);
+}
+
+void check_rowptr_device(const int64_t* rp, int64_t n, int64_t nnz, hipStream_t s) {
+    int* bad = nullptr;
+    UMX_HIP(hipMallocAsync(reinterpret_cast<void**>(&bad), sizeof(int), s));
+    UMX_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
+    const int64_t blocks = (n + 1 + 255) / 256;
+    k_check_rowptr<<<static_cast<unsigned>(blocks), 256, 0, s>>>(rp, n, nnz, bad);
+    int h = 0;
+    UMX_HIP(hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    UMX_HIP(hipFreeAsync(bad, s));
+    UMX_HIP(hipStreamSynchronize(s));
+    require(h == 0, UMX_ERR_ARGUMENT, "row_ptr must start at 0, be non-decreasing and end at nnz");
+}
+
+size_t dsize(int dtype) { return dtype == UMX_F32 ? sizeof(float) : sizeof(double); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Graph handle
+// ---------------------------------------------------------------------------
+struct umx_graph {
+    int dtype = UMX_F32;
+    int64_t n = 0, nnz = 0, max_degree = 0;
+    hipStream_t s = nullptr;
+    DevBuf<int64_t> rp;
+    DevBuf<int32_t> idx;
+    DevBuf<float> vf;
+    DevBuf<double> vd;
+    DevBuf<double> rho, sigma;
+    bool have_rho_sigma = false;
+
+    void* val() { return dtype == UMX_F32 ? static_cast<void*>(vf.p) : static_cast<void*>(vd.p); }
+    const void* val() const { return dtype == UMX_F32 ? static_cast<const void*>(vf.p) : static_cast<const void*>(vd.p); }
+    ~umx_graph() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+namespace {
+
+void graph_upload(umx_graph* g, const int64_t* row_ptr, const int32_t* idx, const void* val, int device_ptrs) {
+    const int64_t n = g->n;
+    int64_t nnz = 0;
+    if (device_ptrs) {
+        UMX_HIP(hipMemcpyAsync(&nnz, row_ptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, g->s));
+        UMX_HIP(hipStreamSynchronize(g->s));
+    } else {
+        require(row_ptr[0] == 0, UMX_ERR_ARGUMENT, "row_ptr[0] must be 0");
+        for (int64_t i = 0; i < n; ++i) require(row_ptr[i + 1] >= row_ptr[i], UMX_ERR_ARGUMENT, "row_ptr must be non-decreasing");
+        nnz = row_ptr[n];
+    }
+    require(nnz >= 0, UMX_ERR_ARGUMENT, "negative entry count");
+    g->nnz = nnz;
+    g->rp.alloc(static_cast<size_t>(n + 1));
+    g->idx.alloc(static_cast<size_t>(nnz));
+    if (g->dtype == UMX_F32) g->vf.alloc(static_cast<size_t>(nnz));
+    else g->vd.alloc(static_cast<size_t>(nnz));
+    const hipMemcpyKind kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    UMX_HIP(hipMemcpyAsync(g->rp.p, row_ptr, sizeof(int64_t) * static_cast<size_t>(n + 1), kind, g->s));
+    if (nnz > 0) {
+        if (idx) UMX_HIP(hipMemcpyAsync(g->idx.p, idx, sizeof(int32_t) * static_cast<size_t>(nnz), kind, g->s));
+        else UMX_HIP(hipMemsetAsync(g->idx.p, 0, sizeof(int32_t) * static_cast<size_t>(nnz), g->s));
+        UMX_HIP(hipMemcpyAsync(g->val(), val, dsize(g->dtype) * static_cast<size_t>(nnz), kind, g->s));
+    }
+    if (device_ptrs) check_rowptr_device(g->rp.p, n, nnz, g->s);
+    DevBuf<int64_t> unused;
+    g->max_degree = umx::max_row_length(g->rp.p, n, unused, g->s);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* umx_last_error(void) { return g_last_error.c_str(); }
+int umx_version(void) { return 100; }
+
+int umx_graph_create(int dtype, int64_t n, const int64_t* row_ptr, const int32_t* idx, const void* val,
+                     int device_ptrs, umx_graph** out) {
+    return guard([&] {
+        require(out != nullptr, UMX_ERR_ARGUMENT, "out is NULL");
+        *out = nullptr;
+        require(dtype == UMX_F32 || dtype == UMX_F64, UMX_ERR_ARGUMENT, "dtype must be UMX_F32 or UMX_F64");
+        require(n >= 0 && n < std::numeric_limits<int32_t>::max(), UMX_ERR_OVERFLOW, "n must fit a 32-bit index");
+        require(row_ptr != nullptr && (n == 0 || val != nullptr), UMX_ERR_ARGUMENT, "row_ptr/val is NULL");
+        std::unique_ptr<umx_graph> g(new umx_graph());
+        g->dtype = dtype;
+        g->n = n;
+        g->s = make_stream();
+        graph_upload(g.get(), row_ptr, idx, val, device_ptrs);
+        *out = g.release();
+    });
+}
+
+void umx_graph_destroy(umx_graph* g) { delete g; }
+
+int umx_graph_info(const umx_graph* g, int64_t* n, int64_t* nnz, int* dtype) {
+    return guard([&] {
+        require(g != nullptr, UMX_ERR_ARGUMENT, "graph is NULL");
+        if (n) *n = g->n;
+        if (nnz) *nnz = g->nnz;
+        if (dtype) *dtype = g->dtype;
+    });
+}
+
+int umx_graph_neighbor_similarities(umx_graph* g, double local_connectivity, double bandwidth,
+                                    double min_k_dist_scale, int use_newton) {
+    return guard([&] {
+        require(g != nullptr, UMX_ERR_ARGUMENT, "graph is NULL");
+        // Options are stored in the caller's Float_ first (initialize.hpp:71-74),
+        // then the solve runs in double.
+        double lc = local_connectivity, bw = bandwidth, mk = min_k_dist_scale;
+        if (g->dtype == UMX_F32) {
+            lc = static_cast<float>(lc);
+            bw = static_cast<float>(bw);
+            mk = static_cast<float>(mk);
+        }
+        require(std::isfinite(lc) && lc >= 0, UMX_ERR_ARGUMENT, "local_connectivity must be finite and >= 0");
+        require(lc < 2147483647.0, UMX_ERR_OVERFLOW, "local_connectivity does not fit an index");
+        g->rho.alloc(static_cast<size_t>(g->n));
+        g->sigma.alloc(static_cast<size_t>(g->n));
+        if (g->dtype == UMX_F32)
+            umx::launch_similarities<float>(g->n, g->rp.p, g->vf.p, lc, bw, mk, use_newton, g->rho.p, g->sigma.p,
+                                            g->max_degree, g->s);
+        else
+            umx::launch_similarities<double>(g->n, g->rp.p, g->vd.p, lc, bw, mk, use_newton, g->rho.p, g->sigma.p,
+                                             g->max_degree, g->s);
+        UMX_HIP(hipStreamSynchronize(g->s));
+        g->have_rho_sigma = true;
+    });
+}
+
+int umx_graph_combine(umx_graph* g, double mix_ratio) {
+    return guard([&] {
+        require(g != nullptr, UMX_ERR_ARGUMENT, "graph is NULL");
+        require(std::isfinite(mix_ratio), UMX_ERR_ARGUMENT, "mix_ratio must be finite");
+        DevBuf<int64_t> rp;
+        DevBuf<int32_t> idx;
+        if (g->dtype == UMX_F32) {
+            DevBuf<float> v;
+            umx::run_combine<float>(g->n, g->rp.p, g->idx.p, g->vf.p, static_cast<float>(mix_ratio), g->max_degree,
+                                    rp, idx, v, g->s);
+            g->vf.swap(v);
+        } else {
+            DevBuf<double> v;
+            umx::run_combine<double>(g->n, g->rp.p, g->idx.p, g->vd.p, mix_ratio, g->max_degree, rp, idx, v, g->s);
+            g->vd.swap(v);
+        }
+        g->rp.swap(rp);
+        g->idx.swap(idx);
+        UMX_HIP(hipMemcpyAsync(&g->nnz, g->rp.p + g->n, sizeof(int64_t), hipMemcpyDeviceToHost, g->s));
+        DevBuf<int64_t> unused;
+        g->max_degree = umx::max_row_length(g->rp.p, g->n, unused, g->s);
+    });
+}
+
+int umx_graph_export(const umx_graph* g, int64_t* row_ptr, int32_t* idx, void* val) {
+    return guard([&] {
+        require(g != nullptr, UMX_ERR_ARGUMENT, "graph is NULL");
+        if (row_ptr)
+            UMX_HIP(hipMemcpyAsync(row_ptr, g->rp.p, sizeof(int64_t) * static_cast<size_t>(g->n + 1), hipMemcpyDefault, g->s));
+        if (g->nnz > 0) {
+            if (idx) UMX_HIP(hipMemcpyAsync(idx, g->idx.p, sizeof(int32_t) * static_cast<size_t>(g->nnz), hipMemcpyDefault, g->s));
+            if (val) UMX_HIP(hipMemcpyAsync(val, g->val(), dsize(g->dtype) * static_cast<size_t>(g->nnz), hipMemcpyDefault, g->s));
+        }
+        UMX_HIP(hipStreamSynchronize(g->s));
+    });
+}
+
+int umx_graph_export_rho_sigma(const umx_graph* g, double* rho, double* sigma) {
+    return guard([&] {
+        require(g != nullptr, UMX_ERR_ARGUMENT, "graph is NULL");
+        require(g->have_rho_sigma, UMX_ERR_STATE, "neighbor_similarities has not run on this graph");
+        if (g->n > 0) {
+            if (rho) UMX_HIP(hipMemcpyAsync(rho, g->rho.p, sizeof(double) * static_cast<size_t>(g->n), hipMemcpyDefault, g->s));
+            if (sigma) UMX_HIP(hipMemcpyAsync(sigma, g->sigma.p, sizeof(double) * static_cast<size_t>(g->n), hipMemcpyDefault, g->s));
+        }
+        UMX_HIP(hipStreamSynchronize(g->s));
+    });
+}
+
+int umx_neighbor_similarities(int dtype, int64_t n, const int64_t* row_ptr, void* val, double local_connectivity,
+                              double bandwidth, double min_k_dist_scale, int use_newton, double* rho_out,
+                              double* sigma_out) {
+    umx_graph* g = nullptr;
+    int rc = umx_graph_create(dtype, n, row_ptr, nullptr, val, 0, &g);
+    if (rc != UMX_OK) return rc;
+    rc = umx_graph_neighbor_similarities(g, local_connectivity, bandwidth, min_k_dist_scale, use_newton);
+    if (rc == UMX_OK) rc = umx_graph_export(g, nullptr, nullptr, val);
+    if (rc == UMX_OK && (rho_out || sigma_out)) rc = umx_graph_export_rho_sigma(g, rho_out, sigma_out);
+    umx_graph_destroy(g);
+    return rc;
+}
+
+int umx_combine_neighbor_sets(int dtype, int64_t n, const int64_t* row_ptr, const int32_t* idx, const void* val,
+                              double mix_ratio, int64_t cap, int64_t* out_row_ptr, int32_t* out_idx, void* out_val,
+                              int64_t* out_nnz) {
+    umx_graph* g = nullptr;
+    int rc = guard([&] { require(idx != nullptr, UMX_ERR_ARGUMENT, "idx is NULL"); });
+    if (rc != UMX_OK) return rc;
+    rc = umx_graph_create(dtype, n, row_ptr, idx, val, 0, &g);
+    if (rc != UMX_OK) return rc;
+    rc = umx_graph_combine(g, mix_ratio);
+    if (rc == UMX_OK) {
+        if (out_nnz) *out_nnz = g->nnz;
+        if (g->nnz > cap) {
+            g_last_error = "output capacity too small";
+            rc = UMX_ERR_CAPACITY;
+        } else {
+            rc = umx_graph_export(g, out_row_ptr, out_idx, out_val);
+        }
+    }
+    umx_graph_destroy(g);
+    return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Layout handle
+// ---------------------------------------------------------------------------
+struct umx_layout {
+    int dtype = UMX_F32;
+    int64_t n = 0, nnz = 0, max_degree = 0;
+    int total_epochs = 0, current_epoch = 0;
+    double nsr = 5;
+    hipStream_t s = nullptr;
+    DevBuf<int64_t> rp;
+    DevBuf<int32_t> tg;
+    DevBuf<float> ef, nf, gf, mf;  // eps, next, next_neg, embedding (float)
+    DevBuf<double> ed, nd, gd, md;  // same (double)
+    DevBuf<unsigned long long> tally;
+    DevBuf<unsigned long long> cursor;  // serial replay cursor + overrun flag
+    int dim = 0;
+    double a = 0, b = 0, gamma = 1, alpha0 = 1;
+    uint64_t seed = 0;
+    bool params = false, have_emb = false;
+    bool timing = false;
+    double kernel_ms = 0;
+    int64_t launches = 0;
+    std::vector<hipEvent_t> events;
+
+    ~umx_layout() {
+        for (auto e : events) (void)hipEventDestroy(e);
+        if (s) (void)hipStreamDestroy(s);
+    }
+    void* emb() { return dtype == UMX_F32 ? static_cast<void*>(mf.p) : static_cast<void*>(md.p); }
+    const void* emb() const { return dtype == UMX_F32 ? static_cast<const void*>(mf.p) : static_cast<const void*>(md.p); }
+};
+
+namespace {
+
+template <typename T>
+umx::LayoutView<T> view_of(umx_layout* L, DevBuf<T>& eps, DevBuf<T>& nxt, DevBuf<T>& nng, DevBuf<T>& emb) {
+    umx::LayoutView<T> v;
+    v.n = L->n;
+    v.nnz = L->nnz;
+    v.rp = L->rp.p;
+    v.tg = L->tg.p;
+    v.eps = eps.p;
+    v.nxt = nxt.p;
+    v.nxt_neg = nng.p;
+    v.emb = emb.p;
+    v.dim = L->dim;
+    v.a = static_cast<T>(L->a);
+    v.b = static_cast<T>(L->b);
+    v.gamma = static_cast<T>(L->gamma);
+    v.nsr = static_cast<T>(L->nsr);
+    v.tally = L->tally.p;
+    return v;
+}
+
+// Learning rate of epoch n with the reference's types (optimize_layout.hpp:
+// 132-133): epoch / num_epochs in T, 1.0 - x in double, times alpha0 (T),
+// rounded to T.
+template <typename T>
+double epoch_alpha(double alpha0, int n, int total) {
+    const T ep = static_cast<T>(n);
+    const T a0 = static_cast<T>(alpha0);
+    return static_cast<double>(static_cast<T>(a0 * (1.0 - ep / static_cast<T>(total))));
+}
+
+void layout_from_graph_arrays(umx_layout* L, const int64_t* rp, const int32_t* idx, const void* val, int num_epochs,
+                              double nsr) {
+    if (L->dtype == UMX_F32) {
+        umx::run_to_epochs<float>(L->n, rp, idx, static_cast<const float*>(val), num_epochs, static_cast<float>(nsr),
+                                  L->max_degree, L->rp, L->tg, L->ef, L->nf, L->gf, L->s);
+    } else {
+        umx::run_to_epochs<double>(L->n, rp, idx, static_cast<const double*>(val), num_epochs, nsr, L->max_degree,
+                                   L->rp, L->tg, L->ed, L->nd, L->gd, L->s);
+    }
+    UMX_HIP(hipMemcpyAsync(&L->nnz, L->rp.p + L->n, sizeof(int64_t), hipMemcpyDeviceToHost, L->s));
+    L->tally.alloc(static_cast<size_t>(3) * static_cast<size_t>(num_epochs > 0 ? num_epochs : 1));
+    UMX_HIP(hipMemsetAsync(L->tally.p, 0, sizeof(unsigned long long) * L->tally.n, L->s));
+    L->cursor.alloc(2);
+    UMX_HIP(hipMemsetAsync(L->cursor.p, 0, sizeof(unsigned long long) * 2, L->s));
+    DevBuf<int64_t> unused;
+    L->max_degree = umx::max_row_length(L->rp.p, L->n, unused, L->s);
+    UMX_HIP(hipStreamSynchronize(L->s));
+}
+
+void check_epoch_args(int num_epochs, double nsr, int dtype) {
+    require(num_epochs >= 1, UMX_ERR_ARGUMENT, "num_epochs must be >= 1");
+    require(std::isfinite(nsr) && nsr > 0, UMX_ERR_ARGUMENT, "negative_sample_rate must be finite and > 0");
+    // The reference checks that num_epochs * negative_sample_rate fits an int
+    // (optimize_layout.hpp:84-89), in Float_.
+    const double prod = dtype == UMX_F32 ? static_cast<double>(static_cast<float>(num_epochs) * static_cast<float>(nsr))
+                                         : static_cast<double>(num_epochs) * nsr;
+    require(prod <= static_cast<double>(std::numeric_limits<int>::max()), UMX_ERR_OVERFLOW,
+            "num_epochs * negative_sample_rate does not fit an int");
+}
+
+}  // namespace
+
+extern "C" {
+
+int umx_layout_create(const umx_graph* g, int num_epochs, double negative_sample_rate, umx_layout** out) {
+    return guard([&] {
+        require(g != nullptr && out != nullptr, UMX_ERR_ARGUMENT, "NULL argument");
+        *out = nullptr;
+        check_epoch_args(num_epochs, negative_sample_rate, g->dtype);
+        std::unique_ptr<umx_layout> L(new umx_layout());
+        L->dtype = g->dtype;
+        L->n = g->n;
+        L->total_epochs = num_epochs;
+        L->nsr = g->dtype == UMX_F32 ? static_cast<double>(static_cast<float>(negative_sample_rate)) : negative_sample_rate;
+        L->s = make_stream();
+        L->max_degree = g->max_degree;
+        UMX_HIP(hipStreamSynchronize(g->s));
+        layout_from_graph_arrays(L.get(), g->rp.p, g->idx.p, g->val(), num_epochs, negative_sample_rate);
+        *out = L.release();
+    });
+}
+
+int umx_layout_create_csr(int dtype, int64_t n, const int64_t* row_ptr, const int32_t* idx, const void* val,
+                          int device_ptrs, int num_epochs, double negative_sample_rate, umx_layout** out) {
+    umx_graph* g = nullptr;
+    int rc = umx_graph_create(dtype, n, row_ptr, idx, val, device_ptrs, &g);
+    if (rc != UMX_OK) return rc;
+    rc = umx_layout_create(g, num_epochs, negative_sample_rate, out);
+    umx_graph_destroy(g);
+    return rc;
+}
+
+void umx_layout_destroy(umx_layout* L) { delete L; }
+
+int umx_layout_info(const umx_layout* L, int64_t* n, int64_t* nnz, int* total_epochs, int* current_epoch) {
+    return guard([&] {
+        require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
+        if (n) *n = L->n;
+        if (nnz) *nnz = L->nnz;
+        if (total_epochs) *total_epochs = L->total_epochs;
+        if (current_epoch) *current_epoch = L->current_epoch;
+    });
+}
+
+int umx_layout_export_epochs(const umx_layout* L, int64_t* row_ptr, int32_t* targets, void* eps, void* nxt,
+                             void* nxt_neg) {
+    return guard([&] {
+        require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
+        const size_t E = static_cast<size_t>(L->nnz);
+        const size_t es = dsize(L->dtype);
+        if (row_ptr) UMX_HIP(hipMemcpyAsync(row_ptr, L->rp.p, sizeof(int64_t) * static_cast<size_t>(L->n + 1), hipMemcpyDefault, L->s));
+        if (E > 0) {
+            const bool f = L->dtype == UMX_F32;
+            if (targets) UMX_HIP(hipMemcpyAsync(targets, L->tg.p, sizeof(int32_t) * E, hipMemcpyDefault, L->s));
+            if (eps) UMX_HIP(hipMemcpyAsync(eps, f ? static_cast<const void*>(L->ef.p) : L->ed.p, es * E, hipMemcpyDefault, L->s));
+            if (nxt) UMX_HIP(hipMemcpyAsync(nxt, f ? static_cast<const void*>(L->nf.p) : L->nd.p, es * E, hipMemcpyDefault, L->s));
+            if (nxt_neg) UMX_HIP(hipMemcpyAsync(nxt_neg, f ? static_cast<const void*>(L->gf.p) : L->gd.p, es * E, hipMemcpyDefault, L->s));
+        }
+        UMX_HIP(hipStreamSynchronize(L->s));
+    });
+}
+
+int umx_layout_set_params(umx_layout* L, int num_dim, double a, double b, double gamma, double learning_rate,
+                          uint64_t seed) {
+    return guard([&] {
+        require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
+        require(num_dim >= 1, UMX_ERR_ARGUMENT, "num_dim must be >= 1");
+        require(std::isfinite(a) && std::isfinite(b) && std::isfinite(gamma) && std::isfinite(learning_rate),
+                UMX_ERR_ARGUMENT, "a, b, gamma and learning_rate must be finite");
+        if (L->dim != num_dim) {
+            L->have_emb = false;
+            if (L->dtype == UMX_F32) L->mf.alloc(static_cast<size_t>(L->n) * static_cast<size_t>(num_dim));
+            else L->md.alloc(static_cast<size_t>(L->n) * static_cast<size_t>(num_dim));
+        }
+        L->dim = num_dim;
+        L->a = a;
+        L->b = b;
+        L->gamma = gamma;
+        L->alpha0 = learning_rate;
+        L->seed = seed;
+        L->params = true;
+    });
+}
+
+int umx_layout_set_embedding(umx_layout* L, const void* embedding, int device_ptr) {
+    return guard([&] {
+        require(L != nullptr && embedding != nullptr, UMX_ERR_ARGUMENT, "NULL argument");
+        require(L->params, UMX_ERR_STATE, "umx_layout_set_params must be called first");
+        const size_t bytes = dsize(L->dtype) * static_cast<size_t>(L->n) * static_cast<size_t>(L->dim);
+        if (bytes) UMX_HIP(hipMemcpyAsync(L->emb(), embedding, bytes, device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, L->s));
+        UMX_HIP(hipStreamSynchronize(L->s));
+        L->have_emb = true;
+    });
+}
+
+int umx_layout_get_embedding(const umx_layout* L, void* embedding, int device_ptr) {
+    return guard([&] {
+        require(L != nullptr && embedding != nullptr, UMX_ERR_ARGUMENT, "NULL argument");
+        require(L->have_emb, UMX_ERR_STATE, "no embedding has been set");
+        const size_t bytes = dsize(L->dtype) * static_cast<size_t>(L->n) * static_cast<size_t>(L->dim);
+        if (bytes) UMX_HIP(hipMemcpyAsync(embedding, L->emb(), bytes, device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, L->s));
+        UMX_HIP(hipStreamSynchronize(L->s));
+    });
+}
+
+int umx_layout_run(umx_layout* L, int epoch_limit) {
+    return guard([&] {
+        require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
+        require(L->params && L->have_emb, UMX_ERR_STATE, "parameters and embedding must be set before run");
+        require(epoch_limit >= L->current_epoch && epoch_limit <= L->total_epochs, UMX_ERR_ARGUMENT,
+                "epoch_limit must lie in [epoch(), num_epochs()]");
+        const int first = L->current_epoch;
+        const int count = epoch_limit - first;
+        if (L->timing && static_cast<int>(L->events.size()) < 2 * count) {
+            while (static_cast<int>(L->events.size()) < 2 * count) {
+                hipEvent_t e;
+                UMX_HIP(hipEventCreate(&e));
+                L->events.push_back(e);
+            }
+        }
+        for (int q = 0; q < count; ++q) {
+            const int ep = first + q;
+            umx::EpochArgs ea;
+            ea.epoch = ep;
+            ea.ekey = umx::counter_epoch_key(L->seed, ep);
+            if (L->timing) UMX_HIP(hipEventRecord(L->events[2 * q], L->s));
+            if (L->dtype == UMX_F32) {
+                ea.alpha = epoch_alpha<float>(L->alpha0, ep, L->total_epochs);
+                umx::launch_layout_epoch<float>(view_of<float>(L, L->ef, L->nf, L->gf, L->mf), ea, L->s);
+            } else {
+                ea.alpha = epoch_alpha<double>(L->alpha0, ep, L->total_epochs);
+                umx::launch_layout_epoch<double>(view_of<double>(L, L->ed, L->nd, L->gd, L->md), ea, L->s);
+            }
+            if (L->timing) UMX_HIP(hipEventRecord(L->events[2 * q + 1], L->s));
+        }
+        UMX_HIP(hipStreamSynchronize(L->s));
+        if (L->timing) {
+            for (int q = 0; q < count; ++q) {
+                float ms = 0;
+                UMX_HIP(hipEventElapsedTime(&ms, L->events[2 * q], L->events[2 * q + 1]));
+                L->kernel_ms += ms;
+            }
+            L->launches += count;
+        }
+        L->current_epoch = epoch_limit;
+    });
+}
+
+int umx_layout_run_serial(umx_layout* L, int epoch_limit, const int32_t* samples, int64_t count) {
+    return guard([&] {
+        require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
+        require(L->params && L->have_emb, UMX_ERR_STATE, "parameters and embedding must be set before run");
+        require(epoch_limit >= L->current_epoch && epoch_limit <= L->total_epochs, UMX_ERR_ARGUMENT,
+                "epoch_limit must lie in [epoch(), num_epochs()]");
+        require(samples == nullptr || count >= 0, UMX_ERR_ARGUMENT, "negative sample count");
+        int32_t* dsamples = nullptr;
+        if (samples && count > 0) {
+            UMX_HIP(hipMallocAsync(reinterpret_cast<void**>(&dsamples), sizeof(int32_t) * static_cast<size_t>(count), L->s));
+            UMX_HIP(hipMemcpyAsync(dsamples, samples, sizeof(int32_t) * static_cast<size_t>(count), hipMemcpyHostToDevice, L->s));
+        }
+        UMX_HIP(hipMemsetAsync(L->cursor.p, 0, sizeof(unsigned long long) * 2, L->s));
+        const int32_t* sp = samples ? (dsamples ? dsamples : reinterpret_cast<const int32_t*>(L->cursor.p)) : nullptr;
+        for (int ep = L->current_epoch; ep < epoch_limit; ++ep) {
+            umx::EpochArgs ea;
+            ea.epoch = ep;
+            ea.ekey = umx::counter_epoch_key(L->seed, ep);
+            if (L->dtype == UMX_F32) {
+                ea.alpha = epoch_alpha<float>(L->alpha0, ep, L->total_epochs);
+                umx::launch_layout_serial<float>(view_of<float>(L, L->ef, L->nf, L->gf, L->mf), ea, sp, count,
+                                                 L->cursor.p, L->s);
+            } else {
+                ea.alpha = epoch_alpha<double>(L->alpha0, ep, L->total_epochs);
+                umx::launch_layout_serial<double>(view_of<double>(L, L->ed, L->nd, L->gd, L->md), ea, sp, count,
+                                                  L->cursor.p, L->s);
+            }
+        }
+        unsigned long long host_cursor[2] = {0, 0};
+        UMX_HIP(hipMemcpyAsync(host_cursor, L->cursor.p, sizeof(host_cursor), hipMemcpyDeviceToHost, L->s));
+        if (dsamples) UMX_HIP(hipFreeAsync(dsamples, L->s));
+        UMX_HIP(hipStreamSynchronize(L->s));
+        L->current_epoch = epoch_limit;
+        require(samples == nullptr || (host_cursor[1] & 0xffffffffULL) == 0, UMX_ERR_ARGUMENT,
+                "replayed samples ran out or held an index outside [0, n)");
+    });
+}
+
+int umx_layout_tallies(const umx_layout* L, int64_t* out, int64_t cap) {
+    return guard([&] {
+        require(L != nullptr && out != nullptr, UMX_ERR_ARGUMENT, "NULL argument");
+        const int64_t m = std::min<int64_t>(cap, static_cast<int64_t>(L->tally.n));
+        if (m > 0) UMX_HIP(hipMemcpyAsync(out, L->tally.p, sizeof(int64_t) * static_cast<size_t>(m), hipMemcpyDeviceToHost, L->s));
+        UMX_HIP(hipStreamSynchronize(L->s));
+    });
+}
+
+int umx_layout_set_timing(umx_layout* L, int on) {
+    return guard([&] {
+        require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
+        L->timing = on != 0;
+        L->kernel_ms = 0;
+        L->launches = 0;
+    });
+}
+
+int umx_layout_timing(const umx_layout* L, double* kernel_ms, int64_t* launches) {
+    return guard([&] {
+        require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
+        if (kernel_ms) *kernel_ms = L->kernel_ms;
+        if (launches) *launches = L->launches;
+    });
+}
+
+}  // extern "C"

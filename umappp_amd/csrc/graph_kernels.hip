@@ -1,0 +1,840 @@
+// Graph-side kernels for gfx950:
+//   K1  fuzzy similarities (rho / sigma / weights), FP64 per row
+//   K2  fuzzy-union symmetrisation, integer rank sorts + prefix scans
+//   K3  similarities -> epochs_per_sample schedule
+//
+// Rows are processed by "row groups": G consecutive lanes of a 64-lane
+// wavefront own one row (G = smallest power of two >= the longest row,
+// 4..64), so a k = 15 kNN graph packs four rows per wavefront.  Lane-parallel
+// sums use xor-butterfly shuffles inside the group, which leave bit-identical
+// totals in every lane, so control flow stays uniform per row.
+//
+// Built with -ffp-contract=off: the union weights and schedule values must
+// round exactly as the reference's separate multiply and add do.
+#include "umx_internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <limits>
+
+namespace umx {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// Stream-ordered temporary buffer.
+template <typename T>
+struct Tmp {
+    T* p = nullptr;
+    hipStream_t s;
+    Tmp(size_t count, hipStream_t st) : s(st) {
+        if (count == 0) count = 1;
+        UMX_HIP(hipMallocAsync(reinterpret_cast<void**>(&p), count * sizeof(T), s));
+    }
+    ~Tmp() {
+        if (p) (void)hipFreeAsync(p, s);
+    }
+    Tmp(const Tmp&) = delete;
+    Tmp& operator=(const Tmp&) = delete;
+};
+
+inline unsigned grid_for(int64_t threads) {
+    const int64_t b = (threads + kBlock - 1) / kBlock;
+    if (b > 0x7fffffffLL) throw Error(-5, "grid too large");
+    return static_cast<unsigned>(b < 1 ? 1 : b);
+}
+
+template <int G>
+__device__ inline double group_sum(double v) {
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, G);
+    return v;
+}
+template <int G>
+__device__ inline long long group_min(long long v) {
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1) {
+        const long long o = __shfl_xor(v, m, G);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+template <int G>
+__device__ inline long long group_max(long long v) {
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1) {
+        const long long o = __shfl_xor(v, m, G);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+template <int G>
+__device__ inline long long group_add(long long v) {
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, G);
+    return v;
+}
+// Value held by entry q of the row (entry q lives in lane q % G, slot q / G).
+template <int G, int M>
+__device__ inline double group_fetch(const double (&d)[M], long long q) {
+    const int src = static_cast<int>(q & (G - 1));
+    const int slot = static_cast<int>(q / G);
+    double out = 0;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const double x = __shfl(d[m], src, G);
+        if (m == slot) out = x;
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// K1: neighbor_similarities (neighbor_similarities.hpp:53-207), double path.
+// ---------------------------------------------------------------------------
+struct SimArgs {
+    int64_t n;
+    const int64_t* rp;
+    long long raw;  // truncated local_connectivity
+    double interp;  // its fractional part
+    double bw;
+    double mkds;
+    int newton;
+    double* rho;
+    double* sigma;
+};
+
+template <typename T, int G, int M>
+__global__ void __launch_bounds__(kBlock) k_similarities(SimArgs a, T* __restrict__ val) {
+    const int64_t row = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / G;
+    const int lane = threadIdx.x & (G - 1);
+    if (row >= a.n) return;  // uniform per group
+    const int64_t start = a.rp[row];
+    const long long k = a.rp[row + 1] - start;
+    const double nan = __builtin_nan("");
+    if (k == 0) {
+        if (lane == 0) {
+            if (a.rho) a.rho[row] = nan;
+            if (a.sigma) a.sigma[row] = nan;
+        }
+        return;
+    }
+
+    double d[M];
+    bool ok[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const long long q = lane + static_cast<long long>(G) * m;
+        ok[m] = q < k;
+        d[m] = ok[m] ? static_cast<double>(val[start + q]) : 0.0;
+    }
+
+    // Leading exact zeros (:72-78): position of the first non-zero entry.
+    long long first_nz = k;
+#pragma unroll
+    for (int m = M - 1; m >= 0; --m) {
+        const long long q = lane + static_cast<long long>(G) * m;
+        if (ok[m] && d[m] != 0.0) first_nz = q;
+    }
+    const long long nzero = group_min<G>(first_nz);
+
+    auto all_ones = [&](double rho_v) {
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+            if (ok[m]) val[start + lane + static_cast<long long>(G) * m] = static_cast<T>(1);
+        if (lane == 0) {
+            if (a.rho) a.rho[row] = rho_v;
+            if (a.sigma) a.sigma[row] = nan;
+        }
+    };
+
+    if (k - nzero <= a.raw) {  // :80-88
+        all_ones(nan);
+        return;
+    }
+    const long long ci = nzero + a.raw;
+    const double lower = ci > 0 ? group_fetch<G, M>(d, ci - 1) : 0.0;
+    const double upper = group_fetch<G, M>(d, ci);
+    const double rho = lower + a.interp * (upper - lower);  // :92
+
+    // Active deltas (:95-104).
+    bool act[M];
+    double delta[M];
+    long long nle = 0, last_active = -1;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const long long q = lane + static_cast<long long>(G) * m;
+        const bool in = ok[m] && q >= nzero;
+        act[m] = in && d[m] > rho;
+        delta[m] = act[m] ? d[m] - rho : 0.0;
+        if (in && !act[m]) ++nle;
+        if (act[m]) last_active = q;
+    }
+    nle = group_add<G>(nle);
+    last_active = group_max<G>(last_active);
+    if (last_active < 0) {  // :106-112
+        all_ones(rho);
+        return;
+    }
+
+    const double below = static_cast<double>(nzero + nle);
+    double sigma = group_fetch<G, M>(d, last_active) - rho;  // active_delta.back() (:117)
+    double lo = 0.0, hi = DBL_MAX;
+    const double target = log2(static_cast<double>(k + 1)) * a.bw;  // :127
+
+    for (int it = 0; it < 64; ++it) {  // :129-184
+        const double inv = 1.0 / sigma;
+        const double inv2 = inv * inv;
+        double so = 0.0, sd = 0.0;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            if (act[m]) {
+                const double e = exp(-delta[m] * inv);
+                so += e;
+                sd += delta[m] * e * inv2;
+            }
+        }
+        so = group_sum<G>(so);
+        sd = group_sum<G>(sd);
+        const double diff = (below + so) - target;
+        if (fabs(diff) < 1e-5) break;
+        if (diff > 0) hi = sigma;
+        else lo = sigma;
+        bool stepped = false;
+        if (a.newton && sd != 0.0) {
+            const double cand = sigma - diff / sd;
+            if (cand > lo && cand < hi) {
+                sigma = cand;
+                stepped = true;
+            }
+        }
+        if (!stepped) {
+            if (diff > 0) sigma += (lo - sigma) / 2;
+            else if (hi == DBL_MAX) sigma *= 2;
+            else sigma += (hi - sigma) / 2;
+        }
+    }
+
+    double sum = 0.0;  // :187-192
+#pragma unroll
+    for (int m = 0; m < M; ++m) sum += d[m];
+    sum = group_sum<G>(sum);
+    const double mean = sum / static_cast<double>(k);
+    sigma = fmax(a.mkds * mean, sigma);
+
+    const double inv = 1.0 / sigma;  // :194-202
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        if (ok[m]) {
+            const double w = d[m] > rho ? exp(-(d[m] - rho) * inv) : 1.0;
+            val[start + lane + static_cast<long long>(G) * m] = static_cast<T>(w);
+        }
+    }
+    if (lane == 0) {
+        if (a.rho) a.rho[row] = rho;
+        if (a.sigma) a.sigma[row] = sigma;
+    }
+}
+
+// One thread per row, reference summation order; used for rows longer than
+// 512 entries, where the register-resident group kernel does not fit.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_similarities_seq(SimArgs a, T* __restrict__ val) {
+    const int64_t row = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (row >= a.n) return;
+    const int64_t start = a.rp[row];
+    const long long k = a.rp[row + 1] - start;
+    T* x = val + start;
+    const double nan = __builtin_nan("");
+    if (a.rho) a.rho[row] = nan;
+    if (a.sigma) a.sigma[row] = nan;
+    if (k == 0) return;
+    long long nzero = 0;
+    while (nzero < k && static_cast<double>(x[nzero]) == 0.0) ++nzero;
+    if (k - nzero <= a.raw) {
+        for (long long q = 0; q < k; ++q) x[q] = static_cast<T>(1);
+        return;
+    }
+    const long long ci = nzero + a.raw;
+    const double lower = ci > 0 ? static_cast<double>(x[ci - 1]) : 0.0;
+    const double upper = static_cast<double>(x[ci]);
+    const double rho = lower + a.interp * (upper - lower);
+    if (a.rho) a.rho[row] = rho;
+    double below = static_cast<double>(nzero);
+    long long last_active = -1;
+    for (long long q = nzero; q < k; ++q) {
+        if (static_cast<double>(x[q]) > rho) last_active = q;
+        else below += 1.0;
+    }
+    if (last_active < 0) {
+        for (long long q = 0; q < k; ++q) x[q] = static_cast<T>(1);
+        return;
+    }
+    double sigma = static_cast<double>(x[last_active]) - rho;
+    double lo = 0.0, hi = DBL_MAX;
+    const double target = log2(static_cast<double>(k + 1)) * a.bw;
+    for (int it = 0; it < 64; ++it) {
+        double obs = below, slope = 0.0;
+        const double inv = 1.0 / sigma, inv2 = inv * inv;
+        for (long long q = nzero; q < k; ++q) {
+            const double dq = static_cast<double>(x[q]);
+            if (dq > rho) {
+                const double dl = dq - rho;
+                const double e = exp(-dl * inv);
+                obs += e;
+                slope += dl * e * inv2;
+            }
+        }
+        const double diff = obs - target;
+        if (fabs(diff) < 1e-5) break;
+        if (diff > 0) hi = sigma;
+        else lo = sigma;
+        bool stepped = false;
+        if (a.newton && slope != 0.0) {
+            const double cand = sigma - diff / slope;
+            if (cand > lo && cand < hi) {
+                sigma = cand;
+                stepped = true;
+            }
+        }
+        if (!stepped) {
+            if (diff > 0) sigma += (lo - sigma) / 2;
+            else if (hi == DBL_MAX) sigma *= 2;
+            else sigma += (hi - sigma) / 2;
+        }
+    }
+    double mean = 0.0;
+    for (long long q = 0; q < k; ++q) mean += static_cast<double>(x[q]);
+    mean /= static_cast<double>(k);
+    sigma = fmax(a.mkds * mean, sigma);
+    if (a.sigma) a.sigma[row] = sigma;
+    const double inv = 1.0 / sigma;
+    for (long long q = 0; q < k; ++q) {
+        const double dq = static_cast<double>(x[q]);
+        x[q] = static_cast<T>(dq > rho ? exp(-(dq - rho) * inv) : 1.0);
+    }
+}
+
+template <typename T, int G>
+void sim_dispatch_m(const SimArgs& a, T* val, int M, hipStream_t s) {
+    const unsigned grid = grid_for(a.n * G);
+    switch (M) {
+        case 1: k_similarities<T, G, 1><<<grid, kBlock, 0, s>>>(a, val); break;
+        case 2: k_similarities<T, G, 2><<<grid, kBlock, 0, s>>>(a, val); break;
+        case 4: k_similarities<T, G, 4><<<grid, kBlock, 0, s>>>(a, val); break;
+        default: k_similarities<T, G, 8><<<grid, kBlock, 0, s>>>(a, val); break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Scan and reductions over int64 row data.
+// ---------------------------------------------------------------------------
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__global__ void __launch_bounds__(kBlock) k_scan_tiles(const int64_t* __restrict__ in, int64_t* __restrict__ out,
+                                                     int64_t count, int64_t* __restrict__ tile_sums) {
+    __shared__ int64_t warp_tot[kBlock / kWave];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + static_cast<int64_t>(threadIdx.x) * kScanItems;
+    int64_t v[kScanItems];
+    int64_t run = 0;
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) {
+        const int64_t idx = base + q;
+        v[q] = idx < count ? in[idx] : 0;
+        run += v[q];
+    }
+    // inclusive scan of per-thread totals across the wavefront
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    int64_t incl = run;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int64_t o = __shfl_up(incl, off, kWave);
+        if (lane >= off) incl += o;
+    }
+    if (lane == kWave - 1) warp_tot[wid] = incl;
+    __syncthreads();
+    int64_t wave_off = 0;
+    for (int w = 0; w < wid; ++w) wave_off += warp_tot[w];
+    int64_t acc = wave_off + incl - run;
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) {
+        const int64_t idx = base + q;
+        if (idx < count) out[idx] = acc;
+        acc += v[q];
+    }
+    if (threadIdx.x == kBlock - 1) {
+        int64_t tot = 0;
+        for (int w = 0; w < kBlock / kWave; ++w) tot += warp_tot[w];
+        tile_sums[blockIdx.x] = tot;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_add_tile_offsets(int64_t* __restrict__ out, int64_t count,
+                                                           const int64_t* __restrict__ tile_offs) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (idx < count) out[idx] += tile_offs[idx / kScanTile];
+}
+
+__global__ void k_scan_total(const int64_t* __restrict__ in, int64_t* __restrict__ out, int64_t count) {
+    out[count] = count > 0 ? out[count - 1] + in[count - 1] : 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_max_row(const int64_t* __restrict__ rp, int64_t n,
+                                                  unsigned long long* __restrict__ out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    long long len = 0;
+    if (i < n) len = rp[i + 1] - rp[i];
+#pragma unroll
+    for (int m = kWave / 2; m >= 1; m >>= 1) {
+        const long long o = __shfl_xor(len, m, kWave);
+        len = o > len ? o : len;
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0 && len > 0) atomicMax(out, static_cast<unsigned long long>(len));
+}
+
+// ---------------------------------------------------------------------------
+// K2: combine_neighbor_sets (combine_neighbor_sets.hpp:15-100).
+//
+// Result per unordered pair {i, t}: if both directed entries exist, both
+// output entries hold f(a, b) (union (a+b)-ab, product ab, or the mix of the
+// two); if only i->t exists, mix = 1 keeps a on i->t and adds t->i = a,
+// mix = 0 drops it, other mixes keep and mirror a*mix.  mix = 0 also removes
+// every exact zero.  Rows come out sorted by index.
+// ---------------------------------------------------------------------------
+enum MixMode { MIX_INTERSECT = 0, MIX_UNION = 1, MIX_OTHER = 2 };
+
+// Rank sort of every row by index, with validation of the indices.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_sort_rows(int64_t n, int G, const int64_t* __restrict__ rp,
+                                                    const int32_t* __restrict__ idx, const T* __restrict__ val,
+                                                    int32_t* __restrict__ oidx, T* __restrict__ oval,
+                                                    int* __restrict__ err) {
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t row = gid / G;
+    const int lane = static_cast<int>(gid & (G - 1));
+    if (row >= n) return;
+    const int64_t s = rp[row], e = rp[row + 1];
+    for (int64_t q = s + lane; q < e; q += G) {
+        const int32_t key = idx[q];
+        if (key < 0 || key >= n || key == row) atomicOr(err, 1);
+        int64_t rank = 0;
+        for (int64_t f = s; f < e; ++f) {
+            const int32_t o = idx[f];
+            rank += o < key;
+            if (o == key && f != q) atomicOr(err, 2);
+        }
+        oidx[s + rank] = key;
+        oval[s + rank] = val[q];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_reciprocal(int64_t n, int G, const int64_t* __restrict__ rp,
+                                                     const int32_t* __restrict__ sidx, const T* __restrict__ sval,
+                                                     T mix, int mode, T* __restrict__ nval,
+                                                     unsigned char* __restrict__ flags,
+                                                     int64_t* __restrict__ keep_cnt, int64_t* __restrict__ inc_cnt) {
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t row = gid / G;
+    const int lane = static_cast<int>(gid & (G - 1));
+    if (row >= n) return;
+    const int64_t s = rp[row], e = rp[row + 1];
+    int64_t kept = 0;
+    for (int64_t q = s + lane; q < e; q += G) {
+        const int32_t t = sidx[q];
+        const T a = sval[q];
+        // binary search for 'row' in the sorted row t
+        int64_t lo = rp[t], hi = rp[t + 1];
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (sidx[mid] < row) lo = mid + 1;
+            else hi = mid;
+        }
+        const bool found = lo < rp[t + 1] && sidx[lo] == row;
+        T v;
+        unsigned char keep, mirror = 0;
+        if (found) {
+            const T b = sval[lo];
+            const T prod = a * b;
+            if (mode == MIX_UNION) v = a + b - prod;
+            else if (mode == MIX_INTERSECT) v = prod;
+            else v = mix * (a + b - prod) + (1 - mix) * prod;
+            keep = mode == MIX_INTERSECT ? (v != 0) : 1;
+        } else if (mode == MIX_UNION) {
+            v = a;
+            keep = 1;
+            mirror = 1;
+        } else if (mode == MIX_INTERSECT) {
+            v = 0;
+            keep = 0;
+        } else {
+            v = a * mix;
+            keep = 1;
+            mirror = 1;
+        }
+        nval[q] = v;
+        flags[q] = static_cast<unsigned char>(keep | (mirror << 1));
+        kept += keep;
+        if (mirror) atomicAdd(reinterpret_cast<unsigned long long*>(inc_cnt + t), 1ULL);
+    }
+    if (kept) atomicAdd(reinterpret_cast<unsigned long long*>(keep_cnt + row), static_cast<unsigned long long>(kept));
+}
+
+__global__ void __launch_bounds__(kBlock) k_add_counts(const int64_t* __restrict__ a, const int64_t* __restrict__ b,
+                                                     int64_t* __restrict__ out, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n) out[i] = a[i] + b[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_stage_mirrors(int64_t n, int G, const int64_t* __restrict__ rp,
+                                                        const int32_t* __restrict__ sidx, const T* __restrict__ nval,
+                                                        const unsigned char* __restrict__ flags,
+                                                        const int64_t* __restrict__ inc_rp,
+                                                        unsigned long long* __restrict__ fill,
+                                                        int32_t* __restrict__ stage_src, T* __restrict__ stage_val) {
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t row = gid / G;
+    const int lane = static_cast<int>(gid & (G - 1));
+    if (row >= n) return;
+    const int64_t s = rp[row], e = rp[row + 1];
+    for (int64_t q = s + lane; q < e; q += G) {
+        if (flags[q] & 2) {
+            const int32_t t = sidx[q];
+            const unsigned long long slot = atomicAdd(fill + t, 1ULL);
+            const int64_t at = inc_rp[t] + static_cast<int64_t>(slot);
+            stage_src[at] = static_cast<int32_t>(row);
+            stage_val[at] = nval[q];
+        }
+    }
+}
+
+// Final rows: kept entries (already in index order) merged with the staged
+// mirrored entries (arbitrary order) by rank, so the output order is exactly
+// ascending index whatever order the staging atomics produced.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_merge_rows(int64_t n, int G, const int64_t* __restrict__ rp,
+                                                     const int32_t* __restrict__ sidx, const T* __restrict__ nval,
+                                                     const unsigned char* __restrict__ flags,
+                                                     const int64_t* __restrict__ inc_rp,
+                                                     const int32_t* __restrict__ stage_src,
+                                                     const T* __restrict__ stage_val, const int64_t* __restrict__ out_rp,
+                                                     int32_t* __restrict__ oidx, T* __restrict__ oval) {
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t row = gid / G;
+    const int lane = static_cast<int>(gid & (G - 1));
+    if (row >= n) return;
+    const int64_t s = rp[row], e = rp[row + 1];
+    const int64_t ss = inc_rp[row], se = inc_rp[row + 1];
+    const int64_t ob = out_rp[row];
+    for (int64_t q = s + lane; q < e; q += G) {
+        if (!(flags[q] & 1)) continue;
+        const int32_t key = sidx[q];
+        int64_t rank = 0;
+        for (int64_t f = s; f < q; ++f) rank += flags[f] & 1;
+        for (int64_t f = ss; f < se; ++f) rank += stage_src[f] < key;
+        oidx[ob + rank] = key;
+        oval[ob + rank] = nval[q];
+    }
+    for (int64_t q = ss + lane; q < se; q += G) {
+        const int32_t key = stage_src[q];
+        // every original entry is kept whenever mirrors exist (mix != 0)
+        int64_t lo = s, hi = e;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (sidx[mid] < key) lo = mid + 1;
+            else hi = mid;
+        }
+        int64_t rank = lo - s;
+        for (int64_t f = ss; f < se; ++f) rank += stage_src[f] < key;
+        oidx[ob + rank] = key;
+        oval[ob + rank] = stage_val[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K3: similarities_to_epochs (optimize_layout.hpp:47-92).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Bits;
+template <>
+struct Bits<float> {
+    using S = int;
+};
+template <>
+struct Bits<double> {
+    using S = long long;
+};
+
+// Largest non-NaN weight, starting from 0 like the reference's std::max fold.
+// Non-negative IEEE values order like their bit patterns as signed integers;
+// negative values map to negative integers and never beat the initial +0.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_max_weight(const T* __restrict__ val, int64_t nnz,
+                                                     typename Bits<T>::S* __restrict__ out) {
+    using S = typename Bits<T>::S;
+    S best = 0;
+    for (int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; q < nnz;
+         q += static_cast<int64_t>(gridDim.x) * kBlock) {
+        const T v = val[q];
+        if (v == v) {
+            S bits;
+            __builtin_memcpy(&bits, &v, sizeof(S));
+            best = bits > best ? bits : best;
+        }
+    }
+#pragma unroll
+    for (int m = kWave / 2; m >= 1; m >>= 1) {
+        const S o = __shfl_xor(best, m, kWave);
+        best = o > best ? o : best;
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0 && best > 0) atomicMax(out, best);
+}
+
+template <typename T>
+__device__ inline T bits_to(typename Bits<T>::S b) {
+    T v;
+    __builtin_memcpy(&v, &b, sizeof(T));
+    return v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_count_survivors(int64_t n, int G, const int64_t* __restrict__ rp,
+                                                          const T* __restrict__ val,
+                                                          const typename Bits<T>::S* __restrict__ maxbits,
+                                                          int num_epochs, int64_t* __restrict__ cnt) {
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t row = gid / G;
+    const int lane = static_cast<int>(gid & (G - 1));
+    if (row >= n) return;
+    const T limit = bits_to<T>(*maxbits) / static_cast<T>(num_epochs);  // :63
+    const int64_t s = rp[row], e = rp[row + 1];
+    int64_t c = 0;
+    for (int64_t q = s + lane; q < e; q += G) c += val[q] >= limit;
+    if (c) atomicAdd(reinterpret_cast<unsigned long long*>(cnt + row), static_cast<unsigned long long>(c));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_fill_epochs(int64_t n, int G, const int64_t* __restrict__ rp,
+                                                      const int32_t* __restrict__ idx, const T* __restrict__ val,
+                                                      const typename Bits<T>::S* __restrict__ maxbits,
+                                                      int num_epochs, T nsr, const int64_t* __restrict__ orp,
+                                                      int32_t* __restrict__ tg, T* __restrict__ eps,
+                                                      T* __restrict__ nxt, T* __restrict__ nxt_neg) {
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t row = gid / G;
+    const int lane = static_cast<int>(gid & (G - 1));
+    if (row >= n) return;
+    const T maxed = bits_to<T>(*maxbits);
+    const T limit = maxed / static_cast<T>(num_epochs);
+    const int64_t s = rp[row], e = rp[row + 1];
+    const int64_t ob = orp[row];
+    for (int64_t q = s + lane; q < e; q += G) {
+        const T w = val[q];
+        if (!(w >= limit)) continue;
+        int64_t rank = 0;
+        for (int64_t f = s; f < q; ++f) rank += val[f] >= limit;
+        const T ep = maxed / w;  // :70
+        tg[ob + rank] = idx[q];
+        eps[ob + rank] = ep;
+        nxt[ob + rank] = ep;  // :77
+        nxt_neg[ob + rank] = ep / nsr;  // :78-81
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host-side launchers
+// ---------------------------------------------------------------------------
+int lanes_for_degree(int64_t max_degree) {
+    int g = 4;
+    while (g < 64 && g < max_degree) g <<= 1;
+    return g;
+}
+
+void exclusive_scan_i64(const int64_t* in, int64_t* out, int64_t count, DevBuf<int64_t>& tmp, hipStream_t s) {
+    (void)tmp;
+    if (count > 0) {
+        const int64_t tiles = (count + kScanTile - 1) / kScanTile;
+        Tmp<int64_t> sums(static_cast<size_t>(tiles), s);
+        k_scan_tiles<<<static_cast<unsigned>(tiles), kBlock, 0, s>>>(in, out, count, sums.p);
+        UMX_HIP(hipGetLastError());
+        if (tiles > 1) {
+            Tmp<int64_t> offs(static_cast<size_t>(tiles + 1), s);
+            DevBuf<int64_t> unused;
+            exclusive_scan_i64(sums.p, offs.p, tiles, unused, s);
+            k_add_tile_offsets<<<grid_for(count), kBlock, 0, s>>>(out, count, offs.p);
+            UMX_HIP(hipGetLastError());
+        }
+    }
+    k_scan_total<<<1, 1, 0, s>>>(in, out, count);
+    UMX_HIP(hipGetLastError());
+}
+
+int64_t max_row_length(const int64_t* rp, int64_t n, DevBuf<int64_t>& tmp, hipStream_t s) {
+    (void)tmp;
+    if (n == 0) return 0;
+    Tmp<unsigned long long> m(1, s);
+    UMX_HIP(hipMemsetAsync(m.p, 0, sizeof(unsigned long long), s));
+    k_max_row<<<grid_for(n), kBlock, 0, s>>>(rp, n, m.p);
+    UMX_HIP(hipGetLastError());
+    unsigned long long h = 0;
+    UMX_HIP(hipMemcpyAsync(&h, m.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    UMX_HIP(hipStreamSynchronize(s));
+    return static_cast<int64_t>(h);
+}
+
+template <typename T>
+void launch_similarities(int64_t n, const int64_t* rp, T* val, double lc, double bw, double mkds, int newton,
+                         double* rho, double* sigma, int64_t max_degree, hipStream_t s) {
+    if (n == 0) return;
+    SimArgs a;
+    a.n = n;
+    a.rp = rp;
+    a.raw = static_cast<long long>(lc);
+    a.interp = lc - static_cast<double>(a.raw);
+    a.bw = bw;
+    a.mkds = mkds;
+    a.newton = newton;
+    a.rho = rho;
+    a.sigma = sigma;
+    if (max_degree > 512) {
+        k_similarities_seq<T><<<grid_for(n), kBlock, 0, s>>>(a, val);
+    } else {
+        const int G = lanes_for_degree(max_degree);
+        int M = 1;
+        while (static_cast<int64_t>(G) * M < max_degree) M <<= 1;
+        switch (G) {
+            case 4: sim_dispatch_m<T, 4>(a, val, M, s); break;
+            case 8: sim_dispatch_m<T, 8>(a, val, M, s); break;
+            case 16: sim_dispatch_m<T, 16>(a, val, M, s); break;
+            case 32: sim_dispatch_m<T, 32>(a, val, M, s); break;
+            default: sim_dispatch_m<T, 64>(a, val, M, s); break;
+        }
+    }
+    UMX_HIP(hipGetLastError());
+}
+
+template <typename T>
+void run_combine(int64_t n, const int64_t* rp, const int32_t* idx, const T* val, T mix, int64_t max_degree,
+                 DevBuf<int64_t>& out_rp, DevBuf<int32_t>& out_idx, DevBuf<T>& out_val, hipStream_t s) {
+    const int64_t nnz_in = [&] {
+        int64_t v = 0;
+        UMX_HIP(hipMemcpyAsync(&v, rp + n, sizeof(v), hipMemcpyDeviceToHost, s));
+        UMX_HIP(hipStreamSynchronize(s));
+        return v;
+    }();
+    const int mode = mix == static_cast<T>(1) ? MIX_UNION : (mix == static_cast<T>(0) ? MIX_INTERSECT : MIX_OTHER);
+    const int G = lanes_for_degree(max_degree);
+    const unsigned grid = grid_for(n * G);
+
+    Tmp<int32_t> sidx(static_cast<size_t>(nnz_in), s);
+    Tmp<T> sval(static_cast<size_t>(nnz_in), s);
+    Tmp<int> err(1, s);
+    UMX_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
+    k_sort_rows<T><<<grid, kBlock, 0, s>>>(n, G, rp, idx, val, sidx.p, sval.p, err.p);
+    UMX_HIP(hipGetLastError());
+    int herr = 0;
+    UMX_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    UMX_HIP(hipStreamSynchronize(s));
+    if (herr & 1) throw Error(-2, "neighbour index out of range or equal to its own row");
+    if (herr & 2) throw Error(-2, "duplicate neighbour index within a row");
+
+    Tmp<T> nval(static_cast<size_t>(nnz_in), s);
+    Tmp<unsigned char> flags(static_cast<size_t>(nnz_in), s);
+    Tmp<int64_t> cnt(static_cast<size_t>(2 * n), s);  // keep counts, then incoming counts
+    int64_t* keep_cnt = cnt.p;
+    int64_t* inc_cnt = cnt.p + n;
+    UMX_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int64_t) * static_cast<size_t>(2 * n), s));
+    k_reciprocal<T><<<grid, kBlock, 0, s>>>(n, G, rp, sidx.p, sval.p, mix, mode, nval.p, flags.p, keep_cnt, inc_cnt);
+    UMX_HIP(hipGetLastError());
+
+    DevBuf<int64_t> unused;
+    Tmp<int64_t> inc_rp(static_cast<size_t>(n + 1), s);
+    exclusive_scan_i64(inc_cnt, inc_rp.p, n, unused, s);
+    Tmp<int64_t> deg(static_cast<size_t>(n), s);
+    k_add_counts<<<grid_for(n), kBlock, 0, s>>>(keep_cnt, inc_cnt, deg.p, n);
+    UMX_HIP(hipGetLastError());
+    out_rp.alloc(static_cast<size_t>(n + 1));
+    exclusive_scan_i64(deg.p, out_rp.p, n, unused, s);
+
+    int64_t nnz_out = 0, n_inc = 0;
+    UMX_HIP(hipMemcpyAsync(&nnz_out, out_rp.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    UMX_HIP(hipMemcpyAsync(&n_inc, inc_rp.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    UMX_HIP(hipStreamSynchronize(s));
+
+    Tmp<int32_t> stage_src(static_cast<size_t>(n_inc), s);
+    Tmp<T> stage_val(static_cast<size_t>(n_inc), s);
+    if (n_inc > 0) {
+        UMX_HIP(hipMemsetAsync(keep_cnt, 0, sizeof(int64_t) * static_cast<size_t>(n), s));  // reused as fill cursors
+        k_stage_mirrors<T><<<grid, kBlock, 0, s>>>(n, G, rp, sidx.p, nval.p, flags.p, inc_rp.p,
+                                                   reinterpret_cast<unsigned long long*>(keep_cnt), stage_src.p,
+                                                   stage_val.p);
+        UMX_HIP(hipGetLastError());
+    }
+    out_idx.alloc(static_cast<size_t>(nnz_out));
+    out_val.alloc(static_cast<size_t>(nnz_out));
+    k_merge_rows<T><<<grid, kBlock, 0, s>>>(n, G, rp, sidx.p, nval.p, flags.p, inc_rp.p, stage_src.p, stage_val.p,
+                                            out_rp.p, out_idx.p, out_val.p);
+    UMX_HIP(hipGetLastError());
+    UMX_HIP(hipStreamSynchronize(s));
+}
+
+template <typename T>
+void run_to_epochs(int64_t n, const int64_t* rp, const int32_t* idx, const T* val, int num_epochs, T nsr,
+                   int64_t max_degree, DevBuf<int64_t>& out_rp, DevBuf<int32_t>& out_tg, DevBuf<T>& eps,
+                   DevBuf<T>& nxt, DevBuf<T>& nxt_neg, hipStream_t s) {
+    using S = typename Bits<T>::S;
+    int64_t nnz = 0;
+    UMX_HIP(hipMemcpyAsync(&nnz, rp + n, sizeof(nnz), hipMemcpyDeviceToHost, s));
+    UMX_HIP(hipStreamSynchronize(s));
+    const int G = lanes_for_degree(max_degree);
+    const unsigned grid = grid_for(n * G);
+
+    Tmp<S> maxbits(1, s);
+    UMX_HIP(hipMemsetAsync(maxbits.p, 0, sizeof(S), s));
+    if (nnz > 0) {
+        const int64_t want = (nnz + kBlock - 1) / kBlock;
+        const unsigned g = static_cast<unsigned>(std::min<int64_t>(want, 8192));
+        k_max_weight<T><<<g, kBlock, 0, s>>>(val, nnz, maxbits.p);
+        UMX_HIP(hipGetLastError());
+    }
+    Tmp<int64_t> cnt(static_cast<size_t>(n), s);
+    UMX_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int64_t) * static_cast<size_t>(n), s));
+    k_count_survivors<T><<<grid, kBlock, 0, s>>>(n, G, rp, val, maxbits.p, num_epochs, cnt.p);
+    UMX_HIP(hipGetLastError());
+    out_rp.alloc(static_cast<size_t>(n + 1));
+    DevBuf<int64_t> unused;
+    exclusive_scan_i64(cnt.p, out_rp.p, n, unused, s);
+    int64_t kept = 0;
+    UMX_HIP(hipMemcpyAsync(&kept, out_rp.p + n, sizeof(kept), hipMemcpyDeviceToHost, s));
+    UMX_HIP(hipStreamSynchronize(s));
+    out_tg.alloc(static_cast<size_t>(kept));
+    eps.alloc(static_cast<size_t>(kept));
+    nxt.alloc(static_cast<size_t>(kept));
+    nxt_neg.alloc(static_cast<size_t>(kept));
+    k_fill_epochs<T><<<grid, kBlock, 0, s>>>(n, G, rp, idx, val, maxbits.p, num_epochs, nsr, out_rp.p, out_tg.p,
+                                             eps.p, nxt.p, nxt_neg.p);
+    UMX_HIP(hipGetLastError());
+    UMX_HIP(hipStreamSynchronize(s));
+}
+
+#define UMX_INST(T)                                                                                              \
+    template void launch_similarities<T>(int64_t, const int64_t*, T*, double, double, double, int, double*,     \
+                                         double*, int64_t, hipStream_t);                                         \
+    template void run_combine<T>(int64_t, const int64_t*, const int32_t*, const T*, T, int64_t, DevBuf<int64_t>&, \
+                                 DevBuf<int32_t>&, DevBuf<T>&, hipStream_t);                                     \
+    template void run_to_epochs<T>(int64_t, const int64_t*, const int32_t*, const T*, int, T, int64_t,           \
+                                   DevBuf<int64_t>&, DevBuf<int32_t>&, DevBuf<T>&, DevBuf<T>&, DevBuf<T>&,       \
+                                   hipStream_t);
+UMX_INST(float)
+UMX_INST(double)
+#undef UMX_INST
+
+}  // namespace umx

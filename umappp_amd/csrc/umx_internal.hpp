@@ -1,0 +1,116 @@
+// Internal declarations shared by the HIP translation units of libumappp_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace umx {
+
+// Error carrying a umx_status code; the C ABI layer turns it into a return
+// value plus a thread-local message.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define UMX_HIP(call)                                                                                    \
+    do {                                                                                                 \
+        hipError_t umx_e_ = (call);                                                                      \
+        if (umx_e_ != hipSuccess)                                                                        \
+            throw ::umx::Error(-3, std::string(#call) + ": " + hipGetErrorString(umx_e_));               \
+    } while (0)
+
+constexpr int kWave = 64;
+
+// Device buffer owned by a handle.
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        UMX_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+        n = count;
+    }
+    void swap(DevBuf& o) {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
+    }
+};
+
+// ---- graph kernels (graph_kernels.hip) ----
+
+// Lanes per row for the row-group kernels: the smallest power of two >= the
+// longest row, between 4 and 64.
+int lanes_for_degree(int64_t max_degree);
+
+// Exclusive scan of 'count' int64 values into out[0..count] (out[count] =
+// total).  Uses 'tmp' scratch that the function sizes itself.
+void exclusive_scan_i64(const int64_t* in, int64_t* out, int64_t count, DevBuf<int64_t>& tmp, hipStream_t s);
+
+// Longest row of a CSR.
+int64_t max_row_length(const int64_t* rp, int64_t n, DevBuf<int64_t>& tmp, hipStream_t s);
+
+template <typename T>
+void launch_similarities(int64_t n, const int64_t* rp, T* val, double lc, double bw, double mkds, int newton,
+                         double* rho, double* sigma, int64_t max_degree, hipStream_t s);
+
+// Symmetrise (combine_neighbor_sets).  Allocates the output CSR.
+template <typename T>
+void run_combine(int64_t n, const int64_t* rp, const int32_t* idx, const T* val, T mix, int64_t max_degree,
+                 DevBuf<int64_t>& out_rp, DevBuf<int32_t>& out_idx, DevBuf<T>& out_val, hipStream_t s);
+
+// similarities_to_epochs.  Allocates the EpochData arrays.
+template <typename T>
+void run_to_epochs(int64_t n, const int64_t* rp, const int32_t* idx, const T* val, int num_epochs, T nsr,
+                   int64_t max_degree, DevBuf<int64_t>& out_rp, DevBuf<int32_t>& out_tg, DevBuf<T>& eps,
+                   DevBuf<T>& nxt, DevBuf<T>& nxt_neg, hipStream_t s);
+
+// ---- layout kernels (layout_kernels.hip) ----
+
+struct EpochArgs {
+    int epoch;        // epoch number n
+    double alpha;     // learning rate of this epoch, already rounded to T by the host
+    uint64_t ekey;    // counter-sampler key of this epoch
+};
+
+template <typename T>
+struct LayoutView {
+    int64_t n;
+    int64_t nnz;
+    const int64_t* rp;
+    const int32_t* tg;
+    const T* eps;
+    T* nxt;
+    T* nxt_neg;
+    T* emb;
+    int dim;
+    T a, b, gamma, nsr;
+    unsigned long long* tally;  // 3 counters per epoch
+};
+
+template <typename T>
+void launch_layout_epoch(const LayoutView<T>& v, const EpochArgs& ea, hipStream_t s);
+
+template <typename T>
+void launch_layout_serial(const LayoutView<T>& v, const EpochArgs& ea, const int32_t* samples, int64_t count,
+                          unsigned long long* cursor, hipStream_t s);
+
+// Counter sampler key of one epoch (host side; the device derives per-edge
+// and per-draw keys from it).
+uint64_t counter_epoch_key(uint64_t seed, int epoch);
+
+}  // namespace umx
