@@ -144,6 +144,16 @@ int umx_layout_run_serial(umx_layout* L, int epoch_limit, const int32_t* samples
 /* Per-epoch tallies since creation: out[3*e + {0,1,2}] = firing edges, negative
  * draws, self-hits of epoch e; cap = number of int64 slots. */
 int umx_layout_tallies(const umx_layout* L, int64_t* out, int64_t cap);
+/* Update trace (parity mode).  After umx_layout_set_trace(L, m) with m > 0,
+ * every attractive or repulsive step of the next run / run_serial calls is
+ * recorded as 3 + 3*num_dim doubles: kind (0 attractive, 1 repulsive), head,
+ * other, the head's and the other point's coordinates before the step, and
+ * the head's move (the tail of an attractive step moves by its negative).
+ * umx_layout_get_trace copies up to cap_records records (the first m taken in
+ * order of recording), reports how many steps were recorded in total, and
+ * resets the count.  m = 0 disables tracing. */
+int umx_layout_set_trace(umx_layout* L, int64_t max_records);
+int umx_layout_get_trace(umx_layout* L, double* out, int64_t cap_records, int64_t* total_records);
 /* HIP-event timing of the epoch kernels (on the handle's stream). */
 int umx_layout_set_timing(umx_layout* L, int on);
 int umx_layout_timing(const umx_layout* L, double* kernel_ms, int64_t* launches);

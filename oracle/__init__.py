@@ -58,6 +58,11 @@ def lib():
             "orc_layout_run": (C.c_int, [_vp, C.c_int, _vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
                                          C.c_int]),
             "orc_layout_state": (C.c_int, [_vp, _vp, _vp, C.POINTER(C.c_int), _vp, C.c_int64]),
+            "orc_layout_set_trace": (C.c_int, [_vp, _vp, C.c_int64]),
+            "orc_layout_trace_len": (C.c_int64, [_vp]),
+            "orc_step_delta": (C.c_int, [C.c_int, C.c_int, C.c_int, _vp, _vp, C.c_double, C.c_double, C.c_double,
+                                         C.c_double, _vp]),
+            "orc_epoch_alpha": (C.c_double, [C.c_int, C.c_double, C.c_int, C.c_int]),
             "orc_counter_draw": (C.c_uint32, [C.c_uint64, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32]),
             "orc_mt19937_64_nth": (C.c_uint64, [C.c_uint64, C.c_int64]),
             "orc_discrete_uniform_stream": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, _vp]),
@@ -182,6 +187,17 @@ class Layout:
         if rc != 0:
             raise ValueError(f"oracle layout run failed ({rc})")
 
+    def trace(self, cap):
+        """Record (kind, head, other) of every step of the next runs."""
+        self._trace = np.zeros((int(cap), 3), dtype=np.int64)
+        lib().orc_layout_set_trace(self._h, _p(self._trace), int(cap))
+
+    def traced(self):
+        m = lib().orc_layout_trace_len(self._h)
+        if m > self._trace.shape[0]:
+            raise ValueError("oracle trace overflow")
+        return self._trace[:m].copy()
+
     def state(self):
         nx = np.empty(max(self.E, 1), dtype=self.dtype)
         nn = np.empty(max(self.E, 1), dtype=self.dtype)
@@ -189,6 +205,22 @@ class Layout:
         tal = np.zeros(3 * self.total_epochs, dtype=np.int64)
         lib().orc_layout_state(self._h, _p(nx), _p(nn), C.byref(cur), _p(tal), tal.size)
         return nx[: self.E].copy(), nn[: self.E].copy(), cur.value, tal.reshape(-1, 3)[: cur.value].copy()
+
+
+def step_delta(kind, head, other, a, b, gamma, alpha, dtype):
+    """The head's move for one step (kind 0 attractive, 1 repulsive) from the
+    two points' coordinates before it, in the reference's arithmetic."""
+    dt = np.dtype(dtype)
+    h = np.ascontiguousarray(head, dtype=dt)
+    o = np.ascontiguousarray(other, dtype=dt)
+    out = np.empty(h.size, dtype=dt)
+    lib().orc_step_delta(_dt(dt), int(kind), h.size, _p(h), _p(o), float(a), float(b), float(gamma), float(alpha),
+                         _p(out))
+    return out
+
+
+def epoch_alpha(alpha0, n, total, dtype):
+    return float(lib().orc_epoch_alpha(_dt(dtype), float(alpha0), int(n), int(total)))
 
 
 def counter_draw(seed, epoch, edge, p, n):

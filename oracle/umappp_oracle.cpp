@@ -336,6 +336,17 @@ struct Layout {
     int64_t rec_cap = 0, rec_len = 0;
     // per-epoch tallies: firing edges, draws, self-hits
     std::vector<int64_t> fires, draws, selfhits;
+    // optional update trace: (kind 0 = attractive / 1 = repulsive, head, other)
+    int64_t* trace = nullptr;
+    int64_t trace_cap = 0, trace_len = 0;
+    void note(int kind, int64_t head, int64_t other) {
+        if (trace && trace_len < trace_cap) {
+            trace[3 * trace_len] = kind;
+            trace[3 * trace_len + 1] = head;
+            trace[3 * trace_len + 2] = other;
+        }
+        ++trace_len;
+    }
 
     uint64_t ek = 0;
     int32_t draw(int64_t edge, uint32_t p) {
@@ -393,6 +404,7 @@ void run_serial(Layout<T>& L, int D, T* emb, T a, T b, T gamma, T alpha0, int li
                 if (L.nxt[j] > ep) continue;
                 ++f;
                 T* right = emb + static_cast<int64_t>(L.tg[j]) * D;
+                L.note(0, i, L.tg[j]);
                 const T gc = attract_coef<T>(sqdist<T>(left, right, D), a, b);
                 for (int d = 0; d < D; ++d) {
                     const T g = alpha * clip4<T>(gc * (left[d] - right[d]));
@@ -408,6 +420,7 @@ void run_serial(Layout<T>& L, int D, T* emb, T a, T b, T gamma, T alpha0, int li
                         ++sh;
                         continue;
                     }
+                    L.note(1, i, s);
                     const T* r2 = emb + static_cast<int64_t>(s) * D;
                     const T rc = repulse_coef<T>(sqdist<T>(left, r2, D), a, b, gamma);
                     for (int d = 0; d < D; ++d) left[d] += alpha * clip4<T>(rc * (left[d] - r2[d]));
@@ -684,6 +697,15 @@ Layout<T>* make_layout(int64_t n, const int64_t* rp, const int32_t* tg, const T*
     return L;
 }
 
+// One update's move of the head, as a pure function of the two points'
+// coordinates before the step (optimize_layout.hpp:145-158 and :169-175):
+// kind 0 = attractive (the tail moves by the negative), 1 = repulsive.
+template <typename T>
+void step_delta(int kind, int D, const T* head, const T* other, T a, T b, T gamma, T alpha, T* delta) {
+    const T d2 = sqdist<T>(head, other, D);
+    const T c = kind == 0 ? attract_coef<T>(d2, a, b) : repulse_coef<T>(d2, a, b, gamma);
+    for (int d = 0; d < D; ++d) delta[d] = alpha * clip4<T>(c * (head[d] - other[d]));
+}
 struct OrcLayout {
     int dtype;
     void* impl;
@@ -806,6 +828,38 @@ int orc_layout_state(void* hp, void* nxt, void* nxt_neg, int* current_epoch, int
     if (h->dtype == 0) dump(static_cast<Layout<float>*>(h->impl), static_cast<float*>(nxt), static_cast<float*>(nxt_neg));
     else dump(static_cast<Layout<double>*>(h->impl), static_cast<double*>(nxt), static_cast<double*>(nxt_neg));
     return 0;
+}
+
+// Serial-run trace of (kind, head, other) triples; returns via orc_layout_trace_len.
+int orc_layout_set_trace(void* hp, int64_t* buf, int64_t cap) {
+    auto* h = static_cast<OrcLayout*>(hp);
+    if (h->dtype == 0) {
+        auto* L = static_cast<Layout<float>*>(h->impl);
+        L->trace = buf, L->trace_cap = cap, L->trace_len = 0;
+    } else {
+        auto* L = static_cast<Layout<double>*>(h->impl);
+        L->trace = buf, L->trace_cap = cap, L->trace_len = 0;
+    }
+    return 0;
+}
+int64_t orc_layout_trace_len(void* hp) {
+    auto* h = static_cast<OrcLayout*>(hp);
+    return h->dtype == 0 ? static_cast<Layout<float>*>(h->impl)->trace_len : static_cast<Layout<double>*>(h->impl)->trace_len;
+}
+
+int orc_step_delta(int dtype, int kind, int D, const void* head, const void* other, double a, double b, double gamma,
+                   double alpha, void* delta) {
+    if (dtype == 0)
+        step_delta<float>(kind, D, static_cast<const float*>(head), static_cast<const float*>(other), static_cast<float>(a),
+                          static_cast<float>(b), static_cast<float>(gamma), static_cast<float>(alpha), static_cast<float*>(delta));
+    else
+        step_delta<double>(kind, D, static_cast<const double*>(head), static_cast<const double*>(other), a, b, gamma, alpha,
+                           static_cast<double*>(delta));
+    return 0;
+}
+double orc_epoch_alpha(int dtype, double alpha0, int n, int total) {
+    return dtype == 0 ? static_cast<double>(epoch_alpha<float>(static_cast<float>(alpha0), n, total))
+                      : epoch_alpha<double>(alpha0, n, total);
 }
 
 // Counter sampler exposed for known-answer tests.

@@ -86,6 +86,8 @@ def _declare(L):
         "umx_layout_run": (C.c_int, [_vp, C.c_int]),
         "umx_layout_run_serial": (C.c_int, [_vp, C.c_int, _vp, C.c_int64]),
         "umx_layout_tallies": (C.c_int, [_vp, _vp, C.c_int64]),
+        "umx_layout_set_trace": (C.c_int, [_vp, C.c_int64]),
+        "umx_layout_get_trace": (C.c_int, [_vp, _vp, C.c_int64, _i64p]),
         "umx_layout_set_timing": (C.c_int, [_vp, C.c_int]),
         "umx_layout_timing": (C.c_int, [_vp, _f64p, _i64p]),
         "umx_layout_destroy": (None, [_vp]),
@@ -339,6 +341,21 @@ class Layout:
         out = np.zeros(3 * te, dtype=np.int64)
         _check(lib().umx_layout_tallies(self._h, _ptr(out), out.size))
         return out.reshape(te, 3)
+
+    def set_trace(self, max_records: int):
+        _check(lib().umx_layout_set_trace(self._h, int(max_records)))
+        self._trace_cap = int(max_records)
+
+    def get_trace(self) -> np.ndarray:
+        """Recorded steps since the last call, as an (m, 3 + 3*num_dim) array:
+        kind, head, other, head before, other before, head move."""
+        width = 3 + 3 * self.num_dim
+        out = np.zeros((self._trace_cap, width))
+        total = C.c_int64()
+        _check(lib().umx_layout_get_trace(self._h, _ptr(out), self._trace_cap, C.byref(total)))
+        if total.value > self._trace_cap:
+            raise UmxError(-6, f"trace overflow: {total.value} steps, capacity {self._trace_cap}")
+        return out[: total.value]
 
     def set_timing(self, on: bool):
         _check(lib().umx_layout_set_timing(self._h, 1 if on else 0))

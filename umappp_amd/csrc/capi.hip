@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -291,6 +292,9 @@ struct umx_layout {
     double a = 0, b = 0, gamma = 1, alpha0 = 1;
     uint64_t seed = 0;
     bool params = false, have_emb = false;
+    DevBuf<double> trace;                 // update trace records (parity mode)
+    DevBuf<unsigned long long> trace_len;
+    int64_t trace_cap = 0;
     bool timing = false;
     double kernel_ms = 0;
     int64_t launches = 0;
@@ -323,6 +327,9 @@ umx::LayoutView<T> view_of(umx_layout* L, DevBuf<T>& eps, DevBuf<T>& nxt, DevBuf
     v.gamma = static_cast<T>(L->gamma);
     v.nsr = static_cast<T>(L->nsr);
     v.tally = L->tally.p;
+    v.trace = L->trace_cap > 0 ? L->trace.p : nullptr;
+    v.trace_cap = L->trace_cap;
+    v.trace_len = L->trace_len.p;
     return v;
 }
 
@@ -557,6 +564,41 @@ int umx_layout_tallies(const umx_layout* L, int64_t* out, int64_t cap) {
         require(L != nullptr && out != nullptr, UMX_ERR_ARGUMENT, "NULL argument");
         const int64_t m = std::min<int64_t>(cap, static_cast<int64_t>(L->tally.n));
         if (m > 0) UMX_HIP(hipMemcpyAsync(out, L->tally.p, sizeof(int64_t) * static_cast<size_t>(m), hipMemcpyDeviceToHost, L->s));
+        UMX_HIP(hipStreamSynchronize(L->s));
+    });
+}
+
+int umx_layout_set_trace(umx_layout* L, int64_t max_records) {
+    return guard([&] {
+        require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
+        require(max_records >= 0, UMX_ERR_ARGUMENT, "max_records must be >= 0");
+        require(max_records == 0 || L->params, UMX_ERR_STATE, "umx_layout_set_params must be called first");
+        if (max_records == 0) {
+            L->trace.release();
+            L->trace_cap = 0;
+            return;
+        }
+        L->trace.alloc(static_cast<size_t>(max_records) * static_cast<size_t>(3 + 3 * L->dim));
+        L->trace_len.alloc(1);
+        UMX_HIP(hipMemsetAsync(L->trace_len.p, 0, sizeof(unsigned long long), L->s));
+        UMX_HIP(hipStreamSynchronize(L->s));
+        L->trace_cap = max_records;
+    });
+}
+
+int umx_layout_get_trace(umx_layout* L, double* out, int64_t cap_records, int64_t* total_records) {
+    return guard([&] {
+        require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
+        require(L->trace_cap > 0, UMX_ERR_STATE, "tracing is not enabled");
+        unsigned long long len = 0;
+        UMX_HIP(hipMemcpyAsync(&len, L->trace_len.p, sizeof(len), hipMemcpyDeviceToHost, L->s));
+        UMX_HIP(hipStreamSynchronize(L->s));
+        if (total_records) *total_records = static_cast<int64_t>(len);
+        const int64_t m = std::min<int64_t>({static_cast<int64_t>(len), L->trace_cap, cap_records});
+        const size_t rec = static_cast<size_t>(3 + 3 * L->dim);
+        if (out && m > 0)
+            UMX_HIP(hipMemcpyAsync(out, L->trace.p, sizeof(double) * rec * static_cast<size_t>(m), hipMemcpyDeviceToHost, L->s));
+        UMX_HIP(hipMemsetAsync(L->trace_len.p, 0, sizeof(unsigned long long), L->s));
         UMX_HIP(hipStreamSynchronize(L->s));
     });
 }

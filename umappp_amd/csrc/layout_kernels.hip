@@ -107,12 +107,25 @@ __device__ inline void tally_add(unsigned long long* t, unsigned long long fires
     }
 }
 
+// Reserves one trace record; nullptr when tracing is off or the buffer is full.
+template <typename T>
+__device__ inline double* trace_slot(const LayoutView<T>& v, int kind, int64_t head, int64_t other) {
+    if (!v.trace) return nullptr;
+    const unsigned long long slot = atomicAdd(v.trace_len, 1ULL);
+    if (slot >= static_cast<unsigned long long>(v.trace_cap)) return nullptr;
+    double* rec = v.trace + slot * (3 + 3 * static_cast<long long>(v.dim));
+    rec[0] = kind;
+    rec[1] = static_cast<double>(head);
+    rec[2] = static_cast<double>(other);
+    return rec;
+}
+
 struct EpochConst {
     int epoch;
     uint64_t ekey;
 };
 
-template <typename T, int D>
+template <typename T, int D, bool TRACE>
 __global__ void __launch_bounds__(kBlock) k_layout_rows(LayoutView<T> v, EpochConst ec, T ep, T alpha) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     unsigned long long fires = 0, draws = 0, self_hits = 0;
@@ -138,9 +151,15 @@ __global__ void __launch_bounds__(kBlock) k_layout_rows(LayoutView<T> v, EpochCo
             d2 = d2 > eps_of<T>() ? d2 : eps_of<T>();
             const T pd2b = fast_pow(d2, v.b);
             const T gc = (two_ab * pd2b) / (d2 * (v.a * pd2b + 1));
+            double* rec = TRACE ? trace_slot(v, 0, i, t) : nullptr;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 const T g = alpha * clip4(gc * (h.c[d] - r.c[d]));
+                if (TRACE && rec) {
+                    rec[3 + d] = h.c[d];
+                    rec[3 + D + d] = r.c[d];
+                    rec[3 + 2 * D + d] = g;
+                }
                 h.c[d] += g;
                 add_to(v.emb + static_cast<int64_t>(t) * D + d, -g);
             }
@@ -166,8 +185,17 @@ __global__ void __launch_bounds__(kBlock) k_layout_rows(LayoutView<T> v, EpochCo
                 }
                 e2 = e2 > eps_of<T>() ? e2 : eps_of<T>();
                 const T rc = two_gb / ((static_cast<T>(0.001) + e2) * (v.a * fast_pow(e2, v.b) + 1));
+                double* rec = TRACE ? trace_slot(v, 1, i, k) : nullptr;
 #pragma unroll
-                for (int d = 0; d < D; ++d) h.c[d] += alpha * clip4(rc * (h.c[d] - q.c[d]));
+                for (int d = 0; d < D; ++d) {
+                    const T g = alpha * clip4(rc * (h.c[d] - q.c[d]));
+                    if (TRACE && rec) {
+                        rec[3 + d] = h.c[d];
+                        rec[3 + D + d] = q.c[d];
+                        rec[3 + 2 * D + d] = g;
+                    }
+                    h.c[d] += g;
+                }
             }
             v.nxt[j] = nx + ej_eps;
             v.nxt_neg[j] = nn + static_cast<T>(nneg) * epns;
@@ -261,8 +289,14 @@ __global__ void k_layout_serial(LayoutView<T> v, EpochConst ec, T ep, T alpha, c
             const T pd2b = pow(d2, v.b);
             const T gc = static_cast<T>((-2 * v.a * v.b * pd2b) /
                                         (static_cast<double>(d2) * (static_cast<double>(v.a * pd2b) + 1.0)));
+            double* rec = trace_slot(v, 0, i, v.tg[j]);
             for (int d = 0; d < D; ++d) {
                 const T g = alpha * clip4(gc * (left[d] - right[d]));
+                if (rec) {
+                    rec[3 + d] = left[d];
+                    rec[3 + D + d] = right[d];
+                    rec[3 + 2 * D + d] = g;
+                }
                 left[d] += g;
                 right[d] -= g;
             }
@@ -301,7 +335,16 @@ __global__ void k_layout_serial(LayoutView<T> v, EpochConst ec, T ep, T alpha, c
                 const T rc = static_cast<T>(static_cast<double>(2 * v.gamma * v.b) /
                                             ((0.001 + static_cast<double>(e2)) *
                                              (static_cast<double>(v.a * pow(e2, v.b)) + 1.0)));
-                for (int d = 0; d < D; ++d) left[d] += alpha * clip4(rc * (left[d] - q[d]));
+                double* rec = trace_slot(v, 1, i, k);
+                for (int d = 0; d < D; ++d) {
+                    const T g = alpha * clip4(rc * (left[d] - q[d]));
+                    if (rec) {
+                        rec[3 + d] = left[d];
+                        rec[3 + D + d] = q[d];
+                        rec[3 + 2 * D + d] = g;
+                    }
+                    left[d] += g;
+                }
             }
             v.nxt[j] += v.eps[j];
             v.nxt_neg[j] += static_cast<T>(nneg) * epns;
@@ -325,12 +368,21 @@ void launch_layout_epoch(const LayoutView<T>& v, const EpochArgs& ea, hipStream_
     const T alpha = static_cast<T>(ea.alpha);
     const int64_t blocks = (v.n + kBlock - 1) / kBlock;
     const unsigned grid = static_cast<unsigned>(blocks < 1 ? 1 : blocks);
-    switch (v.dim) {
-        case 1: k_layout_rows<T, 1><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
-        case 2: k_layout_rows<T, 2><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
-        case 3: k_layout_rows<T, 3><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
-        case 4: k_layout_rows<T, 4><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
-        default: k_layout_rows_generic<T><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+    if (v.trace && v.dim <= 4) {
+        switch (v.dim) {
+            case 1: k_layout_rows<T, 1, true><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+            case 2: k_layout_rows<T, 2, true><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+            case 3: k_layout_rows<T, 3, true><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+            default: k_layout_rows<T, 4, true><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+        }
+    } else {
+        switch (v.dim) {
+            case 1: k_layout_rows<T, 1, false><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+            case 2: k_layout_rows<T, 2, false><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+            case 3: k_layout_rows<T, 3, false><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+            case 4: k_layout_rows<T, 4, false><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+            default: k_layout_rows_generic<T><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+        }
     }
     UMX_HIP(hipGetLastError());
 }

@@ -100,6 +100,11 @@ struct LayoutView {
     int dim;
     T a, b, gamma, nsr;
     unsigned long long* tally;  // 3 counters per epoch
+    // Optional update trace (parity mode): records of 3 + 3 * dim doubles
+    // (kind, head, other, head before, other before, head move).
+    double* trace;
+    long long trace_cap;
+    unsigned long long* trace_len;
 };
 
 template <typename T>
