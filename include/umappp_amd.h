@@ -136,6 +136,14 @@ int umx_layout_get_embedding(const umx_layout* L, void* embedding, int device_pt
  * parallel (Hogwild) kernel — replaces optimize_layout /
  * optimize_layout_parallel as dispatched by Status::run (Status.hpp:101-128). */
 int umx_layout_run(umx_layout* L, int epoch_limit);
+/* Update scheme of umx_layout_run.  UMX_LAYOUT_PULL (default): every
+ * observation's owner lane applies all of its moves — its head steps, the
+ * tail steps of the reverse edges (a stored edge and its reverse fire in the
+ * same epochs) and its negative samples — and writes the observation once per
+ * epoch; no atomics.  UMX_LAYOUT_PUSH: each head step also moves the tail
+ * with float atomics, as the reference's loop does. */
+enum umx_layout_mode { UMX_LAYOUT_PULL = 0, UMX_LAYOUT_PUSH = 1 };
+int umx_layout_set_mode(umx_layout* L, int mode);
 /* Deterministic one-thread replay of optimize_layout (optimize_layout.hpp:
  * 116-185) in the reference's edge order and mixed precision.  If 'samples' is
  * non-NULL, negative samples are taken from it in draw order (count entries);
@@ -146,9 +154,12 @@ int umx_layout_run_serial(umx_layout* L, int epoch_limit, const int32_t* samples
 int umx_layout_tallies(const umx_layout* L, int64_t* out, int64_t cap);
 /* Update trace (parity mode).  After umx_layout_set_trace(L, m) with m > 0,
  * every attractive or repulsive step of the next run / run_serial calls is
- * recorded as 3 + 3*num_dim doubles: kind (0 attractive, 1 repulsive), head,
- * other, the head's and the other point's coordinates before the step, and
- * the head's move (the tail of an attractive step moves by its negative).
+ * recorded as 3 + 3*num_dim doubles: kind, head, other, the head's and the
+ * other point's coordinates before the step, and the head's move.  Kind 0 is
+ * an attractive step of edge head->other (in push mode and in the serial
+ * replay the tail moves by the negative of the head's move), kind 1 a
+ * repulsive step, kind 2 (pull mode) the move that edge other->head's
+ * attractive step gives its tail, 'head'.
  * umx_layout_get_trace copies up to cap_records records (the first m taken in
  * order of recording), reports how many steps were recorded in total, and
  * resets the count.  m = 0 disables tracing. */

@@ -161,13 +161,16 @@ def test_serial_replay_injected_mt_samples(umx, dtype, rtol, atol):
         lay2.run_serial(2, samples=samples[:10])
 
 
+@pytest.mark.parametrize("mode", ["pull", "push"])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("dim", [2, 3])
-def test_hogwild_per_step(umx, dtype, dim):
-    """Production (Hogwild) kernel on the config C1 graph: per epoch, the
-    multiset of steps equals the oracle's, each traced step equals the
+def test_hogwild_per_step(umx, dtype, dim, mode):
+    """Production kernels on the config C1 graph.  Per epoch the steps are
+    exactly the oracle's (as a multiset), each traced step equals the
     reference's update from its traced inputs within 1e-5 relative, and no
-    step is lost: the embedding moves by exactly the traced steps' sum."""
+    step is lost: the embedding moves by exactly the traced steps' sum.
+    Push mode records kind 0 (head and tail move) and kind 1; pull mode also
+    records kind 2, the tail move that the reverse edge gives each head."""
     _, (rp, idx, dist) = c1_knn()
     wd, _, _ = oracle.neighbor_similarities(rp, dist)
     crp, cidx, cw = oracle.combine_neighbor_sets(rp, idx, wd, 1.0)
@@ -176,30 +179,76 @@ def test_hogwild_per_step(umx, dtype, dim):
     lay, orc = _pair(umx, crp, cidx, cw, T, nsr, seed, dtype)
     emb = np.random.default_rng(1).uniform(-10, 10, size=(2000, dim)).astype(dtype)
     lay.set_params(dim, A, B, 1.0, 1.0, seed)
+    lay.set_mode(umx.Layout.PULL if mode == "pull" else umx.Layout.PUSH)
     lay.set_embedding(emb)
-    lay.set_trace(400_000)
-    orc.trace(400_000)
+    lay.set_trace(600_000)
+    orc.trace(600_000)
     eo = emb.copy()
+    key = lambda a: a[np.lexsort(tuple(a[:, c] for c in range(a.shape[1] - 1, -1, -1)))]
     for limit in range(1, 7):
         before = lay.get_embedding().astype(np.float64)
         lay.run(limit)
         tr = lay.get_trace()
         orc.run(eo, A, B, epoch_limit=limit)
         seq = orc.traced()
-        orc.trace(400_000)
-        key = lambda a: a[np.lexsort((a[:, 2], a[:, 1], a[:, 0]))]
-        assert np.array_equal(key(tr[:, :3].astype(np.int64)), key(seq))
-        _check_steps(tr, [tr.shape[0]], dim, dtype, A, B, 1.0, T, limit - 1, 1e-5, 1e-6)
+        orc.trace(600_000)
+        kinds = tr[:, 0].astype(np.int64)
+        trip = tr[:, :3].astype(np.int64)
+        assert np.array_equal(key(trip[kinds != 2]), key(seq))
+        alpha = oracle.epoch_alpha(1.0, limit - 1, T, dtype)
+        if mode == "pull":
+            att = seq[seq[:, 0] == 0]
+            rev = trip[kinds == 2][:, [1, 2]]
+            assert np.array_equal(key(rev), key(att[:, [2, 1]]))
+            # kind 2: i's move is the tail move of edge t->i, i.e. minus the
+            # head move of t evaluated at (y_t, y_i)
+            for r in tr[kinds == 2][::5]:
+                h = r[3:3 + dim].astype(dtype)
+                o = r[3 + dim:3 + 2 * dim].astype(dtype)
+                ref = -oracle.step_delta(0, o, h, A, B, 1.0, alpha, dtype).astype(np.float64)
+                got = r[3 + 2 * dim:]
+                assert np.all(np.abs(got - ref) <= 1e-5 * np.abs(ref) + 1e-6)
+        _check_steps(tr[kinds != 2], [int((kinds != 2).sum())], dim, dtype, A, B, 1.0, T, limit - 1, 1e-5, 1e-6)
         moved = np.zeros_like(before)
         heads = tr[:, 1].astype(np.int64)
         others = tr[:, 2].astype(np.int64)
         d = tr[:, 3 + 2 * dim:]
         np.add.at(moved, heads, d)
-        att = tr[:, 0] == 0
-        np.add.at(moved, others[att], -d[att])
+        if mode == "push":
+            att = tr[:, 0] == 0
+            np.add.at(moved, others[att], -d[att])
         after = lay.get_embedding().astype(np.float64)
         tol = (1e-4 if dtype == np.float32 else 1e-10) * max(1.0, np.max(np.abs(before)))
         assert np.max(np.abs(after - (before + moved))) <= tol
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_pull_and_push_quality_c1(umx, dtype):
+    """Both update schemes reach the reference path's final quality on C1."""
+    xh, (rp, idx, dist) = c1_knn()
+    a, b = umx.find_ab(1.0, 0.1)
+    wd, _, _ = oracle.neighbor_similarities(rp, dist)
+    crp, cidx, cw = oracle.combine_neighbor_sets(rp, idx, wd, 1.0)
+    cw = cw.astype(dtype)
+    emb0 = np.random.default_rng(42).uniform(-10, 10, size=(2000, 2)).astype(dtype)
+    hi = idx.reshape(2000, 15)
+    eo = emb0.copy()
+    ep = oracle.similarities_to_epochs(crp, cidx, cw, 500, 5.0)
+    oracle.Layout(ep, 500, 5.0, rng_kind=oracle.RNG_MT, seed=1234567890, dtype=dtype).run(eo, a, b)
+    p_cpu = knn_preservation(xh, eo, 15, high_idx=hi)
+    t_cpu = trustworthiness(xh, eo, 15)
+    for mode in (umx.Layout.PULL, umx.Layout.PUSH):
+        lay = umx.Layout(csr=umx.NeighborList(crp, cidx, cw), num_epochs=500, negative_sample_rate=5.0)
+        lay.set_params(2, a, b, 1.0, 1.0, 1234567890)
+        lay.set_mode(mode)
+        lay.set_embedding(emb0)
+        lay.run(500)
+        eg = lay.get_embedding()
+        p_gpu = knn_preservation(xh, eg, 15, high_idx=hi)
+        t_gpu = trustworthiness(xh, eg, 15)
+        print(f"C1 {np.dtype(dtype).name} mode {mode}: preservation cpu {p_cpu:.4f} gpu {p_gpu:.4f}; "
+              f"trustworthiness cpu {t_cpu:.4f} gpu {t_gpu:.4f}")
+        assert abs(p_cpu - p_gpu) <= 0.01 and abs(t_cpu - t_gpu) <= 0.01
 
 
 def test_serial_replay_restart_bit_exact(umx):

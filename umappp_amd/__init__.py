@@ -86,6 +86,7 @@ def _declare(L):
         "umx_layout_run": (C.c_int, [_vp, C.c_int]),
         "umx_layout_run_serial": (C.c_int, [_vp, C.c_int, _vp, C.c_int64]),
         "umx_layout_tallies": (C.c_int, [_vp, _vp, C.c_int64]),
+        "umx_layout_set_mode": (C.c_int, [_vp, C.c_int]),
         "umx_layout_set_trace": (C.c_int, [_vp, C.c_int64]),
         "umx_layout_get_trace": (C.c_int, [_vp, _vp, C.c_int64, _i64p]),
         "umx_layout_set_timing": (C.c_int, [_vp, C.c_int]),
@@ -341,6 +342,12 @@ class Layout:
         out = np.zeros(3 * te, dtype=np.int64)
         _check(lib().umx_layout_tallies(self._h, _ptr(out), out.size))
         return out.reshape(te, 3)
+
+    PULL, PUSH = 0, 1
+
+    def set_mode(self, mode: int):
+        """Layout.PULL (default, no atomics) or Layout.PUSH (tails via atomics)."""
+        _check(lib().umx_layout_set_mode(self._h, int(mode)))
 
     def set_trace(self, max_records: int):
         _check(lib().umx_layout_set_trace(self._h, int(max_records)))

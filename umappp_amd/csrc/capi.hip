@@ -289,6 +289,7 @@ struct umx_layout {
     DevBuf<unsigned long long> tally;
     DevBuf<unsigned long long> cursor;  // serial replay cursor + overrun flag
     int dim = 0;
+    int mode = UMX_LAYOUT_PULL;
     double a = 0, b = 0, gamma = 1, alpha0 = 1;
     uint64_t seed = 0;
     bool params = false, have_emb = false;
@@ -322,6 +323,7 @@ umx::LayoutView<T> view_of(umx_layout* L, DevBuf<T>& eps, DevBuf<T>& nxt, DevBuf
     v.nxt_neg = nng.p;
     v.emb = emb.p;
     v.dim = L->dim;
+    v.mode = L->mode;
     v.a = static_cast<T>(L->a);
     v.b = static_cast<T>(L->b);
     v.gamma = static_cast<T>(L->gamma);
@@ -565,6 +567,14 @@ int umx_layout_tallies(const umx_layout* L, int64_t* out, int64_t cap) {
         const int64_t m = std::min<int64_t>(cap, static_cast<int64_t>(L->tally.n));
         if (m > 0) UMX_HIP(hipMemcpyAsync(out, L->tally.p, sizeof(int64_t) * static_cast<size_t>(m), hipMemcpyDeviceToHost, L->s));
         UMX_HIP(hipStreamSynchronize(L->s));
+    });
+}
+
+int umx_layout_set_mode(umx_layout* L, int mode) {
+    return guard([&] {
+        require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
+        require(mode == UMX_LAYOUT_PULL || mode == UMX_LAYOUT_PUSH, UMX_ERR_ARGUMENT, "unknown layout mode");
+        L->mode = mode;
     });
 }
 

@@ -357,6 +357,175 @@ __global__ void k_layout_serial(LayoutView<T> v, EpochConst ec, T ep, T alpha, c
     t[2] += self_hits;
 }
 
+template <typename T, int D>
+__device__ inline void store_point(T* emb, int64_t i, const Vec<T, D>& v) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) emb[i * D + d] = v.c[d];
+}
+template <>
+__device__ inline void store_point<float, 2>(float* emb, int64_t i, const Vec<float, 2>& v) {
+    *reinterpret_cast<float2*>(emb + i * 2) = make_float2(v.c[0], v.c[1]);
+}
+template <>
+__device__ inline void store_point<float, 4>(float* emb, int64_t i, const Vec<float, 4>& v) {
+    *reinterpret_cast<float4*>(emb + i * 4) = make_float4(v.c[0], v.c[1], v.c[2], v.c[3]);
+}
+
+template <typename T, int D>
+__device__ inline T sqdist(const Vec<T, D>& x, const Vec<T, D>& y) {
+    T s = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const T q = x.c[d] - y.c[d];
+        s += q * q;
+    }
+    return s > eps_of<T>() ? s : eps_of<T>();
+}
+
+constexpr int kNegBatch = 4;  // negative samples whose loads are issued together
+
+// Pull-mode epoch: the owner lane of observation i applies every move of i.
+// A stored edge i->t fires in exactly the same epochs as its reverse t->i
+// (the fuzzy union makes weights, hence epochs_per_sample, bit-symmetric), so
+// for each firing edge the lane applies i's head step of i->t and then the
+// tail step that t->i gives i (the reference computes it as the negative of
+// t's head move, -alpha*clip(c*(y_t - y_i)) = alpha*clip(c*(y_i - y_t))),
+// followed by i's negative samples.  Only the owner writes i, once, at the
+// end of its row: the epoch is a pure gather with no atomics.
+template <typename T, int D, bool TRACE>
+__global__ void __launch_bounds__(kBlock) k_layout_pull(LayoutView<T> v, EpochConst ec, T ep, T alpha) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    unsigned long long fires = 0, draws = 0, self_hits = 0;
+    if (i < v.n) {
+        const int64_t s = v.rp[i], e = v.rp[i + 1];
+        Vec<T, D> h = load_point<T, D>(v.emb, i);
+        const T two_ab = -2 * v.a * v.b;
+        const T two_gb = 2 * v.gamma * v.b;
+        const uint32_t n32 = static_cast<uint32_t>(v.n);
+        bool moved = false;
+        for (int64_t j = s; j < e; ++j) {
+            const T nx = v.nxt[j];
+            if (nx > ep) continue;
+            ++fires;
+            moved = true;
+            const int32_t t = v.tg[j];
+            const T ej_eps = v.eps[j];
+            const T nn = v.nxt_neg[j];
+            const Vec<T, D> r = load_point<T, D>(v.emb, t);
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                const T d2 = sqdist<T, D>(h, r);
+                const T pd2b = fast_pow(d2, v.b);
+                const T gc = (two_ab * pd2b) / (d2 * (v.a * pd2b + 1));
+                double* rec = TRACE ? trace_slot(v, half == 0 ? 0 : 2, i, t) : nullptr;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const T g = alpha * clip4(gc * (h.c[d] - r.c[d]));
+                    if (TRACE && rec) {
+                        rec[3 + d] = h.c[d];
+                        rec[3 + D + d] = r.c[d];
+                        rec[3 + 2 * D + d] = g;
+                    }
+                    h.c[d] += g;
+                }
+            }
+
+            const T epns = ej_eps / v.nsr;
+            const int nneg = static_cast<int>((ep - nn) / epns);
+            const uint64_t ej = edge_key(ec.ekey, static_cast<uint64_t>(j));
+            for (int p0 = 0; p0 < nneg; p0 += kNegBatch) {
+                int32_t kk[kNegBatch];
+                Vec<T, D> q[kNegBatch];
+#pragma unroll
+                for (int u = 0; u < kNegBatch; ++u) {
+                    kk[u] = p0 + u < nneg ? draw_index(ej, static_cast<uint32_t>(p0 + u), n32) : -1;
+                    q[u] = load_point<T, D>(v.emb, kk[u] >= 0 ? kk[u] : i);
+                }
+#pragma unroll
+                for (int u = 0; u < kNegBatch; ++u) {
+                    if (kk[u] < 0) break;
+                    ++draws;
+                    if (kk[u] == i) {
+                        ++self_hits;
+                        continue;
+                    }
+                    const T e2 = sqdist<T, D>(h, q[u]);
+                    const T rc = two_gb / ((static_cast<T>(0.001) + e2) * (v.a * fast_pow(e2, v.b) + 1));
+                    double* rec = TRACE ? trace_slot(v, 1, i, kk[u]) : nullptr;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        const T g = alpha * clip4(rc * (h.c[d] - q[u].c[d]));
+                        if (TRACE && rec) {
+                            rec[3 + d] = h.c[d];
+                            rec[3 + D + d] = q[u].c[d];
+                            rec[3 + 2 * D + d] = g;
+                        }
+                        h.c[d] += g;
+                    }
+                }
+            }
+            v.nxt[j] = nx + ej_eps;
+            v.nxt_neg[j] = nn + static_cast<T>(nneg) * epns;
+        }
+        if (moved) store_point<T, D>(v.emb, i, h);
+    }
+    tally_add(v.tally + 3 * static_cast<int64_t>(ec.epoch), fires, draws, self_hits);
+}
+
+// Pull mode for any embedding dimension: the owner updates its point in place.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_layout_pull_generic(LayoutView<T> v, EpochConst ec, T ep, T alpha) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    unsigned long long fires = 0, draws = 0, self_hits = 0;
+    if (i < v.n) {
+        const int D = v.dim;
+        const int64_t s = v.rp[i], e = v.rp[i + 1];
+        T* h = v.emb + i * D;
+        const uint32_t n32 = static_cast<uint32_t>(v.n);
+        for (int64_t j = s; j < e; ++j) {
+            const T nx = v.nxt[j];
+            if (nx > ep) continue;
+            ++fires;
+            const T* r = v.emb + static_cast<int64_t>(v.tg[j]) * D;
+            for (int half = 0; half < 2; ++half) {
+                T d2 = 0;
+                for (int d = 0; d < D; ++d) {
+                    const T q = h[d] - r[d];
+                    d2 += q * q;
+                }
+                d2 = d2 > eps_of<T>() ? d2 : eps_of<T>();
+                const T pd2b = fast_pow(d2, v.b);
+                const T gc = (-2 * v.a * v.b * pd2b) / (d2 * (v.a * pd2b + 1));
+                for (int d = 0; d < D; ++d) h[d] += alpha * clip4(gc * (h[d] - r[d]));
+            }
+            const T epns = v.eps[j] / v.nsr;
+            const T nn = v.nxt_neg[j];
+            const int nneg = static_cast<int>((ep - nn) / epns);
+            const uint64_t ej = edge_key(ec.ekey, static_cast<uint64_t>(j));
+            for (int p = 0; p < nneg; ++p) {
+                const int32_t k = draw_index(ej, static_cast<uint32_t>(p), n32);
+                ++draws;
+                if (k == i) {
+                    ++self_hits;
+                    continue;
+                }
+                const T* q = v.emb + static_cast<int64_t>(k) * D;
+                T e2 = 0;
+                for (int d = 0; d < D; ++d) {
+                    const T z = h[d] - q[d];
+                    e2 += z * z;
+                }
+                e2 = e2 > eps_of<T>() ? e2 : eps_of<T>();
+                const T rc = 2 * v.gamma * v.b / ((static_cast<T>(0.001) + e2) * (v.a * fast_pow(e2, v.b) + 1));
+                for (int d = 0; d < D; ++d) h[d] += alpha * clip4(rc * (h[d] - q[d]));
+            }
+            v.nxt[j] = nx + v.eps[j];
+            v.nxt_neg[j] = nn + static_cast<T>(nneg) * epns;
+        }
+    }
+    tally_add(v.tally + 3 * static_cast<int64_t>(ec.epoch), fires, draws, self_hits);
+}
+
 }  // namespace
 
 uint64_t counter_epoch_key(uint64_t seed, int epoch) { return mix64(seed + kGold * (static_cast<uint64_t>(epoch) + 1)); }
@@ -368,20 +537,37 @@ void launch_layout_epoch(const LayoutView<T>& v, const EpochArgs& ea, hipStream_
     const T alpha = static_cast<T>(ea.alpha);
     const int64_t blocks = (v.n + kBlock - 1) / kBlock;
     const unsigned grid = static_cast<unsigned>(blocks < 1 ? 1 : blocks);
-    if (v.trace && v.dim <= 4) {
+    if (v.mode == kModePush) {
+        if (v.trace && v.dim <= 4) {
+            switch (v.dim) {
+                case 1: k_layout_rows<T, 1, true><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+                case 2: k_layout_rows<T, 2, true><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+                case 3: k_layout_rows<T, 3, true><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+                default: k_layout_rows<T, 4, true><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+            }
+        } else {
+            switch (v.dim) {
+                case 1: k_layout_rows<T, 1, false><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+                case 2: k_layout_rows<T, 2, false><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+                case 3: k_layout_rows<T, 3, false><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+                case 4: k_layout_rows<T, 4, false><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+                default: k_layout_rows_generic<T><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+            }
+        }
+    } else if (v.trace && v.dim <= 4) {
         switch (v.dim) {
-            case 1: k_layout_rows<T, 1, true><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
-            case 2: k_layout_rows<T, 2, true><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
-            case 3: k_layout_rows<T, 3, true><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
-            default: k_layout_rows<T, 4, true><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+            case 1: k_layout_pull<T, 1, true><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+            case 2: k_layout_pull<T, 2, true><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+            case 3: k_layout_pull<T, 3, true><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+            default: k_layout_pull<T, 4, true><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
         }
     } else {
         switch (v.dim) {
-            case 1: k_layout_rows<T, 1, false><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
-            case 2: k_layout_rows<T, 2, false><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
-            case 3: k_layout_rows<T, 3, false><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
-            case 4: k_layout_rows<T, 4, false><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
-            default: k_layout_rows_generic<T><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+            case 1: k_layout_pull<T, 1, false><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+            case 2: k_layout_pull<T, 2, false><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+            case 3: k_layout_pull<T, 3, false><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+            case 4: k_layout_pull<T, 4, false><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
+            default: k_layout_pull_generic<T><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
         }
     }
     UMX_HIP(hipGetLastError());

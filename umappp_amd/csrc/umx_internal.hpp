@@ -87,6 +87,11 @@ struct EpochArgs {
     uint64_t ekey;    // counter-sampler key of this epoch
 };
 
+// Layout update modes: pull (default; each observation's owner applies all of
+// its moves, no atomics) and push (tails moved with float atomics).
+constexpr int kModePull = 0;
+constexpr int kModePush = 1;
+
 template <typename T>
 struct LayoutView {
     int64_t n;
@@ -98,6 +103,7 @@ struct LayoutView {
     T* nxt_neg;
     T* emb;
     int dim;
+    int mode;
     T a, b, gamma, nsr;
     unsigned long long* tally;  // 3 counters per epoch
     // Optional update trace (parity mode): records of 3 + 3 * dim doubles
