@@ -222,33 +222,48 @@ def test_hogwild_per_step(umx, dtype, dim, mode):
         assert np.max(np.abs(after - (before + moved))) <= tol
 
 
-@pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_pull_and_push_quality_c1(umx, dtype):
-    """Both update schemes reach the reference path's final quality on C1."""
-    xh, (rp, idx, dist) = c1_knn()
+_MIX = {}
+
+
+def _mixture_graph():
+    """10,000 x 30 float64 Gaussian mixture (10 components), exact kNN k=15,
+    similarities + union from the oracle, float32 weights."""
+    if not _MIX:
+        from _data import knn_lists, mixture
+
+        x, _ = mixture(11, 10000, 30, 10, dtype=np.float64)
+        rp, idx, dist = knn_lists(x, 15)
+        wd, _, _ = oracle.neighbor_similarities(rp, dist)
+        crp, cidx, cw = oracle.combine_neighbor_sets(rp, idx, wd, 1.0)
+        _MIX.update(x=x, idx=idx, g=(crp, cidx, cw.astype(np.float32)))
+    return _MIX
+
+
+@pytest.mark.parametrize("mode", ["pull", "push"])
+def test_quality_mixture_both_modes(umx, mode):
+    """Structured data (the metrics' seed-to-seed spread on the CPU is about
+    0.002): both update schemes within 0.01 of the reference path at the same
+    seed and initial embedding, 500 epochs."""
+    m = _mixture_graph()
+    crp, cidx, cw = m["g"]
     a, b = umx.find_ab(1.0, 0.1)
-    wd, _, _ = oracle.neighbor_similarities(rp, dist)
-    crp, cidx, cw = oracle.combine_neighbor_sets(rp, idx, wd, 1.0)
-    cw = cw.astype(dtype)
-    emb0 = np.random.default_rng(42).uniform(-10, 10, size=(2000, 2)).astype(dtype)
-    hi = idx.reshape(2000, 15)
+    emb0 = np.random.default_rng(42).uniform(-10, 10, size=(10000, 2)).astype(np.float32)
     eo = emb0.copy()
     ep = oracle.similarities_to_epochs(crp, cidx, cw, 500, 5.0)
-    oracle.Layout(ep, 500, 5.0, rng_kind=oracle.RNG_MT, seed=1234567890, dtype=dtype).run(eo, a, b)
-    p_cpu = knn_preservation(xh, eo, 15, high_idx=hi)
-    t_cpu = trustworthiness(xh, eo, 15)
-    for mode in (umx.Layout.PULL, umx.Layout.PUSH):
-        lay = umx.Layout(csr=umx.NeighborList(crp, cidx, cw), num_epochs=500, negative_sample_rate=5.0)
-        lay.set_params(2, a, b, 1.0, 1.0, 1234567890)
-        lay.set_mode(mode)
-        lay.set_embedding(emb0)
-        lay.run(500)
-        eg = lay.get_embedding()
-        p_gpu = knn_preservation(xh, eg, 15, high_idx=hi)
-        t_gpu = trustworthiness(xh, eg, 15)
-        print(f"C1 {np.dtype(dtype).name} mode {mode}: preservation cpu {p_cpu:.4f} gpu {p_gpu:.4f}; "
-              f"trustworthiness cpu {t_cpu:.4f} gpu {t_gpu:.4f}")
-        assert abs(p_cpu - p_gpu) <= 0.01 and abs(t_cpu - t_gpu) <= 0.01
+    oracle.Layout(ep, 500, 5.0, rng_kind=oracle.RNG_MT, seed=1234567890, dtype=np.float32).run(eo, a, b)
+    lay = umx.Layout(csr=umx.NeighborList(crp, cidx, cw), num_epochs=500, negative_sample_rate=5.0)
+    lay.set_params(2, a, b, 1.0, 1.0, 1234567890)
+    lay.set_mode(umx.Layout.PULL if mode == "pull" else umx.Layout.PUSH)
+    lay.set_embedding(emb0)
+    lay.run(500)
+    eg = lay.get_embedding()
+    hi = m["idx"].reshape(-1, 15)
+    q = np.random.default_rng(0).choice(10000, 2000, replace=False)
+    p_cpu, p_gpu = knn_preservation(m["x"], eo, 15, high_idx=hi), knn_preservation(m["x"], eg, 15, high_idx=hi)
+    t_cpu, t_gpu = trustworthiness(m["x"], eo, 15, queries=q), trustworthiness(m["x"], eg, 15, queries=q)
+    print(f"mixture 10k {mode}: preservation cpu {p_cpu:.4f} gpu {p_gpu:.4f}; "
+          f"trustworthiness cpu {t_cpu:.4f} gpu {t_gpu:.4f}")
+    assert abs(p_cpu - p_gpu) <= 0.01 and abs(t_cpu - t_gpu) <= 0.01
 
 
 def test_serial_replay_restart_bit_exact(umx):
@@ -304,7 +319,10 @@ def test_run_argument_errors(umx):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_final_quality_c1_matches_reference_path(umx, dtype):
-    """Config C1 (2000 x 10, k=15, 500 epochs, min_dist 0.1 -> a, b from find_ab)."""
+    """Config C1 (2000 x 10, k=15, 500 epochs, min_dist 0.1 -> a, b from
+    find_ab), production (pull) kernel.  C1 is unstructured N(0,1) data: the
+    reference path's own seed-to-seed spread is about 0.01 in
+    trustworthiness and 0.005 in preservation."""
     xh, (rp, idx, dist) = c1_knn()
     a, b = umx.find_ab(1.0, 0.1)
     T, nsr = 500, 5.0
