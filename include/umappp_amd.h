@@ -144,6 +144,18 @@ int umx_layout_run(umx_layout* L, int epoch_limit);
  * with float atomics, as the reference's loop does. */
 enum umx_layout_mode { UMX_LAYOUT_PULL = 0, UMX_LAYOUT_PUSH = 1 };
 int umx_layout_set_mode(umx_layout* L, int mode);
+/* Load balance of umx_layout_run: one lane runs one work item, normally a
+ * whole row (its steps stay sequential, as in the reference).  Rows expected
+ * to fire more than fires_per_item edges per epoch (sum of
+ * 1 / epochs_per_sample), or longer than edges_per_item, are cut into items
+ * of about equal expected firings whose moves of the head are merged with
+ * atomics; items are run heaviest first.  Defaults 32.0 and 2^31 - 1. */
+int umx_layout_set_work_split(umx_layout* L, double fires_per_item, int edges_per_item);
+/* The current work items (built on demand), in run order: first edge,
+ * length, row (top bit set for rows cut into several items).  *count
+ * receives the number of items; up to cap are copied. */
+int umx_layout_work_items(umx_layout* L, int64_t* begin, int32_t* length, int32_t* row, int64_t cap,
+                          int64_t* count);
 /* Deterministic one-thread replay of optimize_layout (optimize_layout.hpp:
  * 116-185) in the reference's edge order and mixed precision.  If 'samples' is
  * non-NULL, negative samples are taken from it in draw order (count entries);

@@ -303,6 +303,35 @@ def test_status_run_and_restart_schedule(umx):
     assert np.all(np.isfinite(e1)) and np.all(np.isfinite(e2))
 
 
+@pytest.mark.parametrize("fires,cap", [(32.0, 2**31 - 1), (2.0, 2**31 - 1), (4.0, 5), (0.5, 1)])
+def test_work_items_cover_every_edge_once(umx, fires, cap):
+    """Work items partition each row into contiguous runs, respect the split
+    threshold and edge cap, mark split rows, and run heaviest first."""
+    _, (rp, idx, dist) = c1_knn()
+    wd, _, _ = oracle.neighbor_similarities(rp, dist)
+    crp, cidx, cw = oracle.combine_neighbor_sets(rp, idx, wd, 1.0)
+    lay = umx.Layout(csr=umx.NeighborList(crp, cidx, cw.astype(np.float32)), num_epochs=500, negative_sample_rate=5.0)
+    lay.set_work_split(fires, cap)
+    b, ln, row, split = lay.work_items()
+    ed = lay.epoch_data()
+    rpe = ed.cumulative_num_edges
+    assert np.all(ln >= 1) and np.all(ln <= cap)
+    cover = np.zeros(rpe[-1], np.int64)
+    for s_, l_ in zip(b, ln):
+        cover[s_:s_ + l_] += 1
+    assert np.all(cover == 1)
+    assert np.all((b >= rpe[row]) & (b + ln <= rpe[row + 1]))
+    nitems = np.bincount(row, minlength=2000)
+    assert np.array_equal(split, nitems[row] > 1)
+    assert np.array_equal(nitems > 0, np.diff(rpe) > 0)
+    inv = 1.0 / ed.epochs_per_sample.astype(np.float64)
+    work = np.array([inv[s_:s_ + l_].sum() for s_, l_ in zip(b, ln)])
+    lvl = np.where(work * 16 >= 1, 1 + np.floor(np.log2(np.maximum(work * 16, 1e-30))), 0).clip(0, 31)
+    assert np.all(np.diff(lvl) <= 0), "items run heaviest first"
+    rowf = np.add.reduceat(inv, rpe[:-1])[nitems > 1]
+    assert np.all(rowf > fires) or cap < 2**31 - 1
+
+
 def test_run_argument_errors(umx):
     x, rp, idx, w = _graph(50, 5, np.float32)
     lay = umx.Layout(csr=umx.NeighborList(rp, idx, w), num_epochs=10, negative_sample_rate=5.0)

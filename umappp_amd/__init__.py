@@ -87,6 +87,8 @@ def _declare(L):
         "umx_layout_run_serial": (C.c_int, [_vp, C.c_int, _vp, C.c_int64]),
         "umx_layout_tallies": (C.c_int, [_vp, _vp, C.c_int64]),
         "umx_layout_set_mode": (C.c_int, [_vp, C.c_int]),
+        "umx_layout_set_work_split": (C.c_int, [_vp, C.c_double, C.c_int]),
+        "umx_layout_work_items": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int64, _i64p]),
         "umx_layout_set_trace": (C.c_int, [_vp, C.c_int64]),
         "umx_layout_get_trace": (C.c_int, [_vp, _vp, C.c_int64, _i64p]),
         "umx_layout_set_timing": (C.c_int, [_vp, C.c_int]),
@@ -348,6 +350,21 @@ class Layout:
     def set_mode(self, mode: int):
         """Layout.PULL (default, no atomics) or Layout.PUSH (tails via atomics)."""
         _check(lib().umx_layout_set_mode(self._h, int(mode)))
+
+    def set_work_split(self, fires_per_item: float = 32.0, edges_per_item: int = 2**31 - 1):
+        _check(lib().umx_layout_set_work_split(self._h, float(fires_per_item), int(edges_per_item)))
+
+    def work_items(self):
+        """(begin, length, row, split) arrays of the production kernels' work items, in run order."""
+        cnt = C.c_int64()
+        _check(lib().umx_layout_work_items(self._h, None, None, None, 0, C.byref(cnt)))
+        m = cnt.value
+        b = np.empty(m, np.int64)
+        ln = np.empty(m, np.int32)
+        r = np.empty(m, np.int32)
+        _check(lib().umx_layout_work_items(self._h, _ptr(b), _ptr(ln), _ptr(r), m, C.byref(cnt)))
+        ru = r.view(np.uint32)
+        return b, ln, (ru & 0x7FFFFFFF).astype(np.int64), (ru >> 31).astype(bool)
 
     def set_trace(self, max_records: int):
         _check(lib().umx_layout_set_trace(self._h, int(max_records)))

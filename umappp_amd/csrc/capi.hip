@@ -290,6 +290,16 @@ struct umx_layout {
     DevBuf<unsigned long long> cursor;  // serial replay cursor + overrun flag
     int dim = 0;
     int mode = UMX_LAYOUT_PULL;
+    DevBuf<int64_t> item_begin;  // work items of the production kernels
+    DevBuf<int32_t> item_len;
+    DevBuf<int32_t> item_row;
+    int64_t n_items = -1;        // -1: not built yet
+    // Only heavy rows are split: a row's steps should see each other
+    // sequentially (items of a split row start from one snapshot of the head
+    // and their summed moves overshoot), but one very heavy row would
+    // otherwise set the length of the whole epoch.
+    double fires_per_item = 32.0;
+    int edges_per_item = 0x7fffffff;
     double a = 0, b = 0, gamma = 1, alpha0 = 1;
     uint64_t seed = 0;
     bool params = false, have_emb = false;
@@ -329,6 +339,10 @@ umx::LayoutView<T> view_of(umx_layout* L, DevBuf<T>& eps, DevBuf<T>& nxt, DevBuf
     v.gamma = static_cast<T>(L->gamma);
     v.nsr = static_cast<T>(L->nsr);
     v.tally = L->tally.p;
+    v.item_begin = L->item_begin.p;
+    v.item_len = L->item_len.p;
+    v.item_row = L->item_row.p;
+    v.n_items = L->n_items;
     v.trace = L->trace_cap > 0 ? L->trace.p : nullptr;
     v.trace_cap = L->trace_cap;
     v.trace_len = L->trace_len.p;
@@ -486,6 +500,14 @@ int umx_layout_run(umx_layout* L, int epoch_limit) {
         require(L->params && L->have_emb, UMX_ERR_STATE, "parameters and embedding must be set before run");
         require(epoch_limit >= L->current_epoch && epoch_limit <= L->total_epochs, UMX_ERR_ARGUMENT,
                 "epoch_limit must lie in [epoch(), num_epochs()]");
+        if (L->n_items < 0) {
+            if (L->dtype == UMX_F32)
+                umx::build_work_items<float>(L->n, L->rp.p, L->ef.p, L->fires_per_item, L->edges_per_item,
+                                             L->item_begin, L->item_len, L->item_row, L->n_items, L->s);
+            else
+                umx::build_work_items<double>(L->n, L->rp.p, L->ed.p, L->fires_per_item, L->edges_per_item,
+                                              L->item_begin, L->item_len, L->item_row, L->n_items, L->s);
+        }
         const int first = L->current_epoch;
         const int count = epoch_limit - first;
         if (L->timing && static_cast<int>(L->events.size()) < 2 * count) {
@@ -566,6 +588,40 @@ int umx_layout_tallies(const umx_layout* L, int64_t* out, int64_t cap) {
         require(L != nullptr && out != nullptr, UMX_ERR_ARGUMENT, "NULL argument");
         const int64_t m = std::min<int64_t>(cap, static_cast<int64_t>(L->tally.n));
         if (m > 0) UMX_HIP(hipMemcpyAsync(out, L->tally.p, sizeof(int64_t) * static_cast<size_t>(m), hipMemcpyDeviceToHost, L->s));
+        UMX_HIP(hipStreamSynchronize(L->s));
+    });
+}
+
+int umx_layout_set_work_split(umx_layout* L, double fires_per_item, int edges_per_item) {
+    return guard([&] {
+        require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
+        require(std::isfinite(fires_per_item) && fires_per_item > 0 && edges_per_item >= 1, UMX_ERR_ARGUMENT,
+                "fires_per_item must be > 0 and edges_per_item >= 1");
+        L->fires_per_item = fires_per_item;
+        L->edges_per_item = edges_per_item;
+        L->n_items = -1;  // rebuilt at the next run
+    });
+}
+
+int umx_layout_work_items(umx_layout* L, int64_t* begin, int32_t* length, int32_t* row, int64_t cap,
+                          int64_t* count) {
+    return guard([&] {
+        require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
+        if (L->n_items < 0) {
+            if (L->dtype == UMX_F32)
+                umx::build_work_items<float>(L->n, L->rp.p, L->ef.p, L->fires_per_item, L->edges_per_item,
+                                             L->item_begin, L->item_len, L->item_row, L->n_items, L->s);
+            else
+                umx::build_work_items<double>(L->n, L->rp.p, L->ed.p, L->fires_per_item, L->edges_per_item,
+                                              L->item_begin, L->item_len, L->item_row, L->n_items, L->s);
+        }
+        if (count) *count = L->n_items;
+        const int64_t m = std::min<int64_t>(cap, L->n_items);
+        if (m > 0) {
+            if (begin) UMX_HIP(hipMemcpyAsync(begin, L->item_begin.p, sizeof(int64_t) * m, hipMemcpyDeviceToHost, L->s));
+            if (length) UMX_HIP(hipMemcpyAsync(length, L->item_len.p, sizeof(int32_t) * m, hipMemcpyDeviceToHost, L->s));
+            if (row) UMX_HIP(hipMemcpyAsync(row, L->item_row.p, sizeof(int32_t) * m, hipMemcpyDeviceToHost, L->s));
+        }
         UMX_HIP(hipStreamSynchronize(L->s));
     });
 }

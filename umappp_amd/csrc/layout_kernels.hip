@@ -127,10 +127,11 @@ struct EpochConst {
 
 template <typename T, int D, bool TRACE>
 __global__ void __launch_bounds__(kBlock) k_layout_rows(LayoutView<T> v, EpochConst ec, T ep, T alpha) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t m = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     unsigned long long fires = 0, draws = 0, self_hits = 0;
-    if (i < v.n) {
-        const int64_t s = v.rp[i], e = v.rp[i + 1];
+    if (m < v.n_items) {
+        const int64_t i = static_cast<int64_t>(static_cast<uint32_t>(v.item_row[m]) & 0x7fffffffu);
+        const int64_t s = v.item_begin[m], e = s + v.item_len[m];
         const Vec<T, D> h0 = load_point<T, D>(v.emb, i);
         Vec<T, D> h = h0;
         const T two_ab = -2 * v.a * v.b;
@@ -210,11 +211,12 @@ __global__ void __launch_bounds__(kBlock) k_layout_rows(LayoutView<T> v, EpochCo
 // every step (no register copy), the rest as above.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_layout_rows_generic(LayoutView<T> v, EpochConst ec, T ep, T alpha) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t m = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     unsigned long long fires = 0, draws = 0, self_hits = 0;
-    if (i < v.n) {
+    if (m < v.n_items) {
         const int D = v.dim;
-        const int64_t s = v.rp[i], e = v.rp[i + 1];
+        const int64_t i = static_cast<int64_t>(static_cast<uint32_t>(v.item_row[m]) & 0x7fffffffu);
+        const int64_t s = v.item_begin[m], e = s + v.item_len[m];
         T* h = v.emb + i * D;
         const uint32_t n32 = static_cast<uint32_t>(v.n);
         for (int64_t j = s; j < e; ++j) {
@@ -382,7 +384,21 @@ __device__ inline T sqdist(const Vec<T, D>& x, const Vec<T, D>& y) {
     return s > eps_of<T>() ? s : eps_of<T>();
 }
 
-constexpr int kNegBatch = 4;  // negative samples whose loads are issued together
+
+// Streaming loads and stores of the per-edge schedule arrays: marked
+// non-temporal so that they do not evict the embedding, which every lane
+// gathers from at random, from the L2.
+template <typename T>
+__device__ inline T ld_stream(const T* p) {
+    return __builtin_nontemporal_load(p);
+}
+template <typename T>
+__device__ inline void st_stream(T* p, T v) {
+    __builtin_nontemporal_store(v, p);
+}
+
+constexpr int kWindow = 8;      // edges whose schedule state is fetched together
+constexpr int kNegPrefetch = 8;  // negative samples whose coordinates are fetched together
 
 // Pull-mode epoch: the owner lane of observation i applies every move of i.
 // A stored edge i->t fires in exactly the same epochs as its reverse t->i
@@ -391,83 +407,121 @@ constexpr int kNegBatch = 4;  // negative samples whose loads are issued togethe
 // tail step that t->i gives i (the reference computes it as the negative of
 // t's head move, -alpha*clip(c*(y_t - y_i)) = alpha*clip(c*(y_i - y_t))),
 // followed by i's negative samples.  Only the owner writes i, once, at the
-// end of its row: the epoch is a pure gather with no atomics.
+// end of its row (rows split over several work items add their moves with
+// atomics instead): the epoch is a pure gather.
+//
+// Latency hiding: the lane walks its edges in windows of kWindow; the
+// schedule state of a window, then its targets and firing edges' state, then
+// the tails' coordinates are each fetched as one batch of independent loads,
+// and negative samples are fetched kNegPrefetch at a time.  The steps
+// themselves are applied strictly in the reference's order.
 template <typename T, int D, bool TRACE>
 __global__ void __launch_bounds__(kBlock) k_layout_pull(LayoutView<T> v, EpochConst ec, T ep, T alpha) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t m = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     unsigned long long fires = 0, draws = 0, self_hits = 0;
-    if (i < v.n) {
-        const int64_t s = v.rp[i], e = v.rp[i + 1];
-        Vec<T, D> h = load_point<T, D>(v.emb, i);
+    if (m < v.n_items) {
+        const uint32_t packed = static_cast<uint32_t>(v.item_row[m]);
+        const int64_t i = static_cast<int64_t>(packed & 0x7fffffffu);
+        const bool split = (packed >> 31) != 0;
+        const int64_t s = v.item_begin[m], e = s + v.item_len[m];
+        const Vec<T, D> h0 = load_point<T, D>(v.emb, i);
+        Vec<T, D> h = h0;
         const T two_ab = -2 * v.a * v.b;
         const T two_gb = 2 * v.gamma * v.b;
         const uint32_t n32 = static_cast<uint32_t>(v.n);
         bool moved = false;
-        for (int64_t j = s; j < e; ++j) {
-            const T nx = v.nxt[j];
-            if (nx > ep) continue;
-            ++fires;
-            moved = true;
-            const int32_t t = v.tg[j];
-            const T ej_eps = v.eps[j];
-            const T nn = v.nxt_neg[j];
-            const Vec<T, D> r = load_point<T, D>(v.emb, t);
+        for (int64_t base = s; base < e; base += kWindow) {
+            T nx[kWindow];
 #pragma unroll
-            for (int half = 0; half < 2; ++half) {
-                const T d2 = sqdist<T, D>(h, r);
-                const T pd2b = fast_pow(d2, v.b);
-                const T gc = (two_ab * pd2b) / (d2 * (v.a * pd2b + 1));
-                double* rec = TRACE ? trace_slot(v, half == 0 ? 0 : 2, i, t) : nullptr;
+            for (int w = 0; w < kWindow; ++w)
+                nx[w] = base + w < e ? ld_stream(v.nxt + base + w) : ep + 1;
+            bool any = false;
 #pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    const T g = alpha * clip4(gc * (h.c[d] - r.c[d]));
-                    if (TRACE && rec) {
-                        rec[3 + d] = h.c[d];
-                        rec[3 + D + d] = r.c[d];
-                        rec[3 + 2 * D + d] = g;
-                    }
-                    h.c[d] += g;
-                }
+            for (int w = 0; w < kWindow; ++w) any |= !(nx[w] > ep);
+            if (!any) continue;
+            int32_t t[kWindow];
+            T eps_w[kWindow], nn_w[kWindow];
+#pragma unroll
+            for (int w = 0; w < kWindow; ++w) {
+                const bool f = !(nx[w] > ep);
+                t[w] = f ? ld_stream(v.tg + base + w) : static_cast<int32_t>(i);
+                eps_w[w] = f ? ld_stream(v.eps + base + w) : static_cast<T>(1);
+                nn_w[w] = f ? ld_stream(v.nxt_neg + base + w) : static_cast<T>(0);
             }
+            Vec<T, D> r[kWindow];
+#pragma unroll
+            for (int w = 0; w < kWindow; ++w) r[w] = load_point<T, D>(v.emb, t[w]);
 
-            const T epns = ej_eps / v.nsr;
-            const int nneg = static_cast<int>((ep - nn) / epns);
-            const uint64_t ej = edge_key(ec.ekey, static_cast<uint64_t>(j));
-            for (int p0 = 0; p0 < nneg; p0 += kNegBatch) {
-                int32_t kk[kNegBatch];
-                Vec<T, D> q[kNegBatch];
 #pragma unroll
-                for (int u = 0; u < kNegBatch; ++u) {
-                    kk[u] = p0 + u < nneg ? draw_index(ej, static_cast<uint32_t>(p0 + u), n32) : -1;
-                    q[u] = load_point<T, D>(v.emb, kk[u] >= 0 ? kk[u] : i);
-                }
+            for (int w = 0; w < kWindow; ++w) {
+                if (nx[w] > ep) continue;
+                const int64_t j = base + w;
+                ++fires;
+                moved = true;
 #pragma unroll
-                for (int u = 0; u < kNegBatch; ++u) {
-                    if (kk[u] < 0) break;
-                    ++draws;
-                    if (kk[u] == i) {
-                        ++self_hits;
-                        continue;
-                    }
-                    const T e2 = sqdist<T, D>(h, q[u]);
-                    const T rc = two_gb / ((static_cast<T>(0.001) + e2) * (v.a * fast_pow(e2, v.b) + 1));
-                    double* rec = TRACE ? trace_slot(v, 1, i, kk[u]) : nullptr;
+                for (int half = 0; half < 2; ++half) {
+                    const T d2 = sqdist<T, D>(h, r[w]);
+                    const T pd2b = fast_pow(d2, v.b);
+                    const T gc = (two_ab * pd2b) / (d2 * (v.a * pd2b + 1));
+                    double* rec = TRACE ? trace_slot(v, half == 0 ? 0 : 2, i, t[w]) : nullptr;
 #pragma unroll
                     for (int d = 0; d < D; ++d) {
-                        const T g = alpha * clip4(rc * (h.c[d] - q[u].c[d]));
+                        const T g = alpha * clip4(gc * (h.c[d] - r[w].c[d]));
                         if (TRACE && rec) {
                             rec[3 + d] = h.c[d];
-                            rec[3 + D + d] = q[u].c[d];
+                            rec[3 + D + d] = r[w].c[d];
                             rec[3 + 2 * D + d] = g;
                         }
                         h.c[d] += g;
                     }
                 }
+
+                const T epns = eps_w[w] / v.nsr;
+                const int nneg = static_cast<int>((ep - nn_w[w]) / epns);
+                const uint64_t ej = edge_key(ec.ekey, static_cast<uint64_t>(j));
+                for (int p0 = 0; p0 < nneg; p0 += kNegPrefetch) {
+                    int32_t kk[kNegPrefetch];
+                    Vec<T, D> q[kNegPrefetch];
+#pragma unroll
+                    for (int u = 0; u < kNegPrefetch; ++u) {
+                        kk[u] = p0 + u < nneg ? draw_index(ej, static_cast<uint32_t>(p0 + u), n32) : -1;
+                        q[u] = load_point<T, D>(v.emb, kk[u] >= 0 ? kk[u] : i);
+                    }
+#pragma unroll
+                    for (int u = 0; u < kNegPrefetch; ++u) {
+                        if (kk[u] < 0) break;
+                        ++draws;
+                        if (kk[u] == i) {
+                            ++self_hits;
+                            continue;
+                        }
+                        const T e2 = sqdist<T, D>(h, q[u]);
+                        const T rc = two_gb / ((static_cast<T>(0.001) + e2) * (v.a * fast_pow(e2, v.b) + 1));
+                        double* rec = TRACE ? trace_slot(v, 1, i, kk[u]) : nullptr;
+#pragma unroll
+                        for (int d = 0; d < D; ++d) {
+                            const T g = alpha * clip4(rc * (h.c[d] - q[u].c[d]));
+                            if (TRACE && rec) {
+                                rec[3 + d] = h.c[d];
+                                rec[3 + D + d] = q[u].c[d];
+                                rec[3 + 2 * D + d] = g;
+                            }
+                            h.c[d] += g;
+                        }
+                    }
+                }
+                st_stream(v.nxt + j, nx[w] + eps_w[w]);
+                st_stream(v.nxt_neg + j, nn_w[w] + static_cast<T>(nneg) * epns);
             }
-            v.nxt[j] = nx + ej_eps;
-            v.nxt_neg[j] = nn + static_cast<T>(nneg) * epns;
         }
-        if (moved) store_point<T, D>(v.emb, i, h);
+        if (moved) {
+            if (split) {  // several items move this observation: merge their moves
+#pragma unroll
+                for (int d = 0; d < D; ++d) add_to(v.emb + i * D + d, h.c[d] - h0.c[d]);
+            } else {
+                store_point<T, D>(v.emb, i, h);
+            }
+        }
     }
     tally_add(v.tally + 3 * static_cast<int64_t>(ec.epoch), fires, draws, self_hits);
 }
@@ -475,11 +529,12 @@ __global__ void __launch_bounds__(kBlock) k_layout_pull(LayoutView<T> v, EpochCo
 // Pull mode for any embedding dimension: the owner updates its point in place.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_layout_pull_generic(LayoutView<T> v, EpochConst ec, T ep, T alpha) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t m = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     unsigned long long fires = 0, draws = 0, self_hits = 0;
-    if (i < v.n) {
+    if (m < v.n_items) {
         const int D = v.dim;
-        const int64_t s = v.rp[i], e = v.rp[i + 1];
+        const int64_t i = static_cast<int64_t>(static_cast<uint32_t>(v.item_row[m]) & 0x7fffffffu);
+        const int64_t s = v.item_begin[m], e = s + v.item_len[m];
         T* h = v.emb + i * D;
         const uint32_t n32 = static_cast<uint32_t>(v.n);
         for (int64_t j = s; j < e; ++j) {
@@ -496,7 +551,7 @@ __global__ void __launch_bounds__(kBlock) k_layout_pull_generic(LayoutView<T> v,
                 d2 = d2 > eps_of<T>() ? d2 : eps_of<T>();
                 const T pd2b = fast_pow(d2, v.b);
                 const T gc = (-2 * v.a * v.b * pd2b) / (d2 * (v.a * pd2b + 1));
-                for (int d = 0; d < D; ++d) h[d] += alpha * clip4(gc * (h[d] - r[d]));
+                for (int d = 0; d < D; ++d) add_to(h + d, alpha * clip4(gc * (h[d] - r[d])));
             }
             const T epns = v.eps[j] / v.nsr;
             const T nn = v.nxt_neg[j];
@@ -517,13 +572,135 @@ __global__ void __launch_bounds__(kBlock) k_layout_pull_generic(LayoutView<T> v,
                 }
                 e2 = e2 > eps_of<T>() ? e2 : eps_of<T>();
                 const T rc = 2 * v.gamma * v.b / ((static_cast<T>(0.001) + e2) * (v.a * fast_pow(e2, v.b) + 1));
-                for (int d = 0; d < D; ++d) h[d] += alpha * clip4(rc * (h[d] - q[d]));
+                for (int d = 0; d < D; ++d) add_to(h + d, alpha * clip4(rc * (h[d] - q[d])));
             }
             v.nxt[j] = nx + v.eps[j];
             v.nxt_neg[j] = nn + static_cast<T>(nneg) * epns;
         }
     }
     tally_add(v.tally + 3 * static_cast<int64_t>(ec.epoch), fires, draws, self_hits);
+}
+
+// Work items (see build_work_items).  One thread per row: the expected
+// firings per epoch f = sum of 1 / eps over the row; a row with f above the
+// split threshold (or longer than the edge cap) is cut into about
+// c = ceil(f / threshold) items, a cut falling where the running sum of 1/eps
+// would pass q * f / c (q = 1..c-1) or where the edge cap is reached.  The
+// same walk counts the items (WRITE = false) and then writes them.
+template <typename T, bool WRITE>
+__device__ inline int64_t split_row(const int64_t* __restrict__ rp, const T* __restrict__ eps, int64_t i,
+                                    double fires_split, int edges_cap, int64_t m, int64_t* __restrict__ ibeg,
+                                    int32_t* __restrict__ ilen, int32_t* __restrict__ irow, float* __restrict__ iwork,
+                                    bool split_tag) {
+    const int64_t r0 = rp[i], r1 = rp[i + 1];
+    if (r1 == r0) return 0;
+    double f = 0;
+    for (int64_t j = r0; j < r1; ++j) f += 1.0 / static_cast<double>(eps[j]);
+    const double c = f > fires_split ? ceil(f / fires_split) : 1.0;
+    const int32_t tag = static_cast<int32_t>(static_cast<uint32_t>(i) | (split_tag ? 0x80000000u : 0u));
+    int64_t items = 0, start = r0;
+    double acc = 0, item_f = 0, q = 1;
+    for (int64_t j = r0; j < r1; ++j) {
+        const double w = 1.0 / static_cast<double>(eps[j]);
+        const bool cut = j > start && ((q < c && acc + w > q * f / c) || j - start >= edges_cap);
+        if (cut) {
+            if (WRITE) {
+                ibeg[m + items] = start;
+                ilen[m + items] = static_cast<int32_t>(j - start);
+                irow[m + items] = tag;
+                iwork[m + items] = static_cast<float>(item_f);
+            }
+            ++items;
+            if (acc + w > q * f / c) q += 1;
+            start = j;
+            item_f = 0;
+        }
+        acc += w;
+        item_f += w;
+    }
+    if (WRITE) {
+        ibeg[m + items] = start;
+        ilen[m + items] = static_cast<int32_t>(r1 - start);
+        irow[m + items] = tag;
+        iwork[m + items] = static_cast<float>(item_f);
+    }
+    return items + 1;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_count_items(int64_t n, const int64_t* __restrict__ rp,
+                                                      const T* __restrict__ eps, double fires_split, int edges_cap,
+                                                      int64_t* __restrict__ count) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    count[i] = split_row<T, false>(rp, eps, i, fires_split, edges_cap, 0, nullptr, nullptr, nullptr, nullptr, false);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_fill_items(int64_t n, const int64_t* __restrict__ rp,
+                                                     const T* __restrict__ eps, double fires_split, int edges_cap,
+                                                     const int64_t* __restrict__ offs, int64_t* __restrict__ ibeg,
+                                                     int32_t* __restrict__ ilen, int32_t* __restrict__ irow,
+                                                     float* __restrict__ iwork) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int64_t c = offs[i + 1] - offs[i];
+    if (c == 0) return;
+    split_row<T, true>(rp, eps, i, fires_split, edges_cap, offs[i], ibeg, ilen, irow, iwork, c > 1);
+}
+
+// Heaviest items first: a stable counting sort of the items on a 32-level
+// key (descending log2 of expected firings), so that long items start at
+// once and each wavefront gets items of similar length.  Pass A counts keys
+// per tile of kBlock items; pass B scatters each item to
+// offset[key][tile] + its rank among the tile's items with that key.
+constexpr int kKeys = 32;
+
+__device__ inline int work_key(float f) {
+    // 31 for the heaviest items down to 0: key = 31 - clamp(floor(log2(16 f)) + 1, 0, 31)
+    int lvl = 0;
+    const float x = f * 16.0f;
+    if (x >= 1.0f) lvl = 1 + static_cast<int>(floorf(log2f(x)));
+    lvl = lvl > 31 ? 31 : lvl;
+    return 31 - lvl;
+}
+
+__global__ void __launch_bounds__(kBlock) k_item_key_hist(const float* __restrict__ iwork, int64_t n_items,
+                                                        int64_t ntiles, int64_t* __restrict__ hist) {
+    __shared__ int cnt[kKeys];
+    if (threadIdx.x < kKeys) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t m = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (m < n_items) atomicAdd(&cnt[work_key(iwork[m])], 1);
+    __syncthreads();
+    if (threadIdx.x < kKeys) hist[static_cast<int64_t>(threadIdx.x) * ntiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kBlock) k_item_scatter(const int64_t* __restrict__ ibeg, const int32_t* __restrict__ ilen,
+                                                       const int32_t* __restrict__ irow, const float* __restrict__ iwork,
+                                                       int64_t n_items, int64_t ntiles,
+                                                       const int64_t* __restrict__ offs, int64_t* __restrict__ obeg,
+                                                       int32_t* __restrict__ olen, int32_t* __restrict__ orow) {
+    __shared__ int wave_cnt[kBlock / kWave][kKeys];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int64_t m = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const bool live = m < n_items;
+    const int key = live ? work_key(iwork[m]) : -1;
+    // rank among earlier lanes of this wave with the same key
+    int rank = 0;
+    for (int k = 0; k < kKeys; ++k) {
+        const unsigned long long bal = __ballot(key == k);
+        if (key == k) rank = __popcll(bal & ((1ULL << lane) - 1ULL));
+        if (lane == 0) wave_cnt[wid][k] = __popcll(bal);
+    }
+    __syncthreads();
+    if (!live) return;
+    for (int w = 0; w < wid; ++w) rank += wave_cnt[w][key];
+    const int64_t at = offs[static_cast<int64_t>(key) * ntiles + blockIdx.x] + rank;
+    obeg[at] = ibeg[m];
+    olen[at] = ilen[m];
+    orow[at] = irow[m];
 }
 
 }  // namespace
@@ -535,7 +712,7 @@ void launch_layout_epoch(const LayoutView<T>& v, const EpochArgs& ea, hipStream_
     const EpochConst ec{ea.epoch, ea.ekey};
     const T ep = static_cast<T>(ea.epoch);
     const T alpha = static_cast<T>(ea.alpha);
-    const int64_t blocks = (v.n + kBlock - 1) / kBlock;
+    const int64_t blocks = (v.n_items + kBlock - 1) / kBlock;
     const unsigned grid = static_cast<unsigned>(blocks < 1 ? 1 : blocks);
     if (v.mode == kModePush) {
         if (v.trace && v.dim <= 4) {
@@ -582,6 +759,56 @@ void launch_layout_serial(const LayoutView<T>& v, const EpochArgs& ea, const int
                                        cursor, overrun);
     UMX_HIP(hipGetLastError());
 }
+
+template <typename T>
+void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_split, int edges_cap,
+                      DevBuf<int64_t>& item_begin, DevBuf<int32_t>& item_len, DevBuf<int32_t>& item_row,
+                      int64_t& n_items, hipStream_t s) {
+    if (!(fires_split > 0) || edges_cap < 1) throw Error(-1, "work split parameters must be positive");
+    DevBuf<int64_t> count, offs;
+    count.alloc(static_cast<size_t>(n > 0 ? n : 1));
+    offs.alloc(static_cast<size_t>(n + 1));
+    const unsigned grid = static_cast<unsigned>((n + kBlock - 1) / kBlock > 0 ? (n + kBlock - 1) / kBlock : 1);
+    if (n > 0) k_count_items<T><<<grid, kBlock, 0, s>>>(n, rp, eps, fires_split, edges_cap, count.p);
+    UMX_HIP(hipGetLastError());
+    DevBuf<int64_t> unused;
+    exclusive_scan_i64(count.p, offs.p, n, unused, s);
+    UMX_HIP(hipMemcpyAsync(&n_items, offs.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    UMX_HIP(hipStreamSynchronize(s));
+    if (n_items > 0x7fffffffLL) throw Error(-5, "too many work items");
+    const size_t M = static_cast<size_t>(n_items > 0 ? n_items : 1);
+    DevBuf<int64_t> ib;
+    DevBuf<int32_t> il, ir;
+    DevBuf<float> iw;
+    ib.alloc(M);
+    il.alloc(M);
+    ir.alloc(M);
+    iw.alloc(M);
+    if (n > 0)
+        k_fill_items<T><<<grid, kBlock, 0, s>>>(n, rp, eps, fires_split, edges_cap, offs.p, ib.p, il.p, ir.p, iw.p);
+    UMX_HIP(hipGetLastError());
+    item_begin.alloc(M);
+    item_len.alloc(M);
+    item_row.alloc(M);
+    if (n_items > 0) {
+        const int64_t ntiles = (n_items + kBlock - 1) / kBlock;
+        DevBuf<int64_t> hist, hoff;
+        hist.alloc(static_cast<size_t>(kKeys * ntiles));
+        hoff.alloc(static_cast<size_t>(kKeys * ntiles + 1));
+        k_item_key_hist<<<static_cast<unsigned>(ntiles), kBlock, 0, s>>>(iw.p, n_items, ntiles, hist.p);
+        UMX_HIP(hipGetLastError());
+        exclusive_scan_i64(hist.p, hoff.p, kKeys * ntiles, unused, s);
+        k_item_scatter<<<static_cast<unsigned>(ntiles), kBlock, 0, s>>>(ib.p, il.p, ir.p, iw.p, n_items, ntiles, hoff.p,
+                                                                       item_begin.p, item_len.p, item_row.p);
+        UMX_HIP(hipGetLastError());
+    }
+    UMX_HIP(hipStreamSynchronize(s));
+}
+
+template void build_work_items<float>(int64_t, const int64_t*, const float*, double, int, DevBuf<int64_t>&,
+                                      DevBuf<int32_t>&, DevBuf<int32_t>&, int64_t&, hipStream_t);
+template void build_work_items<double>(int64_t, const int64_t*, const double*, double, int, DevBuf<int64_t>&,
+                                       DevBuf<int32_t>&, DevBuf<int32_t>&, int64_t&, hipStream_t);
 
 template void launch_layout_epoch<float>(const LayoutView<float>&, const EpochArgs&, hipStream_t);
 template void launch_layout_epoch<double>(const LayoutView<double>&, const EpochArgs&, hipStream_t);

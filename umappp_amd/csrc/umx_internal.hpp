@@ -106,6 +106,14 @@ struct LayoutView {
     int mode;
     T a, b, gamma, nsr;
     unsigned long long* tally;  // 3 counters per epoch
+    // Work items of the production kernels: item m covers edges
+    // [item_begin[m], item_begin[m] + item_len[m]) of row
+    // item_row[m] & 0x7fffffff; the top bit marks rows split over several
+    // items.  Items are ordered heaviest first.
+    const int64_t* item_begin;
+    const int32_t* item_len;
+    const int32_t* item_row;
+    int64_t n_items;
     // Optional update trace (parity mode): records of 3 + 3 * dim doubles
     // (kind, head, other, head before, other before, head move).
     double* trace;
@@ -119,6 +127,15 @@ void launch_layout_epoch(const LayoutView<T>& v, const EpochArgs& ea, hipStream_
 template <typename T>
 void launch_layout_serial(const LayoutView<T>& v, const EpochArgs& ea, const int32_t* samples, int64_t count,
                           unsigned long long* cursor, hipStream_t s);
+
+// Work items of the production layout kernels: one per row, except that rows
+// expected to fire more than 'fires_split' edges per epoch (sum of 1 / eps)
+// or longer than 'edges_cap' are cut into items of about equal expected
+// firings.  Items are sorted heaviest first.
+template <typename T>
+void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_split, int edges_cap,
+                      DevBuf<int64_t>& item_begin, DevBuf<int32_t>& item_len, DevBuf<int32_t>& item_row,
+                      int64_t& n_items, hipStream_t s);
 
 // Counter sampler key of one epoch (host side; the device derives per-edge
 // and per-draw keys from it).
