@@ -325,9 +325,14 @@ def test_work_items_cover_every_edge_once(umx, fires, cap):
     assert np.array_equal(split, nitems[row] > 1)
     assert np.array_equal(nitems > 0, np.diff(rpe) > 0)
     inv = 1.0 / ed.epochs_per_sample.astype(np.float64)
-    work = np.array([inv[s_:s_ + l_].sum() for s_, l_ in zip(b, ln)])
-    lvl = np.where(work * 16 >= 1, 1 + np.floor(np.log2(np.maximum(work * 16, 1e-30))), 0).clip(0, 31)
-    assert np.all(np.diff(lvl) <= 0), "items run heaviest first"
+    work = np.array([inv[s_:s_ + l_].sum() for s_, l_ in zip(b, ln)]) * 16
+    lg = np.log2(np.maximum(work, 1e-30))
+    lvl = np.where(work >= 1, 1 + np.floor(lg), 0).clip(0, 31)
+    # heaviest first, by power-of-two level (levels can differ by one where
+    # the work sits on a level boundary to float rounding)
+    drop = np.nonzero(np.diff(lvl) > 0)[0]
+    near = np.abs(lg - np.round(lg)) < 1e-4
+    assert np.all(near[drop] | near[drop + 1]), "items run heaviest first"
     rowf = np.add.reduceat(inv, rpe[:-1])[nitems > 1]
     assert np.all(rowf > fires) or cap < 2**31 - 1
 

@@ -397,8 +397,8 @@ __device__ inline void st_stream(T* p, T v) {
     __builtin_nontemporal_store(v, p);
 }
 
-constexpr int kWindow = 8;      // edges whose schedule state is fetched together
-constexpr int kNegPrefetch = 8;  // negative samples whose coordinates are fetched together
+constexpr int kMaskLoads = 8;     // schedule values fetched together when building a fire mask
+constexpr int kNegPrefetch = 8;   // negative samples whose coordinates are fetched together
 
 // Pull-mode epoch: the owner lane of observation i applies every move of i.
 // A stored edge i->t fires in exactly the same epochs as its reverse t->i
@@ -410,11 +410,13 @@ constexpr int kNegPrefetch = 8;  // negative samples whose coordinates are fetch
 // end of its row (rows split over several work items add their moves with
 // atomics instead): the epoch is a pure gather.
 //
-// Latency hiding: the lane walks its edges in windows of kWindow; the
-// schedule state of a window, then its targets and firing edges' state, then
-// the tails' coordinates are each fetched as one batch of independent loads,
-// and negative samples are fetched kNegPrefetch at a time.  The steps
-// themselves are applied strictly in the reference's order.
+// SIMD efficiency: a lane first turns up to 64 edges of its row into a fire
+// mask (schedule values fetched kMaskLoads at a time), then walks only the
+// firing edges with count-trailing-zeros, so on every trip of the loop each
+// lane that still has work applies a whole firing edge; items are sorted by
+// expected firings, so lanes of a wavefront finish together.  The tail and
+// the first kNegPrefetch negative samples of an edge are fetched as one
+// batch of independent loads.  Steps are applied in the reference's order.
 template <typename T, int D, bool TRACE>
 __global__ void __launch_bounds__(kBlock) k_layout_pull(LayoutView<T> v, EpochConst ec, T ep, T alpha) {
     const int64_t m = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
@@ -430,62 +432,63 @@ __global__ void __launch_bounds__(kBlock) k_layout_pull(LayoutView<T> v, EpochCo
         const T two_gb = 2 * v.gamma * v.b;
         const uint32_t n32 = static_cast<uint32_t>(v.n);
         bool moved = false;
-        for (int64_t base = s; base < e; base += kWindow) {
-            T nx[kWindow];
+        for (int64_t seg = s; seg < e; seg += 64) {
+            const int nseg = e - seg < 64 ? static_cast<int>(e - seg) : 64;
+            unsigned long long mask = 0;
+            for (int c = 0; c < nseg; c += kMaskLoads) {
+                T nx[kMaskLoads];
 #pragma unroll
-            for (int w = 0; w < kWindow; ++w)
-                nx[w] = base + w < e ? ld_stream(v.nxt + base + w) : ep + 1;
-            bool any = false;
+                for (int u = 0; u < kMaskLoads; ++u) nx[u] = c + u < nseg ? ld_stream(v.nxt + seg + c + u) : ep + 1;
 #pragma unroll
-            for (int w = 0; w < kWindow; ++w) any |= !(nx[w] > ep);
-            if (!any) continue;
-            int32_t t[kWindow];
-            T eps_w[kWindow], nn_w[kWindow];
-#pragma unroll
-            for (int w = 0; w < kWindow; ++w) {
-                const bool f = !(nx[w] > ep);
-                t[w] = f ? ld_stream(v.tg + base + w) : static_cast<int32_t>(i);
-                eps_w[w] = f ? ld_stream(v.eps + base + w) : static_cast<T>(1);
-                nn_w[w] = f ? ld_stream(v.nxt_neg + base + w) : static_cast<T>(0);
+                for (int u = 0; u < kMaskLoads; ++u)
+                    if (!(nx[u] > ep)) mask |= 1ULL << (c + u);
             }
-            Vec<T, D> r[kWindow];
-#pragma unroll
-            for (int w = 0; w < kWindow; ++w) r[w] = load_point<T, D>(v.emb, t[w]);
-
-#pragma unroll
-            for (int w = 0; w < kWindow; ++w) {
-                if (nx[w] > ep) continue;
-                const int64_t j = base + w;
+            while (mask) {
+                const int w = __builtin_ctzll(mask);
+                mask &= mask - 1;
+                const int64_t j = seg + w;
                 ++fires;
                 moved = true;
+                const int32_t t = ld_stream(v.tg + j);
+                const T nxj = ld_stream(v.nxt + j);
+                const T epsj = ld_stream(v.eps + j);
+                const T nnj = ld_stream(v.nxt_neg + j);
+                const T epns = epsj / v.nsr;
+                const int nneg = static_cast<int>((ep - nnj) / epns);
+                const uint64_t ej = edge_key(ec.ekey, static_cast<uint64_t>(j));
+                // tail and first batch of negatives in flight together
+                const Vec<T, D> r = load_point<T, D>(v.emb, t);
+                int32_t kk[kNegPrefetch];
+                Vec<T, D> q[kNegPrefetch];
+#pragma unroll
+                for (int u = 0; u < kNegPrefetch; ++u) {
+                    kk[u] = u < nneg ? draw_index(ej, static_cast<uint32_t>(u), n32) : -1;
+                    q[u] = load_point<T, D>(v.emb, kk[u] >= 0 ? kk[u] : i);
+                }
 #pragma unroll
                 for (int half = 0; half < 2; ++half) {
-                    const T d2 = sqdist<T, D>(h, r[w]);
+                    const T d2 = sqdist<T, D>(h, r);
                     const T pd2b = fast_pow(d2, v.b);
                     const T gc = (two_ab * pd2b) / (d2 * (v.a * pd2b + 1));
-                    double* rec = TRACE ? trace_slot(v, half == 0 ? 0 : 2, i, t[w]) : nullptr;
+                    double* rec = TRACE ? trace_slot(v, half == 0 ? 0 : 2, i, t) : nullptr;
 #pragma unroll
                     for (int d = 0; d < D; ++d) {
-                        const T g = alpha * clip4(gc * (h.c[d] - r[w].c[d]));
+                        const T g = alpha * clip4(gc * (h.c[d] - r.c[d]));
                         if (TRACE && rec) {
                             rec[3 + d] = h.c[d];
-                            rec[3 + D + d] = r[w].c[d];
+                            rec[3 + D + d] = r.c[d];
                             rec[3 + 2 * D + d] = g;
                         }
                         h.c[d] += g;
                     }
                 }
-
-                const T epns = eps_w[w] / v.nsr;
-                const int nneg = static_cast<int>((ep - nn_w[w]) / epns);
-                const uint64_t ej = edge_key(ec.ekey, static_cast<uint64_t>(j));
                 for (int p0 = 0; p0 < nneg; p0 += kNegPrefetch) {
-                    int32_t kk[kNegPrefetch];
-                    Vec<T, D> q[kNegPrefetch];
+                    if (p0 > 0) {
 #pragma unroll
-                    for (int u = 0; u < kNegPrefetch; ++u) {
-                        kk[u] = p0 + u < nneg ? draw_index(ej, static_cast<uint32_t>(p0 + u), n32) : -1;
-                        q[u] = load_point<T, D>(v.emb, kk[u] >= 0 ? kk[u] : i);
+                        for (int u = 0; u < kNegPrefetch; ++u) {
+                            kk[u] = p0 + u < nneg ? draw_index(ej, static_cast<uint32_t>(p0 + u), n32) : -1;
+                            q[u] = load_point<T, D>(v.emb, kk[u] >= 0 ? kk[u] : i);
+                        }
                     }
 #pragma unroll
                     for (int u = 0; u < kNegPrefetch; ++u) {
@@ -510,8 +513,8 @@ __global__ void __launch_bounds__(kBlock) k_layout_pull(LayoutView<T> v, EpochCo
                         }
                     }
                 }
-                st_stream(v.nxt + j, nx[w] + eps_w[w]);
-                st_stream(v.nxt_neg + j, nn_w[w] + static_cast<T>(nneg) * epns);
+                st_stream(v.nxt + j, nxj + epsj);
+                st_stream(v.nxt_neg + j, nnj + static_cast<T>(nneg) * epns);
             }
         }
         if (moved) {
