@@ -88,6 +88,7 @@ def _declare(L):
         "umx_layout_tallies": (C.c_int, [_vp, _vp, C.c_int64]),
         "umx_layout_set_mode": (C.c_int, [_vp, C.c_int]),
         "umx_layout_set_work_split": (C.c_int, [_vp, C.c_double, C.c_int]),
+        "umx_layout_set_tile": (C.c_int, [_vp, C.c_int]),
         "umx_layout_work_items": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int64, _i64p]),
         "umx_layout_set_trace": (C.c_int, [_vp, C.c_int64]),
         "umx_layout_get_trace": (C.c_int, [_vp, _vp, C.c_int64, _i64p]),
@@ -353,6 +354,9 @@ class Layout:
 
     def set_work_split(self, fires_per_item: float = 32.0, edges_per_item: int = 2**31 - 1):
         _check(lib().umx_layout_set_work_split(self._h, float(fires_per_item), int(edges_per_item)))
+
+    def set_tile(self, tile_cols: int = 16):
+        _check(lib().umx_layout_set_tile(self._h, int(tile_cols)))
 
     def work_items(self):
         """(begin, length, row, split) arrays of the production kernels' work items, in run order."""

@@ -19,7 +19,7 @@ CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "libumappp_amd.so"
 
-SOURCES = ["graph_kernels.hip", "layout_kernels.hip", "capi.hip"]
+SOURCES = ["graph_kernels.hip", "layout_kernels.hip", "sell_kernels.hip", "capi.hip"]
 HOST_SOURCES = ["host_api.cpp"]
 CXX = os.environ.get("CXX", "g++")
 

@@ -276,6 +276,12 @@ int umx_combine_neighbor_sets(int dtype, int64_t n, const int64_t* row_ptr, cons
 // ---------------------------------------------------------------------------
 // Layout handle
 // ---------------------------------------------------------------------------
+// Which copy of the schedule state (epoch_of_next_sample /
+// epoch_of_next_negative_sample) is current.
+constexpr int kHomeCsr = 0;
+constexpr int kHomeSell = 1;
+constexpr int kHomeBoth = 2;
+
 struct umx_layout {
     int dtype = UMX_F32;
     int64_t n = 0, nnz = 0, max_degree = 0;
@@ -303,6 +309,11 @@ struct umx_layout {
     double a = 0, b = 0, gamma = 1, alpha0 = 1;
     uint64_t seed = 0;
     bool params = false, have_emb = false;
+    umx::SellState<float> sell_f;   // sliced-ELL schedule copy (production pull kernel)
+    umx::SellState<double> sell_d;
+    bool sell_built = false;
+    int home = kHomeCsr;            // which schedule copy holds the latest state
+    int tile_cols = 16;
     DevBuf<double> trace;                 // update trace records (parity mode)
     DevBuf<unsigned long long> trace_len;
     int64_t trace_cap = 0;
@@ -339,6 +350,13 @@ umx::LayoutView<T> view_of(umx_layout* L, DevBuf<T>& eps, DevBuf<T>& nxt, DevBuf
     v.gamma = static_cast<T>(L->gamma);
     v.nsr = static_cast<T>(L->nsr);
     v.tally = L->tally.p;
+    v.chunk_width = nullptr;
+    v.chunk_base = nullptr;
+    v.n_chunks = 0;
+    v.sell_tg = nullptr;
+    v.sell_eps = nullptr;
+    v.sell_nx = nullptr;
+    v.sell_nn = nullptr;
     v.item_begin = L->item_begin.p;
     v.item_len = L->item_len.p;
     v.item_row = L->item_row.p;
@@ -347,6 +365,57 @@ umx::LayoutView<T> view_of(umx_layout* L, DevBuf<T>& eps, DevBuf<T>& nxt, DevBuf
     v.trace_cap = L->trace_cap;
     v.trace_len = L->trace_len.p;
     return v;
+}
+
+template <typename T>
+void attach_sell(umx::LayoutView<T>& v, umx::SellState<T>& st) {
+    v.chunk_width = st.width.p;
+    v.chunk_base = st.base.p;
+    v.n_chunks = st.n_chunks;
+    v.sell_tg = st.tg.p;
+    v.sell_eps = st.eps.p;
+    v.sell_nx = st.nx.p;
+    v.sell_nn = st.nn.p;
+}
+
+void ensure_items(umx_layout* L) {
+    if (L->n_items >= 0) return;
+    if (L->dtype == UMX_F32)
+        umx::build_work_items<float>(L->n, L->rp.p, L->ef.p, L->fires_per_item, L->edges_per_item, L->item_begin,
+                                     L->item_len, L->item_row, L->n_items, L->s);
+    else
+        umx::build_work_items<double>(L->n, L->rp.p, L->ed.p, L->fires_per_item, L->edges_per_item, L->item_begin,
+                                      L->item_len, L->item_row, L->n_items, L->s);
+    L->sell_built = false;
+}
+
+// Makes the CSR schedule arrays current (copies the sliced-ELL state back).
+void ensure_csr_current(umx_layout* L) {
+    if (L->home != kHomeSell) return;
+    if (L->dtype == UMX_F32) umx::sync_sell_to_csr<float>(L->nf.p, L->gf.p, L->sell_f, L->s);
+    else umx::sync_sell_to_csr<double>(L->nd.p, L->gd.p, L->sell_d, L->s);
+    UMX_HIP(hipStreamSynchronize(L->s));
+    L->home = kHomeBoth;
+}
+
+// Builds the sliced-ELL copy if needed and makes it current.
+void ensure_sell_current(umx_layout* L) {
+    ensure_items(L);
+    if (!L->sell_built) {
+        if (L->dtype == UMX_F32)
+            umx::build_sell<float>(L->nnz, L->tg.p, L->ef.p, L->item_begin.p, L->item_len.p, L->item_row.p,
+                                   L->n_items, L->sell_f, L->s);
+        else
+            umx::build_sell<double>(L->nnz, L->tg.p, L->ed.p, L->item_begin.p, L->item_len.p, L->item_row.p,
+                                    L->n_items, L->sell_d, L->s);
+        L->sell_built = true;
+        if (L->home == kHomeBoth || L->home == kHomeSell) L->home = kHomeCsr;
+    }
+    if (L->home == kHomeCsr) {
+        if (L->dtype == UMX_F32) umx::sync_csr_to_sell<float>(L->nf.p, L->gf.p, L->sell_f, L->s);
+        else umx::sync_csr_to_sell<double>(L->nd.p, L->gd.p, L->sell_d, L->s);
+        L->home = kHomeBoth;
+    }
 }
 
 // Learning rate of epoch n with the reference's types (optimize_layout.hpp:
@@ -433,10 +502,14 @@ int umx_layout_info(const umx_layout* L, int64_t* n, int64_t* nnz, int* total_ep
     });
 }
 
-int umx_layout_export_epochs(const umx_layout* L, int64_t* row_ptr, int32_t* targets, void* eps, void* nxt,
+int umx_layout_export_epochs(const umx_layout* Lc, int64_t* row_ptr, int32_t* targets, void* eps, void* nxt,
                              void* nxt_neg) {
     return guard([&] {
-        require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
+        require(Lc != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
+        // exporting does not change the logical state; it may copy the
+        // sliced-ELL schedule back into the CSR arrays first
+        umx_layout* L = const_cast<umx_layout*>(Lc);
+        ensure_csr_current(L);
         const size_t E = static_cast<size_t>(L->nnz);
         const size_t es = dsize(L->dtype);
         if (row_ptr) UMX_HIP(hipMemcpyAsync(row_ptr, L->rp.p, sizeof(int64_t) * static_cast<size_t>(L->n + 1), hipMemcpyDefault, L->s));
@@ -500,14 +573,10 @@ int umx_layout_run(umx_layout* L, int epoch_limit) {
         require(L->params && L->have_emb, UMX_ERR_STATE, "parameters and embedding must be set before run");
         require(epoch_limit >= L->current_epoch && epoch_limit <= L->total_epochs, UMX_ERR_ARGUMENT,
                 "epoch_limit must lie in [epoch(), num_epochs()]");
-        if (L->n_items < 0) {
-            if (L->dtype == UMX_F32)
-                umx::build_work_items<float>(L->n, L->rp.p, L->ef.p, L->fires_per_item, L->edges_per_item,
-                                             L->item_begin, L->item_len, L->item_row, L->n_items, L->s);
-            else
-                umx::build_work_items<double>(L->n, L->rp.p, L->ed.p, L->fires_per_item, L->edges_per_item,
-                                              L->item_begin, L->item_len, L->item_row, L->n_items, L->s);
-        }
+        const bool use_sell = L->mode == UMX_LAYOUT_PULL && L->dim <= 4;
+        ensure_items(L);
+        if (use_sell) ensure_sell_current(L);
+        else ensure_csr_current(L);
         const int first = L->current_epoch;
         const int count = epoch_limit - first;
         if (L->timing && static_cast<int>(L->events.size()) < 2 * count) {
@@ -525,10 +594,22 @@ int umx_layout_run(umx_layout* L, int epoch_limit) {
             if (L->timing) UMX_HIP(hipEventRecord(L->events[2 * q], L->s));
             if (L->dtype == UMX_F32) {
                 ea.alpha = epoch_alpha<float>(L->alpha0, ep, L->total_epochs);
-                umx::launch_layout_epoch<float>(view_of<float>(L, L->ef, L->nf, L->gf, L->mf), ea, L->s);
+                auto v = view_of<float>(L, L->ef, L->nf, L->gf, L->mf);
+                if (use_sell) {
+                    attach_sell(v, L->sell_f);
+                    umx::launch_layout_sell<float>(v, ea, L->tile_cols, L->s);
+                } else {
+                    umx::launch_layout_epoch<float>(v, ea, L->s);
+                }
             } else {
                 ea.alpha = epoch_alpha<double>(L->alpha0, ep, L->total_epochs);
-                umx::launch_layout_epoch<double>(view_of<double>(L, L->ed, L->nd, L->gd, L->md), ea, L->s);
+                auto v = view_of<double>(L, L->ed, L->nd, L->gd, L->md);
+                if (use_sell) {
+                    attach_sell(v, L->sell_d);
+                    umx::launch_layout_sell<double>(v, ea, L->tile_cols, L->s);
+                } else {
+                    umx::launch_layout_epoch<double>(v, ea, L->s);
+                }
             }
             if (L->timing) UMX_HIP(hipEventRecord(L->events[2 * q + 1], L->s));
         }
@@ -542,6 +623,7 @@ int umx_layout_run(umx_layout* L, int epoch_limit) {
             L->launches += count;
         }
         L->current_epoch = epoch_limit;
+        if (count > 0) L->home = use_sell ? kHomeSell : kHomeCsr;
     });
 }
 
@@ -552,6 +634,8 @@ int umx_layout_run_serial(umx_layout* L, int epoch_limit, const int32_t* samples
         require(epoch_limit >= L->current_epoch && epoch_limit <= L->total_epochs, UMX_ERR_ARGUMENT,
                 "epoch_limit must lie in [epoch(), num_epochs()]");
         require(samples == nullptr || count >= 0, UMX_ERR_ARGUMENT, "negative sample count");
+        ensure_csr_current(L);
+        if (epoch_limit > L->current_epoch) L->home = kHomeCsr;
         int32_t* dsamples = nullptr;
         if (samples && count > 0) {
             UMX_HIP(hipMallocAsync(reinterpret_cast<void**>(&dsamples), sizeof(int32_t) * static_cast<size_t>(count), L->s));
@@ -597,9 +681,12 @@ int umx_layout_set_work_split(umx_layout* L, double fires_per_item, int edges_pe
         require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
         require(std::isfinite(fires_per_item) && fires_per_item > 0 && edges_per_item >= 1, UMX_ERR_ARGUMENT,
                 "fires_per_item must be > 0 and edges_per_item >= 1");
+        ensure_csr_current(L);
         L->fires_per_item = fires_per_item;
         L->edges_per_item = edges_per_item;
         L->n_items = -1;  // rebuilt at the next run
+        L->sell_built = false;
+        L->home = kHomeCsr;
     });
 }
 
@@ -607,14 +694,7 @@ int umx_layout_work_items(umx_layout* L, int64_t* begin, int32_t* length, int32_
                           int64_t* count) {
     return guard([&] {
         require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
-        if (L->n_items < 0) {
-            if (L->dtype == UMX_F32)
-                umx::build_work_items<float>(L->n, L->rp.p, L->ef.p, L->fires_per_item, L->edges_per_item,
-                                             L->item_begin, L->item_len, L->item_row, L->n_items, L->s);
-            else
-                umx::build_work_items<double>(L->n, L->rp.p, L->ed.p, L->fires_per_item, L->edges_per_item,
-                                              L->item_begin, L->item_len, L->item_row, L->n_items, L->s);
-        }
+        ensure_items(L);
         if (count) *count = L->n_items;
         const int64_t m = std::min<int64_t>(cap, L->n_items);
         if (m > 0) {
@@ -623,6 +703,14 @@ int umx_layout_work_items(umx_layout* L, int64_t* begin, int32_t* length, int32_
             if (row) UMX_HIP(hipMemcpyAsync(row, L->item_row.p, sizeof(int32_t) * m, hipMemcpyDeviceToHost, L->s));
         }
         UMX_HIP(hipStreamSynchronize(L->s));
+    });
+}
+
+int umx_layout_set_tile(umx_layout* L, int tile_cols) {
+    return guard([&] {
+        require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
+        require(tile_cols == 8 || tile_cols == 16 || tile_cols == 32, UMX_ERR_ARGUMENT, "tile_cols must be 8, 16 or 32");
+        L->tile_cols = tile_cols;
     });
 }
 

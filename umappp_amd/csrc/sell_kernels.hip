@@ -1,0 +1,359 @@
+// Production layout epoch for gfx950 over a sliced-ELL copy of the schedule.
+//
+// Layout in HBM.  Work items (rows, heavy rows cut into pieces, sorted
+// heaviest first) are grouped 64 at a time into chunks, one chunk per
+// wavefront.  A chunk of width W (its longest item) holds 64 * W slots stored
+// column-major: slot(k, lane) = chunk_base + k * 64 + lane, slot k of a lane
+// being edge k of its item (padding slots never fire).  So a wavefront's load
+// of "column k" of the targets, epochs_per_sample, epoch_of_next_sample or
+// epoch_of_next_negative_sample is 64 consecutive values: every schedule byte
+// is fetched once per epoch, fully coalesced, instead of each lane walking its
+// own CSR row and touching a different cache line per load.
+//
+// Kernel.  A wavefront stages its chunk TW columns at a time in LDS (each
+// lane only reads back its own entries, so no barriers are needed), builds a
+// fire mask per lane, and walks the firing slots with count-trailing-zeros:
+// on every trip each lane applies one whole firing edge (pull mode: head
+// step, reverse-edge tail step, negative samples, exactly as in
+// k_layout_pull), fetching the tail and up to kNegPrefetch negative samples
+// in one batch of independent loads.  Updated schedule values go back to HBM
+// with coalesced stores.  The CSR arrays remain the canonical schedule state;
+// the host syncs the two copies when it switches between this kernel and the
+// CSR kernels (serial replay, push mode, export).
+#include "layout_device.hpp"
+#include "umx_internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+namespace umx {
+
+namespace {
+
+using namespace umx::dev;
+
+constexpr int kNegFetch = 8;
+
+// ---------------------------------------------------------------------------
+// Build: chunk widths, chunk offsets, constant slot data and the slot map.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kWave) k_chunk_width(const int32_t* __restrict__ item_len, int64_t n_items,
+                                                     int32_t* __restrict__ width, int64_t* __restrict__ slots) {
+    const int64_t c = blockIdx.x;
+    const int64_t m = c * kWave + threadIdx.x;
+    int w = m < n_items ? item_len[m] : 0;
+#pragma unroll
+    for (int s = kWave / 2; s >= 1; s >>= 1) {
+        const int o = __shfl_xor(w, s, kWave);
+        w = o > w ? o : w;
+    }
+    if (threadIdx.x == 0) {
+        width[c] = w;
+        slots[c] = static_cast<int64_t>(w) * kWave;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kWave) k_fill_sell(const int64_t* __restrict__ item_begin,
+                                                   const int32_t* __restrict__ item_len,
+                                                   const int32_t* __restrict__ item_row, int64_t n_items,
+                                                   const int32_t* __restrict__ width, const int64_t* __restrict__ base,
+                                                   const int32_t* __restrict__ tg, const T* __restrict__ eps,
+                                                   int32_t* __restrict__ stg, T* __restrict__ seps,
+                                                   T* __restrict__ snx, T* __restrict__ snn,
+                                                   int64_t* __restrict__ slot_of_edge) {
+    const int64_t c = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t m = c * kWave + lane;
+    const int w = width[c];
+    const int64_t b0 = base[c];
+    const bool live = m < n_items;
+    const int64_t e0 = live ? item_begin[m] : 0;
+    const int len = live ? item_len[m] : 0;
+    const int32_t self = live ? static_cast<int32_t>(static_cast<uint32_t>(item_row[m]) & 0x7fffffffu) : 0;
+    for (int k = 0; k < w; ++k) {
+        const int64_t slot = b0 + static_cast<int64_t>(k) * kWave + lane;
+        if (k < len) {
+            const int64_t e = e0 + k;
+            stg[slot] = tg[e];
+            seps[slot] = eps[e];
+            slot_of_edge[e] = slot;
+        } else {
+            stg[slot] = self;
+            seps[slot] = static_cast<T>(1);
+            snx[slot] = static_cast<T>(INFINITY);  // padding never fires
+            snn[slot] = static_cast<T>(0);
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_csr_to_sell(int64_t E, const int64_t* __restrict__ slot_of_edge,
+                                                   const T* __restrict__ nx, const T* __restrict__ nn,
+                                                   T* __restrict__ snx, T* __restrict__ snn) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= E) return;
+    const int64_t s = slot_of_edge[e];
+    snx[s] = nx[e];
+    snn[s] = nn[e];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_sell_to_csr(int64_t E, const int64_t* __restrict__ slot_of_edge,
+                                                   const T* __restrict__ snx, const T* __restrict__ snn,
+                                                   T* __restrict__ nx, T* __restrict__ nn) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= E) return;
+    const int64_t s = slot_of_edge[e];
+    nx[e] = snx[s];
+    nn[e] = snn[s];
+}
+
+// ---------------------------------------------------------------------------
+// Epoch kernel: one wavefront per chunk.
+// ---------------------------------------------------------------------------
+template <typename T, int TW>
+struct SellTile {
+    T nx[TW][kWave];
+    T nn[TW][kWave];
+    T eps[TW][kWave];
+    int32_t tg[TW][kWave];
+};
+
+template <typename T, int D, int TW, bool TRACE>
+__global__ void __launch_bounds__(kWave) k_layout_sell(LayoutView<T> v, EpochConst ec, T ep, T alpha) {
+    __shared__ SellTile<T, TW> tile;
+    const int64_t c = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t m = c * kWave + lane;
+    const bool live = m < v.n_items;
+    const int width = v.chunk_width[c];
+    const int64_t base = v.chunk_base[c];
+
+    unsigned long long fires = 0, draws = 0, self_hits = 0;
+    int64_t i = 0, e0 = 0;
+    bool split = false;
+    Vec<T, D> h0{}, h{};
+    if (live) {
+        const uint32_t packed = static_cast<uint32_t>(v.item_row[m]);
+        i = static_cast<int64_t>(packed & 0x7fffffffu);
+        split = (packed >> 31) != 0;
+        e0 = v.item_begin[m];
+        h0 = load_point<T, D>(v.emb, i);
+        h = h0;
+    }
+    const T two_ab = -2 * v.a * v.b;
+    const T two_gb = 2 * v.gamma * v.b;
+    const uint32_t n32 = static_cast<uint32_t>(v.n);
+    bool moved = false;
+
+    for (int t0 = 0; t0 < width; t0 += TW) {
+        const int tw = width - t0 < TW ? width - t0 : TW;
+        // coalesced fetch of the tile (all columns in flight together)
+        unsigned int mask = 0;
+#pragma unroll
+        for (int k = 0; k < TW; ++k) {
+            if (k < tw) {
+                const int64_t slot = base + static_cast<int64_t>(t0 + k) * kWave + lane;
+                const T x = v.sell_nx[slot];
+                tile.nx[k][lane] = x;
+                tile.nn[k][lane] = v.sell_nn[slot];
+                tile.eps[k][lane] = v.sell_eps[slot];
+                tile.tg[k][lane] = v.sell_tg[slot];
+                if (!(x > ep)) mask |= 1u << k;
+            }
+        }
+        if (!live) mask = 0;
+        unsigned int fired = mask;
+        while (mask) {
+            const int k = __builtin_ctz(mask);
+            mask &= mask - 1;
+            const int64_t j = e0 + t0 + k;  // CSR index of this edge: keys the sampler
+            ++fires;
+            moved = true;
+            const int32_t t = tile.tg[k][lane];
+            const T nxj = tile.nx[k][lane];
+            const T epsj = tile.eps[k][lane];
+            const T nnj = tile.nn[k][lane];
+            const T epns = epsj / v.nsr;
+            const int nneg = static_cast<int>((ep - nnj) / epns);
+            const uint64_t ej = edge_key(ec.ekey, static_cast<uint64_t>(j));
+            const Vec<T, D> r = load_point<T, D>(v.emb, t);
+            int32_t kk[kNegFetch];
+            Vec<T, D> q[kNegFetch];
+#pragma unroll
+            for (int u = 0; u < kNegFetch; ++u) {
+                kk[u] = u < nneg ? draw_index(ej, static_cast<uint32_t>(u), n32) : -1;
+                q[u] = load_point<T, D>(v.emb, kk[u] >= 0 ? kk[u] : i);
+            }
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                const T d2 = sqdist<T, D>(h, r);
+                const T pd2b = fast_pow(d2, v.b);
+                const T gc = (two_ab * pd2b) / (d2 * (v.a * pd2b + 1));
+                double* rec = TRACE ? trace_slot(v, half == 0 ? 0 : 2, i, t) : nullptr;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const T g = alpha * clip4(gc * (h.c[d] - r.c[d]));
+                    if (TRACE && rec) {
+                        rec[3 + d] = h.c[d];
+                        rec[3 + D + d] = r.c[d];
+                        rec[3 + 2 * D + d] = g;
+                    }
+                    h.c[d] += g;
+                }
+            }
+            for (int p0 = 0; p0 < nneg; p0 += kNegFetch) {
+                if (p0 > 0) {
+#pragma unroll
+                    for (int u = 0; u < kNegFetch; ++u) {
+                        kk[u] = p0 + u < nneg ? draw_index(ej, static_cast<uint32_t>(p0 + u), n32) : -1;
+                        q[u] = load_point<T, D>(v.emb, kk[u] >= 0 ? kk[u] : i);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kNegFetch; ++u) {
+                    if (kk[u] < 0) break;
+                    ++draws;
+                    if (kk[u] == i) {
+                        ++self_hits;
+                        continue;
+                    }
+                    const T e2 = sqdist<T, D>(h, q[u]);
+                    const T rc = two_gb / ((static_cast<T>(0.001) + e2) * (v.a * fast_pow(e2, v.b) + 1));
+                    double* rec = TRACE ? trace_slot(v, 1, i, kk[u]) : nullptr;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        const T g = alpha * clip4(rc * (h.c[d] - q[u].c[d]));
+                        if (TRACE && rec) {
+                            rec[3 + d] = h.c[d];
+                            rec[3 + D + d] = q[u].c[d];
+                            rec[3 + 2 * D + d] = g;
+                        }
+                        h.c[d] += g;
+                    }
+                }
+            }
+            tile.nx[k][lane] = nxj + epsj;
+            tile.nn[k][lane] = nnj + static_cast<T>(nneg) * epns;
+        }
+        // coalesced write-back of the columns that fired somewhere in the wave
+#pragma unroll
+        for (int k = 0; k < TW; ++k) {
+            if (k < tw && (fired >> k) & 1u) {
+                const int64_t slot = base + static_cast<int64_t>(t0 + k) * kWave + lane;
+                v.sell_nx[slot] = tile.nx[k][lane];
+                v.sell_nn[slot] = tile.nn[k][lane];
+            }
+        }
+    }
+    if (live && moved) {
+        if (split) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) add_to(v.emb + i * D + d, h.c[d] - h0.c[d]);
+        } else {
+            store_point<T, D>(v.emb, i, h);
+        }
+    }
+    tally_add(v.tally + 3 * static_cast<int64_t>(ec.epoch), fires, draws, self_hits);
+}
+
+template <typename T, int TW>
+void launch_sell_tw(const LayoutView<T>& v, const EpochConst& ec, T ep, T alpha, hipStream_t s) {
+    const unsigned grid = static_cast<unsigned>(v.n_chunks);
+    if (v.trace) {
+        switch (v.dim) {
+            case 1: k_layout_sell<T, 1, TW, true><<<grid, kWave, 0, s>>>(v, ec, ep, alpha); break;
+            case 2: k_layout_sell<T, 2, TW, true><<<grid, kWave, 0, s>>>(v, ec, ep, alpha); break;
+            case 3: k_layout_sell<T, 3, TW, true><<<grid, kWave, 0, s>>>(v, ec, ep, alpha); break;
+            default: k_layout_sell<T, 4, TW, true><<<grid, kWave, 0, s>>>(v, ec, ep, alpha); break;
+        }
+    } else {
+        switch (v.dim) {
+            case 1: k_layout_sell<T, 1, TW, false><<<grid, kWave, 0, s>>>(v, ec, ep, alpha); break;
+            case 2: k_layout_sell<T, 2, TW, false><<<grid, kWave, 0, s>>>(v, ec, ep, alpha); break;
+            case 3: k_layout_sell<T, 3, TW, false><<<grid, kWave, 0, s>>>(v, ec, ep, alpha); break;
+            default: k_layout_sell<T, 4, TW, false><<<grid, kWave, 0, s>>>(v, ec, ep, alpha); break;
+        }
+    }
+}
+
+}  // namespace
+
+template <typename T>
+void build_sell(int64_t E, const int32_t* tg, const T* eps, const int64_t* item_begin, const int32_t* item_len,
+                const int32_t* item_row, int64_t n_items, SellState<T>& st, hipStream_t s) {
+    st.n_chunks = (n_items + kWave - 1) / kWave;
+    const int64_t nc = st.n_chunks;
+    st.width.alloc(static_cast<size_t>(nc > 0 ? nc : 1));
+    st.base.alloc(static_cast<size_t>(nc + 1));
+    DevBuf<int64_t> slots;
+    slots.alloc(static_cast<size_t>(nc > 0 ? nc : 1));
+    if (nc > 0) {
+        k_chunk_width<<<static_cast<unsigned>(nc), kWave, 0, s>>>(item_len, n_items, st.width.p, slots.p);
+        UMX_HIP(hipGetLastError());
+    }
+    DevBuf<int64_t> unused;
+    exclusive_scan_i64(slots.p, st.base.p, nc, unused, s);
+    UMX_HIP(hipMemcpyAsync(&st.n_slots, st.base.p + nc, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    UMX_HIP(hipStreamSynchronize(s));
+    const size_t ns = static_cast<size_t>(st.n_slots > 0 ? st.n_slots : 1);
+    st.tg.alloc(ns);
+    st.eps.alloc(ns);
+    st.nx.alloc(ns);
+    st.nn.alloc(ns);
+    st.slot_of_edge.alloc(static_cast<size_t>(E > 0 ? E : 1));
+    if (nc > 0) {
+        k_fill_sell<T><<<static_cast<unsigned>(nc), kWave, 0, s>>>(item_begin, item_len, item_row, n_items,
+                                                                    st.width.p, st.base.p, tg, eps, st.tg.p,
+                                                                    st.eps.p, st.nx.p, st.nn.p, st.slot_of_edge.p);
+        UMX_HIP(hipGetLastError());
+    }
+    st.E = E;
+    UMX_HIP(hipStreamSynchronize(s));
+}
+
+template <typename T>
+void sync_csr_to_sell(const T* nx, const T* nn, SellState<T>& st, hipStream_t s) {
+    if (st.E == 0) return;
+    const unsigned grid = static_cast<unsigned>((st.E + 255) / 256);
+    k_csr_to_sell<T><<<grid, 256, 0, s>>>(st.E, st.slot_of_edge.p, nx, nn, st.nx.p, st.nn.p);
+    UMX_HIP(hipGetLastError());
+}
+
+template <typename T>
+void sync_sell_to_csr(T* nx, T* nn, const SellState<T>& st, hipStream_t s) {
+    if (st.E == 0) return;
+    const unsigned grid = static_cast<unsigned>((st.E + 255) / 256);
+    k_sell_to_csr<T><<<grid, 256, 0, s>>>(st.E, st.slot_of_edge.p, st.nx.p, st.nn.p, nx, nn);
+    UMX_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_layout_sell(const LayoutView<T>& v, const EpochArgs& ea, int tile_cols, hipStream_t s) {
+    if (v.n_chunks == 0) {
+        return;
+    }
+    const EpochConst ec{ea.epoch, ea.ekey};
+    const T ep = static_cast<T>(ea.epoch);
+    const T alpha = static_cast<T>(ea.alpha);
+    switch (tile_cols) {
+        case 8: launch_sell_tw<T, 8>(v, ec, ep, alpha, s); break;
+        case 32: launch_sell_tw<T, 32>(v, ec, ep, alpha, s); break;
+        default: launch_sell_tw<T, 16>(v, ec, ep, alpha, s); break;
+    }
+    UMX_HIP(hipGetLastError());
+}
+
+#define UMX_INST(T)                                                                                            \
+    template void build_sell<T>(int64_t, const int32_t*, const T*, const int64_t*, const int32_t*,            \
+                                const int32_t*, int64_t, SellState<T>&, hipStream_t);                          \
+    template void sync_csr_to_sell<T>(const T*, const T*, SellState<T>&, hipStream_t);                         \
+    template void sync_sell_to_csr<T>(T*, T*, const SellState<T>&, hipStream_t);                               \
+    template void launch_layout_sell<T>(const LayoutView<T>&, const EpochArgs&, int, hipStream_t);
+UMX_INST(float)
+UMX_INST(double)
+#undef UMX_INST
+
+}  // namespace umx

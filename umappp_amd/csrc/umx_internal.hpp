@@ -114,6 +114,16 @@ struct LayoutView {
     const int32_t* item_len;
     const int32_t* item_row;
     int64_t n_items;
+    // Sliced-ELL copy of the schedule for the production pull kernel (see
+    // sell_kernels.hip): chunk c of 64 items has chunk_width[c] columns
+    // starting at slot chunk_base[c]; slot (k, lane) = base + k * 64 + lane.
+    const int32_t* chunk_width;
+    const int64_t* chunk_base;
+    int64_t n_chunks;
+    const int32_t* sell_tg;
+    const T* sell_eps;
+    T* sell_nx;
+    T* sell_nn;
     // Optional update trace (parity mode): records of 3 + 3 * dim doubles
     // (kind, head, other, head before, other before, head move).
     double* trace;
@@ -136,6 +146,27 @@ template <typename T>
 void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_split, int edges_cap,
                       DevBuf<int64_t>& item_begin, DevBuf<int32_t>& item_len, DevBuf<int32_t>& item_row,
                       int64_t& n_items, hipStream_t s);
+
+// Sliced-ELL schedule state (sell_kernels.hip).
+template <typename T>
+struct SellState {
+    int64_t n_chunks = 0, n_slots = 0, E = 0;
+    DevBuf<int32_t> width;
+    DevBuf<int64_t> base;
+    DevBuf<int32_t> tg;
+    DevBuf<T> eps, nx, nn;
+    DevBuf<int64_t> slot_of_edge;
+};
+
+template <typename T>
+void build_sell(int64_t E, const int32_t* tg, const T* eps, const int64_t* item_begin, const int32_t* item_len,
+                const int32_t* item_row, int64_t n_items, SellState<T>& st, hipStream_t s);
+template <typename T>
+void sync_csr_to_sell(const T* nx, const T* nn, SellState<T>& st, hipStream_t s);
+template <typename T>
+void sync_sell_to_csr(T* nx, T* nn, const SellState<T>& st, hipStream_t s);
+template <typename T>
+void launch_layout_sell(const LayoutView<T>& v, const EpochArgs& ea, int tile_cols, hipStream_t s);
 
 // Counter sampler key of one epoch (host side; the device derives per-edge
 // and per-draw keys from it).
