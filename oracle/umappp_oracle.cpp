@@ -295,9 +295,11 @@ inline uint64_t discrete_uniform(std::mt19937_64& eng, uint64_t n) {
 // index), epoch n, draw number p is a pure function of (seed, n, j, p):
 //   ek = mix64(seed + GOLD * (n + 1))
 //   ej = mix64(ek ^ (j * ODD))
-//   r  = mix64(ej + GOLD * (p + 1))
-//   k  = high 64 bits of r * N
-// mix64 is the splitmix64 output finaliser (Steele, Lea, Flood 2014).
+//   N <= 2^24: r = lowbias32(lo32(ej) + 0x9e3779b9 * (p + 1)) ^ hi32(ej),
+//              k = high 32 bits of r * N
+//   N >  2^24: r = mix64(ej + GOLD * (p + 1)), k = high 64 bits of r * N
+// mix64 is the splitmix64 output finaliser (Steele, Lea, Flood 2014) and
+// lowbias32 C. Wellons' 32-bit integer hash.
 inline uint64_t mix64(uint64_t z) {
     z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
     z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
@@ -307,7 +309,19 @@ constexpr uint64_t kGold = 0x9e3779b97f4a7c15ULL;
 constexpr uint64_t kOdd = 0xd1b54a32d192ed03ULL;
 inline uint64_t counter_epoch_key(uint64_t seed, int epoch) { return mix64(seed + kGold * (static_cast<uint64_t>(epoch) + 1)); }
 inline uint64_t counter_edge_key(uint64_t ek, uint64_t j) { return mix64(ek ^ (j * kOdd)); }
+inline uint32_t lowbias32(uint32_t x) {  // C. Wellons' 32-bit hash
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
 inline uint32_t counter_draw(uint64_t ej, uint32_t p, uint32_t n) {
+    if (n <= (1u << 24)) {
+        const uint32_t r = lowbias32(static_cast<uint32_t>(ej) + 0x9e3779b9U * (p + 1)) ^ static_cast<uint32_t>(ej >> 32);
+        return static_cast<uint32_t>((static_cast<uint64_t>(r) * n) >> 32);
+    }
     const uint64_t r = mix64(ej + kGold * (static_cast<uint64_t>(p) + 1));
     return static_cast<uint32_t>((static_cast<unsigned __int128>(r) * n) >> 64);
 }

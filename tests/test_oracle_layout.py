@@ -28,9 +28,22 @@ def _mix64(z):
     return z ^ (z >> 31)
 
 
+def _lowbias32(x):
+    x &= 0xFFFFFFFF
+    x ^= x >> 16
+    x = (x * 0x7FEB352D) & 0xFFFFFFFF
+    x ^= x >> 15
+    x = (x * 0x846CA68B) & 0xFFFFFFFF
+    x ^= x >> 16
+    return x
+
+
 def counter_draw_py(seed, epoch, edge, p, n):
     ek = _mix64((seed + 0x9E3779B97F4A7C15 * (epoch + 1)) & M64)
     ej = _mix64(ek ^ ((edge * 0xD1B54A32D192ED03) & M64))
+    if n <= 1 << 24:
+        r = _lowbias32((ej & 0xFFFFFFFF) + 0x9E3779B9 * (p + 1)) ^ (ej >> 32)
+        return (r * n) >> 32
     r = _mix64((ej + 0x9E3779B97F4A7C15 * (p + 1)) & M64)
     return (r * n) >> 64
 
@@ -201,12 +214,19 @@ def test_parallel_equals_serial_float_many_threads():
 
 def test_counter_sampler_definition():
     for seed, epoch, edge, p, n in [(0, 0, 0, 0, 10), (1234567890, 7, 123456, 3, 1000000), (2**63 + 5, 199, 2**33, 9, 7),
-                                    (42, 1, 1, 0, 2**31 - 1)]:
+                                    (42, 1, 1, 0, 2**31 - 1), (7, 3, 99, 4, 2**24), (7, 3, 99, 4, 2**24 + 1), (1, 0, 5, 77, 1000003)]:
         assert oracle.counter_draw(seed, epoch, edge, p, n) == counter_draw_py(seed, epoch, edge, p, n)
-    draws = np.array([oracle.counter_draw(99, 3, e, p, 17) for e in range(400) for p in range(5)])
-    counts = np.bincount(draws, minlength=17)
-    chi2 = ((counts - draws.size / 17) ** 2 / (draws.size / 17)).sum()
-    assert chi2 < 45  # 16 dof, p ~ 2e-4
+    for n in (17, 2**24 + 3):
+        draws = np.array([oracle.counter_draw(99, 3, e, p, n) for e in range(400) for p in range(5)])
+        assert draws.min() >= 0 and draws.max() < n
+        if n == 17:
+            counts = np.bincount(draws, minlength=17)
+            chi2 = ((counts - draws.size / 17) ** 2 / (draws.size / 17)).sum()
+            assert chi2 < 45  # 16 dof, p ~ 2e-4
+        else:
+            counts = np.bincount(draws * 16 // n, minlength=16)  # 16 equal bins
+            chi2 = ((counts - draws.size / 16) ** 2 / (draws.size / 16)).sum()
+            assert chi2 < 45
 
 
 def test_mt19937_64_known_answer():

@@ -709,7 +709,8 @@ int umx_layout_work_items(umx_layout* L, int64_t* begin, int32_t* length, int32_
 int umx_layout_set_tile(umx_layout* L, int tile_cols) {
     return guard([&] {
         require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
-        require(tile_cols == 8 || tile_cols == 16 || tile_cols == 32, UMX_ERR_ARGUMENT, "tile_cols must be 8, 16 or 32");
+        require(tile_cols == 4 || tile_cols == 8 || tile_cols == 16 || tile_cols == 32, UMX_ERR_ARGUMENT,
+                "tile_cols must be 4, 8, 16 or 32");
         L->tile_cols = tile_cols;
     });
 }

@@ -24,7 +24,24 @@ __host__ __device__ inline uint64_t mix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 __device__ inline uint64_t edge_key(uint64_t ekey, uint64_t j) { return mix64(ekey ^ (j * kOdd)); }
+// 32-bit integer hash with low bias (C. Wellons, "lowbias32").
+__host__ __device__ inline uint32_t lowbias32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
+// Draw p of an edge with key ej, uniform in [0, n).  Up to 2^24 observations
+// the draw hashes the low key word with lowbias32 and maps the 32-bit result
+// by a high multiply (relative bias below n / 2^32); beyond, a full 64-bit
+// mix64 draw is mapped by a 64-bit high multiply.
 __device__ inline int32_t draw_index(uint64_t ej, uint32_t p, uint32_t n) {
+    if (n <= (1u << 24)) {
+        const uint32_t r = lowbias32(static_cast<uint32_t>(ej) + 0x9e3779b9U * (p + 1)) ^ static_cast<uint32_t>(ej >> 32);
+        return static_cast<int32_t>(__umulhi(r, n));
+    }
     const uint64_t r = mix64(ej + kGold * (static_cast<uint64_t>(p) + 1));
     return static_cast<int32_t>(__umul64hi(r, static_cast<uint64_t>(n)));
 }
@@ -49,6 +66,11 @@ __device__ inline double eps_of<double>() {
 // transcendental units for float (x >= FLT_EPSILON here, so no denormals).
 __device__ inline float fast_pow(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
 __device__ inline double fast_pow(double x, double y) { return pow(x, y); }
+
+// Quotient for the production kernels: float uses the hardware reciprocal
+// (within about 1 ulp), double the IEEE division.
+__device__ inline float prod_div(float num, float den) { return num * __builtin_amdgcn_rcpf(den); }
+__device__ inline double prod_div(double num, double den) { return num / den; }
 
 template <typename T>
 __device__ inline void add_to(T* p, T v) {

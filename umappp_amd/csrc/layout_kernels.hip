@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(kBlock) k_layout_rows(LayoutView<T> v, EpochCo
             }
             d2 = d2 > eps_of<T>() ? d2 : eps_of<T>();
             const T pd2b = fast_pow(d2, v.b);
-            const T gc = (two_ab * pd2b) / (d2 * (v.a * pd2b + 1));
+            const T gc = prod_div(two_ab * pd2b, d2 * (v.a * pd2b + 1));
             double* rec = TRACE ? trace_slot(v, 0, i, t) : nullptr;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(kBlock) k_layout_rows(LayoutView<T> v, EpochCo
                     e2 += z * z;
                 }
                 e2 = e2 > eps_of<T>() ? e2 : eps_of<T>();
-                const T rc = two_gb / ((static_cast<T>(0.001) + e2) * (v.a * fast_pow(e2, v.b) + 1));
+                const T rc = prod_div(two_gb, (static_cast<T>(0.001) + e2) * (v.a * fast_pow(e2, v.b) + 1));
                 double* rec = TRACE ? trace_slot(v, 1, i, k) : nullptr;
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
@@ -336,7 +336,7 @@ __global__ void __launch_bounds__(kBlock) k_layout_pull(LayoutView<T> v, EpochCo
                 for (int half = 0; half < 2; ++half) {
                     const T d2 = sqdist<T, D>(h, r);
                     const T pd2b = fast_pow(d2, v.b);
-                    const T gc = (two_ab * pd2b) / (d2 * (v.a * pd2b + 1));
+                    const T gc = prod_div(two_ab * pd2b, d2 * (v.a * pd2b + 1));
                     double* rec = TRACE ? trace_slot(v, half == 0 ? 0 : 2, i, t) : nullptr;
 #pragma unroll
                     for (int d = 0; d < D; ++d) {
@@ -366,7 +366,7 @@ __global__ void __launch_bounds__(kBlock) k_layout_pull(LayoutView<T> v, EpochCo
                             continue;
                         }
                         const T e2 = sqdist<T, D>(h, q[u]);
-                        const T rc = two_gb / ((static_cast<T>(0.001) + e2) * (v.a * fast_pow(e2, v.b) + 1));
+                        const T rc = prod_div(two_gb, (static_cast<T>(0.001) + e2) * (v.a * fast_pow(e2, v.b) + 1));
                         double* rec = TRACE ? trace_slot(v, 1, i, kk[u]) : nullptr;
 #pragma unroll
                         for (int d = 0; d < D; ++d) {

@@ -191,7 +191,7 @@ __global__ void __launch_bounds__(kWave) k_layout_sell(LayoutView<T> v, EpochCon
             for (int half = 0; half < 2; ++half) {
                 const T d2 = sqdist<T, D>(h, r);
                 const T pd2b = fast_pow(d2, v.b);
-                const T gc = (two_ab * pd2b) / (d2 * (v.a * pd2b + 1));
+                const T gc = prod_div(two_ab * pd2b, d2 * (v.a * pd2b + 1));
                 double* rec = TRACE ? trace_slot(v, half == 0 ? 0 : 2, i, t) : nullptr;
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(kWave) k_layout_sell(LayoutView<T> v, EpochCon
                         continue;
                     }
                     const T e2 = sqdist<T, D>(h, q[u]);
-                    const T rc = two_gb / ((static_cast<T>(0.001) + e2) * (v.a * fast_pow(e2, v.b) + 1));
+                    const T rc = prod_div(two_gb, (static_cast<T>(0.001) + e2) * (v.a * fast_pow(e2, v.b) + 1));
                     double* rec = TRACE ? trace_slot(v, 1, i, kk[u]) : nullptr;
 #pragma unroll
                     for (int d = 0; d < D; ++d) {
@@ -339,6 +339,7 @@ void launch_layout_sell(const LayoutView<T>& v, const EpochArgs& ea, int tile_co
     const T ep = static_cast<T>(ea.epoch);
     const T alpha = static_cast<T>(ea.alpha);
     switch (tile_cols) {
+        case 4: launch_sell_tw<T, 4>(v, ec, ep, alpha, s); break;
         case 8: launch_sell_tw<T, 8>(v, ec, ep, alpha, s); break;
         case 32: launch_sell_tw<T, 32>(v, ec, ep, alpha, s); break;
         default: launch_sell_tw<T, 16>(v, ec, ep, alpha, s); break;
