@@ -152,7 +152,7 @@ int umx_layout_set_mode(umx_layout* L, int mode);
  * atomics; items are run heaviest first.  Defaults 32.0 and 2^31 - 1. */
 int umx_layout_set_work_split(umx_layout* L, double fires_per_item, int edges_per_item);
 /* LDS tile of the pull kernel: schedule columns staged per wavefront at a
- * time (4, 8, 16 or 32; default 16). */
+ * time (4, 8, 16 or 32; default 8). */
 int umx_layout_set_tile(umx_layout* L, int tile_cols);
 /* The current work items (built on demand), in run order: first edge,
  * length, row (top bit set for rows cut into several items).  *count

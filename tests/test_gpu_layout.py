@@ -354,34 +354,32 @@ def test_run_argument_errors(umx):
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_final_quality_c1_matches_reference_path(umx, dtype):
     """Config C1 (2000 x 10, k=15, 500 epochs, min_dist 0.1 -> a, b from
-    find_ab), production (pull) kernel.  C1 is unstructured N(0,1) data: the
-    reference path's own seed-to-seed spread is about 0.01 in
-    trustworthiness and 0.005 in preservation."""
+    find_ab), production (pull) kernel vs the reference path (serial loop,
+    mt19937_64 sampler) from the same initial embedding.  C1 is unstructured
+    N(0,1) data whose metrics move by about 0.01 between seeds of the
+    reference path itself, so both sides are averaged over three seeds."""
     xh, (rp, idx, dist) = c1_knn()
     a, b = umx.find_ab(1.0, 0.1)
     T, nsr = 500, 5.0
     wd, _, _ = oracle.neighbor_similarities(rp, dist)
     crp, cidx, cw = oracle.combine_neighbor_sets(rp, idx, wd, 1.0)
     cw = cw.astype(dtype)
-    rng = np.random.default_rng(42)
-    emb0 = rng.uniform(-10, 10, size=(2000, 2)).astype(dtype)
-
-    eo = emb0.copy()
+    emb0 = np.random.default_rng(42).uniform(-10, 10, size=(2000, 2)).astype(dtype)
     ep = oracle.similarities_to_epochs(crp, cidx, cw, T, nsr)
-    oracle.Layout(ep, T, nsr, rng_kind=oracle.RNG_MT, seed=1234567890, dtype=dtype).run(eo, a, b)
-
-    lay = umx.Layout(csr=umx.NeighborList(crp, cidx, cw), num_epochs=T, negative_sample_rate=nsr)
-    lay.set_params(2, a, b, 1.0, 1.0, 1234567890)
-    lay.set_embedding(emb0)
-    lay.run(T)
-    eg = lay.get_embedding()
-
     hi = idx.reshape(2000, 15)
-    p_cpu = knn_preservation(xh, eo, 15, high_idx=hi)
-    p_gpu = knn_preservation(xh, eg, 15, high_idx=hi)
-    t_cpu = trustworthiness(xh, eo, 15)
-    t_gpu = trustworthiness(xh, eg, 15)
-    print(f"C1 {np.dtype(dtype).name}: 15-NN preservation cpu {p_cpu:.4f} gpu {p_gpu:.4f}; "
+    cpu, gpu = [], []
+    for seed in (1234567890, 11, 12):
+        eo = emb0.copy()
+        oracle.Layout(ep, T, nsr, rng_kind=oracle.RNG_MT, seed=seed, dtype=dtype).run(eo, a, b)
+        cpu.append((knn_preservation(xh, eo, 15, high_idx=hi), trustworthiness(xh, eo, 15)))
+        lay = umx.Layout(csr=umx.NeighborList(crp, cidx, cw), num_epochs=T, negative_sample_rate=nsr)
+        lay.set_params(2, a, b, 1.0, 1.0, seed)
+        lay.set_embedding(emb0)
+        lay.run(T)
+        eg = lay.get_embedding()
+        gpu.append((knn_preservation(xh, eg, 15, high_idx=hi), trustworthiness(xh, eg, 15)))
+    (p_cpu, t_cpu), (p_gpu, t_gpu) = np.mean(cpu, axis=0), np.mean(gpu, axis=0)
+    print(f"C1 {np.dtype(dtype).name} (3 seeds): 15-NN preservation cpu {p_cpu:.4f} gpu {p_gpu:.4f}; "
           f"trustworthiness cpu {t_cpu:.4f} gpu {t_gpu:.4f}")
     assert abs(p_cpu - p_gpu) <= 0.01
     assert abs(t_cpu - t_gpu) <= 0.01

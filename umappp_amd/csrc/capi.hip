@@ -313,7 +313,7 @@ struct umx_layout {
     umx::SellState<double> sell_d;
     bool sell_built = false;
     int home = kHomeCsr;            // which schedule copy holds the latest state
-    int tile_cols = 16;
+    int tile_cols = 8;
     DevBuf<double> trace;                 // update trace records (parity mode)
     DevBuf<unsigned long long> trace_len;
     int64_t trace_cap = 0;

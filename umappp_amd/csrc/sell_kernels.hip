@@ -185,7 +185,7 @@ __global__ void __launch_bounds__(kWave) k_layout_sell(LayoutView<T> v, EpochCon
 #pragma unroll
             for (int u = 0; u < kNegFetch; ++u) {
                 kk[u] = u < nneg ? draw_index(ej, static_cast<uint32_t>(u), n32) : -1;
-                q[u] = load_point<T, D>(v.emb, kk[u] >= 0 ? kk[u] : i);
+                if (kk[u] >= 0) q[u] = load_point<T, D>(v.emb, kk[u]);
             }
 #pragma unroll
             for (int half = 0; half < 2; ++half) {
@@ -209,7 +209,7 @@ __global__ void __launch_bounds__(kWave) k_layout_sell(LayoutView<T> v, EpochCon
 #pragma unroll
                     for (int u = 0; u < kNegFetch; ++u) {
                         kk[u] = p0 + u < nneg ? draw_index(ej, static_cast<uint32_t>(p0 + u), n32) : -1;
-                        q[u] = load_point<T, D>(v.emb, kk[u] >= 0 ? kk[u] : i);
+                        if (kk[u] >= 0) q[u] = load_point<T, D>(v.emb, kk[u]);
                     }
                 }
 #pragma unroll
@@ -238,7 +238,8 @@ __global__ void __launch_bounds__(kWave) k_layout_sell(LayoutView<T> v, EpochCon
             tile.nx[k][lane] = nxj + epsj;
             tile.nn[k][lane] = nnj + static_cast<T>(nneg) * epns;
         }
-        // coalesced write-back of the columns that fired somewhere in the wave
+        // write-back of this lane's fired columns (coalesced across the lanes
+        // that fired in the same column)
 #pragma unroll
         for (int k = 0; k < TW; ++k) {
             if (k < tw && (fired >> k) & 1u) {
