@@ -100,8 +100,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU layout sample time")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU layout sample time per mode")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 512 MiB device-to-device copy three times (HBM counter calibration)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+
+        a = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 4, "copies": 3}))
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -185,7 +198,7 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": (traffic or {}).get("bytes_per_launch"),
-                "kernel": "k_layout_rows<float,2>", "avg_launch_us": round(avg_launch_s * 1e6, 2),
+                "kernel": "k_layout_sell<float,2,8,false>", "avg_launch_us": round(avg_launch_s * 1e6, 2),
                 "algorithmic_bytes_per_launch": round(avg_bytes)}
 
     cpu = None
@@ -217,6 +230,16 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def _cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(_x, kidx, kdist, n, k, T, nsr, a, b, target_s, U_gpu):
     """The reference CPU path (the oracle's restatement) on the host cores:
     graph build on the full input, then the conflict-scheduled parallel
@@ -237,25 +260,40 @@ def cpu_baseline(_x, kidx, kdist, n, k, T, nsr, a, b, target_s, U_gpu):
     t3 = time.perf_counter()
     graph_s = t3 - t0
     log(f"cpu graph build {graph_s:.2f}s (similarities {t1 - t0:.2f}, union {t2 - t1:.2f}, epochs {t3 - t2:.2f})")
-    lay = oracle.Layout(ep, T, nsr, rng_kind=oracle.RNG_MT, seed=1234567890, dtype=np.float32)
-    emb = np.random.default_rng(7).uniform(-10, 10, size=(n, 2)).astype(np.float32)
-    done, spent = 0, 0.0
-    # epoch 0 never fires; run one epoch at a time until the sample budget is used
-    while done < T and spent < target_s:
-        ts = time.perf_counter()
-        lay.run(emb, a, b, epoch_limit=done + 1, nthreads=cores)
-        spent += time.perf_counter() - ts
-        done += 1
-    _, _, _, tal = lay.state()
-    u_sample = int(tal[:, 0].sum())
-    layout_rate = u_sample / spent if spent > 0 else 0.0
+    def sample(nthreads):
+        lay = oracle.Layout(ep, T, nsr, rng_kind=oracle.RNG_MT, seed=1234567890, dtype=np.float32)
+        emb = np.random.default_rng(7).uniform(-10, 10, size=(n, 2)).astype(np.float32)
+        done, spent = 0, 0.0
+        # epoch 0 never fires; run one epoch at a time until the sample budget is used
+        while done < T and spent < target_s:
+            ts = time.perf_counter()
+            lay.run(emb, a, b, epoch_limit=done + 1, nthreads=nthreads)
+            spent += time.perf_counter() - ts
+            done += 1
+        _, _, _, tal = lay.state()
+        u = int(tal[:, 0].sum())
+        log(f"cpu layout ({nthreads} thread(s)): {u} edge-updates in {spent:.2f}s over epochs 0..{done - 1}")
+        return u, spent, done
+
+    u_par, s_par, d_par = sample(cores)
+    u_ser, s_ser, d_ser = sample(1)
+    rate_par = u_par / s_par if s_par > 0 else 0.0
+    rate_ser = u_ser / s_ser if s_ser > 0 else 0.0
+    if rate_ser >= rate_par:
+        layout_rate, used, u_sample, spent, done, mode = rate_ser, 1, u_ser, s_ser, d_ser, "serial optimize_layout"
+    else:
+        layout_rate, used, u_sample, spent, done, mode = rate_par, cores, u_par, s_par, d_par, \
+            f"optimize_layout_parallel ({cores} threads)"
     # whole-path estimate at the GPU's edge-update count
     est_total_s = graph_s + (U_gpu / layout_rate if layout_rate else float("inf"))
-    return {"value": round(U_gpu / est_total_s, 1), "unit": "edge-updates/s", "cores": cores, "kind": "port",
-            "sample": f"oracle graph build on the full input ({graph_s:.2f}s) + parallel layout epochs 0..{done - 1} "
+    return {"value": round(U_gpu / est_total_s, 1), "unit": "edge-updates/s", "cores": used, "kind": "port",
+            "sample": f"oracle graph build on the full input ({graph_s:.2f}s, {cores} threads for similarities) + "
+                      f"{mode}, the faster of serial and {cores}-thread parallel, epochs 0..{done - 1} "
                       f"({u_sample} edge-updates in {spent:.2f}s, {layout_rate:.3g}/s), "
                       f"extrapolated to the GPU's {U_gpu} edge-updates",
-            "graph_s": round(graph_s, 3), "layout_rate": round(layout_rate, 1)}
+            "graph_s": round(graph_s, 3), "layout_rate": round(layout_rate, 1),
+            "layout_rate_serial": round(rate_ser, 1), "layout_rate_parallel": round(rate_par, 1),
+            "host_cpu": _cpu_model()}
 
 
 if __name__ == "__main__":
