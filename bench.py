@@ -38,6 +38,9 @@ CONFIGS = {
                label="1M x 50 f32, 100-component mixture, exact kNN k=15, D=2, 200 epochs, nsr 5"),
     "c2": dict(n=100_000, dim=50, ncomp=30, seed=2, k=15, epochs=200,
                label="100k x 50 f32, 30-component mixture, exact kNN k=15, D=2, 200 epochs, nsr 5"),
+    "c4": dict(count=64, lo=2_000, hi=200_000, dim=30, ncomp=10, seed=4, k=15, epochs=200,
+               label="64 independent UMAPs, sizes log-uniform in [2k, 200k] (seed 4), each 30 dims f32 from "
+                     "its own 10-component mixture (seed 4000+i), exact kNN k=15, D=2, 200 epochs, nsr 5"),
 }
 
 
@@ -127,6 +130,9 @@ def main():
 
     umx.lib()
     dev = torch.device("cuda:0")
+    if args.config == "c4":
+        main_batched(args, umx, torch, dev)
+        return
     cfg = CONFIGS[args.config]
     n, k, T = cfg["n"], cfg["k"], cfg["epochs"]
     nsr = 5.0
@@ -227,6 +233,159 @@ def main():
     }
     if cpu and cpu.get("value"):
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+def knn_low(y, k, torch):
+    """Exact k nearest neighbours (self excluded) of a 2-D embedding on the GPU."""
+    n = y.shape[0]
+    out = torch.empty((n, k), dtype=torch.int64, device=y.device)
+    sq = (y * y).sum(1)
+    for s in range(0, n, 8192):
+        e = min(n, s + 8192)
+        d = sq[s:e, None] + sq[None, :] - 2.0 * (y[s:e] @ y.T)
+        d[torch.arange(e - s, device=y.device), torch.arange(s, e, device=y.device)] = float("inf")
+        out[s:e] = torch.topk(d, k, dim=1, largest=False).indices
+    return out
+
+
+def preservation(hi_idx, y, k, torch):
+    """Mean fraction of each observation's k high-dimensional neighbours that
+    are among its k nearest in the embedding."""
+    lo = knn_low(y, k, torch)
+    hits = (hi_idx.long()[:, :, None] == lo[:, None, :]).any(2).sum(1)
+    return float(hits.double().mean()) / k
+
+
+def main_batched(args, umx, torch, dev):
+    """Config C4: 64 independent problems run one after another through the
+    plain path (graph build + 200 epochs each) versus all at once through the
+    batched path (one graph build, one launch per epoch for every instance,
+    epochs replayed from captured HIP graphs, launch shape from the
+    autotuner's per-shape cache)."""
+    cfg = CONFIGS["c4"]
+    k, T, nsr = cfg["k"], cfg["epochs"], 5.0
+    a, b = umx.find_ab(1.0, 0.1)
+    rng = np.random.default_rng(cfg["seed"])
+    sizes = np.exp(rng.uniform(np.log(cfg["lo"]), np.log(cfg["hi"]), size=cfg["count"])).astype(np.int64)
+    log(f"generating {cfg['label']}: {int(sizes.sum())} observations")
+    idx_l, dist_l = [], []
+    for q, nq in enumerate(sizes):
+        x = make_mixture(4000 + q, int(nq), cfg["dim"], cfg["ncomp"])
+        i_, d_ = knn_torch(x, k, dev)
+        idx_l.append(i_)
+        dist_l.append(d_)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    N = int(off[-1])
+    cat_idx = torch.cat([t.reshape(-1) for t in idx_l]).contiguous()
+    cat_dist = torch.cat([t.reshape(-1) for t in dist_l]).contiguous()
+    rp = torch.arange(N + 1, dtype=torch.int64, device=dev) * k
+    rps = [torch.arange(int(nq) + 1, dtype=torch.int64, device=dev) * k for nq in sizes]
+    seeds = np.array([1234567890 + q for q in range(len(sizes))], dtype=np.uint64)
+    emb0 = torch.from_numpy(np.random.default_rng(44).uniform(-10, 10, size=(N, 2)).astype(np.float32)).to(dev)
+    out_plain = torch.empty_like(emb0)
+    out_batch = torch.empty_like(emb0)
+    torch.cuda.synchronize()
+    eb = 4  # bytes per f32
+
+    def plain_runs():
+        u = 0
+        for q, nq in enumerate(sizes):
+            e0 = int(off[q]) * k
+            g = umx.Graph((rps[q].data_ptr(), cat_idx.data_ptr() + 4 * e0, cat_dist.data_ptr() + eb * e0),
+                          device_ptrs=True, n=int(nq), dtype=np.float32)
+            g.neighbor_similarities(1.0, 1.0, 1e-3, True)
+            g.combine(1.0)
+            lay = umx.Layout(graph=g, num_epochs=T, negative_sample_rate=nsr)
+            lay.set_params(2, a, b, 1.0, 1.0, int(seeds[q]))
+            lay.set_embedding(device_ptr=emb0.data_ptr() + 2 * eb * int(off[q]))
+            lay.run(T)
+            lay.get_embedding(device_ptr=out_plain.data_ptr() + 2 * eb * int(off[q]))
+            u += int(lay.tallies()[:, 0].sum())
+        return u
+
+    def batched_run(timing=False):
+        g = umx.Graph((rp.data_ptr(), cat_idx.data_ptr(), cat_dist.data_ptr()), device_ptrs=True, n=N,
+                      dtype=np.float32, inst_offsets=off)
+        g.neighbor_similarities(1.0, 1.0, 1e-3, True)
+        g.combine(1.0)
+        lay = umx.Layout(graph=g, num_epochs=np.full(len(sizes), T, np.int32), negative_sample_rate=nsr)
+        lay.set_params(2, a, b, 1.0, 1.0, 0)
+        lay.set_instance_seeds(seeds)
+        lay.set_embedding(device_ptr=emb0.data_ptr())
+        shape = lay.autotune(3)
+        lay.set_graph_block(0 if timing else 25)
+        lay.set_timing(timing)
+        lay.run(T)
+        lay.get_embedding(device_ptr=out_batch.data_ptr())
+        return lay, shape
+
+    for w in range(args.warmup):
+        log(f"warmup {w + 1}/{args.warmup}")
+        plain_runs()
+        batched_run()
+    torch.cuda.synchronize()
+    seq_steps = max(1, min(args.steps, 3))
+    t0 = time.perf_counter()
+    for _ in range(seq_steps):
+        U_plain = plain_runs()
+    torch.cuda.synchronize()
+    seq_ms = 1000.0 * (time.perf_counter() - t0) / seq_steps
+    log(f"64 sequential plain runs: {seq_ms:.1f} ms")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        lay, shape = batched_run()
+    torch.cuda.synchronize()
+    ms_per_step = 1000.0 * (time.perf_counter() - t0) / args.steps
+    log(f"batched run: {ms_per_step:.1f} ms (launch shape tile/waves/groups {shape})")
+    tal = lay.tallies()
+    U = int(tal[:, 0].sum())
+    _, E, _, _ = lay.info()
+    value = U / (ms_per_step / 1000.0)
+
+    # roofline of the batched epoch kernel (one more run with per-launch events)
+    lay_t, _ = batched_run(timing=True)
+    kernel_ms, launches = lay_t.timing()
+    F = tal[:, 0].astype(np.float64)
+    Se = tal[:, 1].astype(np.float64)
+    D = 2
+    avg_bytes = float((4.0 * E + F * 20.0 + F * 4 * 4 * D + Se * 4 * D).mean())
+    avg_launch_s = (kernel_ms / 1000.0) / max(1, launches)
+    achieved = avg_bytes / avg_launch_s / 1e9
+
+    # quality: 15-NN preservation of every instance, batched vs plain
+    pres_b, pres_p = [], []
+    for q in range(len(sizes)):
+        r0, r1 = int(off[q]), int(off[q + 1])
+        pres_b.append(preservation(idx_l[q], out_batch[r0:r1], k, torch))
+        pres_p.append(preservation(idx_l[q], out_plain[r0:r1], k, torch))
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "edge-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"batched UMAP, {cfg['label']}", "instances": len(sizes), "n_total": N,
+                   "sizes_min_max": [int(sizes.min()), int(sizes.max())], "k": k, "epochs": T, "num_dim": 2,
+                   "stored_edges": int(E), "edge_updates": U, "launch_shape_tile_waves_groups": list(shape)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "k_layout_sell<float,2,*,false> (batched)",
+                     "avg_launch_us": round(avg_launch_s * 1e6, 2), "algorithmic_bytes_per_launch": round(avg_bytes)},
+        "cpu_baseline": None,
+        "sequential_plain_ms": round(seq_ms, 3),
+        "sequential_plain_edge_updates": U_plain,
+        "speedup_vs_sequential": round(seq_ms / ms_per_step, 2),
+        "knn15_preservation_batched": round(float(np.mean(pres_b)), 4),
+        "knn15_preservation_plain": round(float(np.mean(pres_p)), 4),
+    }
     print(json.dumps(out), flush=True)
 
 

@@ -104,6 +104,23 @@ int umx_graph_export(const umx_graph* g, int64_t* row_ptr, int32_t* idx, void* v
 int umx_graph_export_rho_sigma(const umx_graph* g, double* rho, double* sigma);
 void umx_graph_destroy(umx_graph* g);
 
+/* Batched graph (no counterpart in the reference, which runs one problem per
+ * initialize() call): n_inst independent kNN graphs stored one after the
+ * other.  Instance q owns observations [inst_offsets[q], inst_offsets[q+1])
+ * (inst_offsets: host int64[n_inst + 1], inst_offsets[0] = 0); row_ptr/val
+ * are the concatenated CSR over all observations, and idx holds
+ * instance-local neighbour indices, exactly as each instance's own
+ * NeighborList would (initialize.hpp:69-70).  The resident graph stores
+ * global indices (local + the instance's offset); neighbor_similarities and
+ * combine_neighbor_sets then act on every instance at once, and umx_layout_*
+ * runs all instances in the same launches.  A neighbour index outside its
+ * instance is UMX_ERR_GRAPH. */
+int umx_graph_create_batch(int dtype, int32_t n_inst, const int64_t* inst_offsets, const int64_t* row_ptr,
+                           const int32_t* idx, const void* val, int device_ptrs, umx_graph** out);
+/* Number of instances and (if inst_offsets is not NULL) their offsets
+ * (int64[n_inst + 1]).  A plain graph is one instance. */
+int umx_graph_instances(const umx_graph* g, int32_t* n_inst, int64_t* inst_offsets);
+
 /* ------------------------------------------------------------------ */
 /* Layout handle: EpochData + SGD epochs                               */
 /* ------------------------------------------------------------------ */
@@ -115,6 +132,24 @@ int umx_layout_create(const umx_graph* g, int num_epochs, double negative_sample
 /* Same from a symmetric weighted CSR (host or GPU pointers). */
 int umx_layout_create_csr(int dtype, int64_t n, const int64_t* row_ptr, const int32_t* idx, const void* val,
                           int device_ptrs, int num_epochs, double negative_sample_rate, umx_layout** out);
+/* similarities_to_epochs per instance of a batched graph: num_epochs[q]
+ * (host int32[n_inst]) for instance q, whose largest weight and survivor
+ * limit are its own (optimize_layout.hpp:49-63); umx_layout_create on a
+ * batched graph gives every instance the same num_epochs.  Each instance has
+ * its own epoch counter and learning-rate decay; umx_layout_run(L, limit)
+ * advances every instance to min(limit, num_epochs[q]), and the handle's
+ * total_epochs is the largest num_epochs[q].  Batched layouts run the
+ * pull-mode sliced-ELL kernel only (num_dim <= 4). */
+int umx_layout_create_batch(const umx_graph* g, const int32_t* num_epochs, double negative_sample_rate,
+                            umx_layout** out);
+/* Instances of a layout: count, observation offsets (int64[n_inst + 1]) and
+ * num_epochs (int32[n_inst]); any output may be NULL. */
+int umx_layout_instances(const umx_layout* L, int32_t* n_inst, int64_t* inst_offsets, int32_t* num_epochs);
+/* Per-instance optimize_seed (host uint64[n_inst]); by default every
+ * instance uses the seed of umx_layout_set_params.  An instance's negative
+ * samples depend only on its seed and its own schedule entries, so a batched
+ * instance draws the same samples as the same problem run alone. */
+int umx_layout_set_instance_seeds(umx_layout* L, const uint64_t* seeds);
 /* Sizes and progress: Status::num_observations / num_epochs / epoch
  * (Status.hpp:63-87). */
 int umx_layout_info(const umx_layout* L, int64_t* n, int64_t* nnz, int* total_epochs, int* current_epoch);
@@ -154,9 +189,33 @@ int umx_layout_set_work_split(umx_layout* L, double fires_per_item, int edges_pe
 /* LDS tile of the pull kernel: schedule columns staged per wavefront at a
  * time (4, 8, 16 or 32; default 8). */
 int umx_layout_set_tile(umx_layout* L, int tile_cols);
+/* Launch shape of the production kernel: 'waves' wavefronts (1, 2 or 4)
+ * per workgroup, and 'groups' placement groups: with groups > 1 the
+ * instances are dealt to that many groups of about equal expected work and
+ * workgroup b only runs items of group b % groups.  Workgroups are dispatched
+ * to the XCDs round-robin, so with groups = the number of XCDs each XCD's L2
+ * holds the embeddings of its own instances only.  groups = 0 (default)
+ * picks the XCD count when the instances can be balanced across it, else 1. */
+int umx_layout_set_placement(umx_layout* L, int waves, int groups);
+/* Epochs per captured HIP graph: umx_layout_run records 'epochs' launches
+ * of the production kernel once (each launch reads the epoch from a counter
+ * in device memory and a one-thread kernel advances it, so one graph serves
+ * every block of epochs) and replays it; 0 launches every epoch directly.
+ * Default 0; timing and tracing always launch directly. */
+int umx_layout_set_graph_block(umx_layout* L, int epochs);
+/* Autotuner for the production kernel's launch shape: from the current
+ * state, runs a few warm-up epochs, then trial_epochs epochs for each
+ * candidate (placement groups, waves per workgroup, LDS tile, by coordinate
+ * descent), timing each with HIP events, restores the embedding, schedule and
+ * tallies it started from, and keeps the fastest.  Results are cached per
+ * process by problem shape (dtype, num_dim, n, entries, instances, epochs),
+ * so later handles of the same shape take the cached choice at no cost.
+ * Outputs (may be NULL) receive the chosen tile, waves and groups. */
+int umx_layout_autotune(umx_layout* L, int trial_epochs, int* tile_cols, int* waves, int* groups);
 /* The current work items (built on demand), in run order: first edge,
  * length, row (top bit set for rows cut into several items).  *count
- * receives the number of items; up to cap are copied. */
+ * receives the number of items; up to cap are copied.  With waves > 1 or
+ * groups > 1 the list holds padding items of length 0. */
 int umx_layout_work_items(umx_layout* L, int64_t* begin, int32_t* length, int32_t* row, int64_t cap,
                           int64_t* count);
 /* Deterministic one-thread replay of optimize_layout (optimize_layout.hpp:

@@ -95,6 +95,15 @@ def _declare(L):
         "umx_layout_set_timing": (C.c_int, [_vp, C.c_int]),
         "umx_layout_timing": (C.c_int, [_vp, _f64p, _i64p]),
         "umx_layout_destroy": (None, [_vp]),
+        "umx_graph_create_batch": (C.c_int, [C.c_int, C.c_int32, _vp, _vp, _vp, _vp, C.c_int, C.POINTER(_vp)]),
+        "umx_graph_instances": (C.c_int, [_vp, _i32p, _vp]),
+        "umx_layout_create_batch": (C.c_int, [_vp, _vp, C.c_double, C.POINTER(_vp)]),
+        "umx_layout_instances": (C.c_int, [_vp, _i32p, _vp, _vp]),
+        "umx_layout_set_instance_seeds": (C.c_int, [_vp, _vp]),
+        "umx_layout_set_placement": (C.c_int, [_vp, C.c_int, C.c_int]),
+        "umx_layout_set_graph_block": (C.c_int, [_vp, C.c_int]),
+        "umx_layout_autotune": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                          C.POINTER(C.c_int)]),
         "umx_find_ab": (C.c_int, [C.c_double, C.c_double, _f64p, _f64p]),
         "umx_choose_num_epochs": (C.c_int, [C.c_int, C.c_int64]),
     }
@@ -195,12 +204,27 @@ class NeighborList:
 # Graph / Layout handles (thin owners of the C handles)
 # ---------------------------------------------------------------------------
 class Graph:
-    """A kNN graph resident on the GPU (umx_graph)."""
+    """A kNN graph resident on the GPU (umx_graph).
 
-    def __init__(self, x: NeighborList, device_ptrs: bool = False, n: int = None, dtype=None):
+    ``Graph.batch([...])`` builds a batched graph of independent instances
+    (one NeighborList each, instance-local indices); the resident graph then
+    holds global indices (local + the instance's offset).
+    """
+
+    def __init__(self, x: NeighborList, device_ptrs: bool = False, n: int = None, dtype=None,
+                 inst_offsets: np.ndarray = None):
         L = lib()
         h = C.c_void_p()
-        if device_ptrs:
+        if inst_offsets is not None:
+            off = np.ascontiguousarray(inst_offsets, dtype=np.int64)
+            if device_ptrs:
+                rp, idx, val = x
+                _check(L.umx_graph_create_batch(_dtype_code(dtype), off.size - 1, _ptr(off), C.c_void_p(rp),
+                                                C.c_void_p(idx), C.c_void_p(val), 1, C.byref(h)))
+            else:
+                _check(L.umx_graph_create_batch(_dtype_code(x.val.dtype), off.size - 1, _ptr(off), _ptr(x.row_ptr),
+                                                _ptr(x.idx), _ptr(x.val), 0, C.byref(h)))
+        elif device_ptrs:
             # x is a tuple of raw device pointers (row_ptr, idx, val)
             rp, idx, val = x
             _check(L.umx_graph_create(_dtype_code(dtype), int(n), C.c_void_p(rp), C.c_void_p(idx),
@@ -218,6 +242,29 @@ class Graph:
         if h is not None and h.value and _lib_handle is not None:
             _lib_handle.umx_graph_destroy(h)
             self._h = None
+
+    @classmethod
+    def batch(cls, lists: Sequence[NeighborList]) -> "Graph":
+        """Batched graph of independent kNN graphs (all of one float type)."""
+        sizes = [x.size() for x in lists]
+        off = np.zeros(len(lists) + 1, dtype=np.int64)
+        off[1:] = np.cumsum(sizes)
+        rp = [np.zeros(1, dtype=np.int64)]
+        base = 0
+        for x in lists:
+            rp.append(x.row_ptr[1:] + base)
+            base += int(x.row_ptr[-1])
+        cat = NeighborList(np.concatenate(rp), np.concatenate([x.idx for x in lists]),
+                           np.concatenate([x.val for x in lists]))
+        return cls(cat, inst_offsets=off)
+
+    def instances(self) -> np.ndarray:
+        """Observation offsets of the instances (n_inst + 1 entries)."""
+        k = C.c_int32()
+        _check(lib().umx_graph_instances(self._h, C.byref(k), None))
+        off = np.empty(k.value + 1, dtype=np.int64)
+        _check(lib().umx_graph_instances(self._h, C.byref(k), _ptr(off)))
+        return off
 
     @property
     def handle(self):
@@ -271,7 +318,11 @@ class Layout:
                  csr: NeighborList = None):
         L = lib()
         h = C.c_void_p()
-        if graph is not None:
+        if graph is not None and np.ndim(num_epochs) == 1:
+            ep = np.ascontiguousarray(num_epochs, dtype=np.int32)
+            _check(L.umx_layout_create_batch(graph.handle, _ptr(ep), float(negative_sample_rate), C.byref(h)))
+            self.dtype = graph.dtype
+        elif graph is not None:
             _check(L.umx_layout_create(graph.handle, int(num_epochs), float(negative_sample_rate), C.byref(h)))
             self.dtype = graph.dtype
         else:
@@ -384,6 +435,33 @@ class Layout:
         if total.value > self._trace_cap:
             raise UmxError(-6, f"trace overflow: {total.value} steps, capacity {self._trace_cap}")
         return out[: total.value]
+
+    def instances(self):
+        """(offsets[n_inst + 1], num_epochs[n_inst]) of the layout's instances."""
+        k = C.c_int32()
+        _check(lib().umx_layout_instances(self._h, C.byref(k), None, None))
+        off = np.empty(k.value + 1, dtype=np.int64)
+        ep = np.empty(k.value, dtype=np.int32)
+        _check(lib().umx_layout_instances(self._h, C.byref(k), _ptr(off), _ptr(ep)))
+        return off, ep
+
+    def set_instance_seeds(self, seeds):
+        s = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64))
+        _check(lib().umx_layout_set_instance_seeds(self._h, _ptr(s)))
+
+    def set_placement(self, waves: int = 1, groups: int = 0):
+        """Wavefronts per workgroup (1, 2, 4) and placement groups (0 = auto, 1 = off)."""
+        _check(lib().umx_layout_set_placement(self._h, int(waves), int(groups)))
+
+    def set_graph_block(self, epochs: int):
+        """Epochs per captured HIP graph (0 = direct launches)."""
+        _check(lib().umx_layout_set_graph_block(self._h, int(epochs)))
+
+    def autotune(self, trial_epochs: int = 3):
+        """Runs (or takes from the per-process cache) the launch-shape search; returns (tile, waves, groups)."""
+        t, w, g = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().umx_layout_autotune(self._h, int(trial_epochs), C.byref(t), C.byref(w), C.byref(g)))
+        return t.value, w.value, g.value
 
     def set_timing(self, on: bool):
         _check(lib().umx_layout_set_timing(self._h, 1 if on else 0))

@@ -10,8 +10,11 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <new>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -86,7 +89,19 @@ struct umx_graph {
     DevBuf<double> vd;
     DevBuf<double> rho, sigma;
     bool have_rho_sigma = false;
+    // instances (a plain graph is one instance)
+    std::vector<int64_t> inst_off;  // host, n_inst + 1
+    DevBuf<int64_t> d_inst_off;
+    DevBuf<int32_t> of_row;         // instance of each row (n_inst > 1 only)
 
+    int n_inst() const { return static_cast<int>(inst_off.size()) - 1; }
+    umx::Instances instances() const {
+        umx::Instances q;
+        q.n_inst = n_inst();
+        q.pt_off = d_inst_off.p;
+        q.of_row = n_inst() > 1 ? of_row.p : nullptr;
+        return q;
+    }
     void* val() { return dtype == UMX_F32 ? static_cast<void*>(vf.p) : static_cast<void*>(vd.p); }
     const void* val() const { return dtype == UMX_F32 ? static_cast<const void*>(vf.p) : static_cast<const void*>(vd.p); }
     ~umx_graph() {
@@ -125,6 +140,19 @@ void graph_upload(umx_graph* g, const int64_t* row_ptr, const int32_t* idx, cons
     g->max_degree = umx::max_row_length(g->rp.p, n, unused, g->s);
 }
 
+// Instance tables of a graph from host offsets.
+void graph_set_instances(umx_graph* g, const int64_t* off, int n_inst) {
+    g->inst_off.assign(off, off + n_inst + 1);
+    g->d_inst_off.alloc(static_cast<size_t>(n_inst + 1));
+    UMX_HIP(hipMemcpyAsync(g->d_inst_off.p, g->inst_off.data(), sizeof(int64_t) * (n_inst + 1), hipMemcpyHostToDevice,
+                           g->s));
+    if (n_inst > 1) {
+        g->of_row.alloc(static_cast<size_t>(g->n));
+        umx::fill_instance_of_row(g->n, g->instances(), g->of_row.p, g->s);
+    }
+    UMX_HIP(hipStreamSynchronize(g->s));
+}
+
 }  // namespace
 
 extern "C" {
@@ -145,7 +173,44 @@ int umx_graph_create(int dtype, int64_t n, const int64_t* row_ptr, const int32_t
         g->n = n;
         g->s = make_stream();
         graph_upload(g.get(), row_ptr, idx, val, device_ptrs);
+        const int64_t off[2] = {0, n};
+        graph_set_instances(g.get(), off, 1);
         *out = g.release();
+    });
+}
+
+int umx_graph_create_batch(int dtype, int32_t n_inst, const int64_t* inst_offsets, const int64_t* row_ptr,
+                           const int32_t* idx, const void* val, int device_ptrs, umx_graph** out) {
+    return guard([&] {
+        require(out != nullptr, UMX_ERR_ARGUMENT, "out is NULL");
+        *out = nullptr;
+        require(dtype == UMX_F32 || dtype == UMX_F64, UMX_ERR_ARGUMENT, "dtype must be UMX_F32 or UMX_F64");
+        require(n_inst >= 1 && inst_offsets != nullptr, UMX_ERR_ARGUMENT, "need n_inst >= 1 and inst_offsets");
+        require(inst_offsets[0] == 0, UMX_ERR_ARGUMENT, "inst_offsets[0] must be 0");
+        for (int q = 0; q < n_inst; ++q)
+            require(inst_offsets[q + 1] >= inst_offsets[q], UMX_ERR_ARGUMENT, "inst_offsets must be non-decreasing");
+        const int64_t n = inst_offsets[n_inst];
+        require(n < std::numeric_limits<int32_t>::max(), UMX_ERR_OVERFLOW, "n must fit a 32-bit index");
+        require(row_ptr != nullptr && idx != nullptr && (n == 0 || val != nullptr), UMX_ERR_ARGUMENT,
+                "row_ptr/idx/val is NULL");
+        std::unique_ptr<umx_graph> g(new umx_graph());
+        g->dtype = dtype;
+        g->n = n;
+        g->s = make_stream();
+        graph_upload(g.get(), row_ptr, idx, val, device_ptrs);
+        graph_set_instances(g.get(), inst_offsets, n_inst);
+        if (n_inst > 1)
+            require(umx::localize_to_global(n, g->rp.p, g->idx.p, g->instances(), g->s), UMX_ERR_GRAPH,
+                    "neighbour index outside its instance");
+        *out = g.release();
+    });
+}
+
+int umx_graph_instances(const umx_graph* g, int32_t* n_inst, int64_t* inst_offsets) {
+    return guard([&] {
+        require(g != nullptr, UMX_ERR_ARGUMENT, "graph is NULL");
+        if (n_inst) *n_inst = g->n_inst();
+        if (inst_offsets) std::copy(g->inst_off.begin(), g->inst_off.end(), inst_offsets);
     });
 }
 
@@ -321,10 +386,43 @@ struct umx_layout {
     double kernel_ms = 0;
     int64_t launches = 0;
     std::vector<hipEvent_t> events;
+    // instances (a plain layout is one instance)
+    std::vector<int64_t> inst_off;     // host, observations, n_inst + 1
+    std::vector<int64_t> inst_eoff;    // host, schedule entries, n_inst + 1
+    std::vector<int32_t> inst_epochs;  // host, num_epochs per instance
+    std::vector<uint64_t> inst_seeds;  // host; empty = the common seed
+    DevBuf<int64_t> d_inst_off;
+    DevBuf<int32_t> of_row;
+    DevBuf<int32_t> d_inst_epochs;
+    DevBuf<umx::InstConst> inst_tbl;   // per-instance kernel constants
+    DevBuf<float> alpha_f;             // learning rate per (instance, epoch)
+    DevBuf<double> alpha_d;
+    bool tables_ok = false;
+    DevBuf<int> epoch_ctr;             // epoch of the next launch of a captured block
+    DevBuf<int32_t> item_inst;
+    // launch shape of the production kernel
+    int waves = 1;
+    int groups_req = 0;                // 0 = auto
+    int groups = 1;                    // groups used by the current items
+    DevBuf<int32_t> group_of_inst;
+    // captured epoch blocks
+    int graph_block = 0;
+    hipGraphExec_t gexec = nullptr;
+    int gexec_block = 0;
+    std::vector<unsigned char> gexec_key;
 
     ~umx_layout() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
         for (auto e : events) (void)hipEventDestroy(e);
         if (s) (void)hipStreamDestroy(s);
+    }
+    int n_inst() const { return static_cast<int>(inst_off.size()) - 1; }
+    umx::Instances instances() const {
+        umx::Instances q;
+        q.n_inst = n_inst();
+        q.pt_off = d_inst_off.p;
+        q.of_row = n_inst() > 1 ? of_row.p : nullptr;
+        return q;
     }
     void* emb() { return dtype == UMX_F32 ? static_cast<void*>(mf.p) : static_cast<void*>(md.p); }
     const void* emb() const { return dtype == UMX_F32 ? static_cast<const void*>(mf.p) : static_cast<const void*>(md.p); }
@@ -333,8 +431,19 @@ struct umx_layout {
 namespace {
 
 template <typename T>
+DevBuf<T>& alpha_buf(umx_layout* L);
+template <>
+DevBuf<float>& alpha_buf<float>(umx_layout* L) {
+    return L->alpha_f;
+}
+template <>
+DevBuf<double>& alpha_buf<double>(umx_layout* L) {
+    return L->alpha_d;
+}
+
+template <typename T>
 umx::LayoutView<T> view_of(umx_layout* L, DevBuf<T>& eps, DevBuf<T>& nxt, DevBuf<T>& nng, DevBuf<T>& emb) {
-    umx::LayoutView<T> v;
+    umx::LayoutView<T> v{};  // value-initialised: padding is zero, so views compare bytewise
     v.n = L->n;
     v.nnz = L->nnz;
     v.rp = L->rp.p;
@@ -361,6 +470,11 @@ umx::LayoutView<T> view_of(umx_layout* L, DevBuf<T>& eps, DevBuf<T>& nxt, DevBuf
     v.item_len = L->item_len.p;
     v.item_row = L->item_row.p;
     v.n_items = L->n_items;
+    v.item_inst = L->item_inst.p;
+    v.inst = L->inst_tbl.p;
+    v.alpha_tbl = alpha_buf<T>(L).p;
+    v.alpha_stride = L->total_epochs;
+    v.epoch_ctr = L->epoch_ctr.p;
     v.trace = L->trace_cap > 0 ? L->trace.p : nullptr;
     v.trace_cap = L->trace_cap;
     v.trace_len = L->trace_len.p;
@@ -378,15 +492,120 @@ void attach_sell(umx::LayoutView<T>& v, umx::SellState<T>& st) {
     v.sell_nn = st.nn.p;
 }
 
+int device_xcds() {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeNumberOfXccs, dev) != hipSuccess || v < 1) return 1;
+    return v;
+}
+
+// Deals instances to 'groups' groups of about equal work: heaviest first, each
+// to the currently lightest group.  Returns the largest group's load over the
+// mean load.
+double deal_instances(const std::vector<double>& work, int groups, std::vector<int32_t>& group_of) {
+    const int ni = static_cast<int>(work.size());
+    std::vector<int> order(static_cast<size_t>(ni));
+    for (int q = 0; q < ni; ++q) order[q] = q;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work[x] > work[y]; });
+    std::vector<double> load(static_cast<size_t>(groups), 0.0);
+    group_of.assign(static_cast<size_t>(ni), 0);
+    double total = 0;
+    for (int q : order) {
+        const int g = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+        group_of[q] = g;
+        load[g] += work[q];
+        total += work[q];
+    }
+    const double mean = total / groups;
+    return mean > 0 ? *std::max_element(load.begin(), load.end()) / mean : 1.0;
+}
+
+template <typename T>
+void build_items_t(umx_layout* L, const T* eps) {
+    const int ni = L->n_inst();
+    int groups = ni > 1 ? L->groups_req : 1;
+    std::vector<int32_t> gof;
+    if (groups != 1) {
+        DevBuf<double> f;
+        f.alloc(static_cast<size_t>(ni));
+        umx::instance_fires<T>(L->rp.p, eps, L->instances(), f.p, L->s);
+        std::vector<double> work(static_cast<size_t>(ni));
+        UMX_HIP(hipMemcpyAsync(work.data(), f.p, sizeof(double) * ni, hipMemcpyDeviceToHost, L->s));
+        UMX_HIP(hipStreamSynchronize(L->s));
+        if (groups == 0) {
+            // automatic: one group per XCD when the instances balance across
+            // them within 15%
+            const int x = std::min(device_xcds(), 32);
+            groups = 1;
+            if (x > 1 && ni >= x && deal_instances(work, x, gof) <= 1.15) groups = x;
+        } else {
+            deal_instances(work, groups, gof);
+        }
+    }
+    umx::ItemPlacement pl;
+    pl.waves = L->waves;
+    pl.n_groups = groups;
+    if (groups > 1) {
+        L->group_of_inst.alloc(static_cast<size_t>(ni));
+        UMX_HIP(hipMemcpyAsync(L->group_of_inst.p, gof.data(), sizeof(int32_t) * ni, hipMemcpyHostToDevice, L->s));
+        pl.group_of_inst = L->group_of_inst.p;
+    }
+    umx::build_work_items<T>(L->n, L->rp.p, eps, L->fires_per_item, L->edges_per_item, L->instances(), pl,
+                             L->item_begin, L->item_len, L->item_row, L->item_inst, L->n_items, L->s);
+    L->groups = groups;
+}
+
 void ensure_items(umx_layout* L) {
     if (L->n_items >= 0) return;
-    if (L->dtype == UMX_F32)
-        umx::build_work_items<float>(L->n, L->rp.p, L->ef.p, L->fires_per_item, L->edges_per_item, L->item_begin,
-                                     L->item_len, L->item_row, L->n_items, L->s);
-    else
-        umx::build_work_items<double>(L->n, L->rp.p, L->ed.p, L->fires_per_item, L->edges_per_item, L->item_begin,
-                                      L->item_len, L->item_row, L->n_items, L->s);
+    if (L->dtype == UMX_F32) build_items_t<float>(L, L->ef.p);
+    else build_items_t<double>(L, L->ed.p);
     L->sell_built = false;
+}
+
+// Drops the work items (and the sliced-ELL copy built from them) after a
+// change of their parameters; the CSR schedule is made current first.
+void invalidate_items(umx_layout* L);
+
+// Learning rate of epoch n with the reference's types (optimize_layout.hpp:
+// 132-133): epoch / num_epochs in T, 1.0 - x in double, times alpha0 (T),
+// rounded to T.
+template <typename T>
+double epoch_alpha(double alpha0, int n, int total) {
+    const T ep = static_cast<T>(n);
+    const T a0 = static_cast<T>(alpha0);
+    return static_cast<double>(static_cast<T>(a0 * (1.0 - ep / static_cast<T>(total))));
+}
+
+// Per-instance constants and learning rates of the production kernel.
+template <typename T>
+void build_tables(umx_layout* L) {
+    const int ni = L->n_inst();
+    std::vector<umx::InstConst> tbl(static_cast<size_t>(ni));
+    for (int q = 0; q < ni; ++q) {
+        tbl[q].pt_off = L->inst_off[q];
+        tbl[q].edge_off = L->inst_eoff[q];
+        tbl[q].n = static_cast<uint32_t>(L->inst_off[q + 1] - L->inst_off[q]);
+        tbl[q].epochs = L->inst_epochs[q];
+        tbl[q].seed = L->inst_seeds.empty() ? L->seed : L->inst_seeds[q];
+    }
+    const size_t stride = static_cast<size_t>(L->total_epochs);
+    std::vector<T> al(static_cast<size_t>(ni) * stride, static_cast<T>(0));
+    for (int q = 0; q < ni; ++q)
+        for (int e = 0; e < L->inst_epochs[q]; ++e)
+            al[q * stride + e] = static_cast<T>(epoch_alpha<T>(L->alpha0, e, L->inst_epochs[q]));
+    L->inst_tbl.alloc(static_cast<size_t>(ni));
+    DevBuf<T>& ab = alpha_buf<T>(L);
+    ab.alloc(al.size());
+    UMX_HIP(hipMemcpyAsync(L->inst_tbl.p, tbl.data(), sizeof(umx::InstConst) * ni, hipMemcpyHostToDevice, L->s));
+    UMX_HIP(hipMemcpyAsync(ab.p, al.data(), sizeof(T) * al.size(), hipMemcpyHostToDevice, L->s));
+    UMX_HIP(hipStreamSynchronize(L->s));
+}
+
+void ensure_tables(umx_layout* L) {
+    if (L->tables_ok) return;
+    if (L->dtype == UMX_F32) build_tables<float>(L);
+    else build_tables<double>(L);
+    L->tables_ok = true;
 }
 
 // Makes the CSR schedule arrays current (copies the sliced-ELL state back).
@@ -418,33 +637,169 @@ void ensure_sell_current(umx_layout* L) {
     }
 }
 
-// Learning rate of epoch n with the reference's types (optimize_layout.hpp:
-// 132-133): epoch / num_epochs in T, 1.0 - x in double, times alpha0 (T),
-// rounded to T.
-template <typename T>
-double epoch_alpha(double alpha0, int n, int total) {
-    const T ep = static_cast<T>(n);
-    const T a0 = static_cast<T>(alpha0);
-    return static_cast<double>(static_cast<T>(a0 * (1.0 - ep / static_cast<T>(total))));
+void invalidate_items(umx_layout* L) {
+    ensure_csr_current(L);
+    L->n_items = -1;
+    L->sell_built = false;
+    L->home = kHomeCsr;
 }
 
-void layout_from_graph_arrays(umx_layout* L, const int64_t* rp, const int32_t* idx, const void* val, int num_epochs,
-                              double nsr) {
+__global__ void k_gather_rows(const int64_t* rp, const int64_t* at, int count, int64_t* out) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < count) out[q] = rp[at[q]];
+}
+
+// EpochData for every instance of graph g (similarities_to_epochs per
+// instance) plus the instance tables.
+void layout_from_graph(umx_layout* L, const umx_graph* g, const std::vector<int32_t>& epochs, double nsr) {
+    const int ni = g->n_inst();
+    L->inst_off = g->inst_off;
+    L->inst_epochs = epochs;
+    L->total_epochs = *std::max_element(epochs.begin(), epochs.end());
+    L->d_inst_off.alloc(static_cast<size_t>(ni + 1));
+    UMX_HIP(hipMemcpyAsync(L->d_inst_off.p, g->d_inst_off.p, sizeof(int64_t) * (ni + 1), hipMemcpyDeviceToDevice, L->s));
+    const int32_t* d_epochs = nullptr;
+    if (ni > 1) {
+        L->of_row.alloc(static_cast<size_t>(L->n));
+        UMX_HIP(hipMemcpyAsync(L->of_row.p, g->of_row.p, sizeof(int32_t) * static_cast<size_t>(L->n),
+                               hipMemcpyDeviceToDevice, L->s));
+        L->d_inst_epochs.alloc(static_cast<size_t>(ni));
+        UMX_HIP(hipMemcpyAsync(L->d_inst_epochs.p, epochs.data(), sizeof(int32_t) * ni, hipMemcpyHostToDevice, L->s));
+        d_epochs = L->d_inst_epochs.p;
+    }
+    const umx::Instances inst = L->instances();
     if (L->dtype == UMX_F32) {
-        umx::run_to_epochs<float>(L->n, rp, idx, static_cast<const float*>(val), num_epochs, static_cast<float>(nsr),
-                                  L->max_degree, L->rp, L->tg, L->ef, L->nf, L->gf, L->s);
+        umx::run_to_epochs<float>(L->n, g->rp.p, g->idx.p, static_cast<const float*>(g->val()), L->total_epochs,
+                                  d_epochs, inst, static_cast<float>(nsr), L->max_degree, L->rp, L->tg, L->ef, L->nf,
+                                  L->gf, L->s);
     } else {
-        umx::run_to_epochs<double>(L->n, rp, idx, static_cast<const double*>(val), num_epochs, nsr, L->max_degree,
-                                   L->rp, L->tg, L->ed, L->nd, L->gd, L->s);
+        umx::run_to_epochs<double>(L->n, g->rp.p, g->idx.p, static_cast<const double*>(g->val()), L->total_epochs,
+                                   d_epochs, inst, nsr, L->max_degree, L->rp, L->tg, L->ed, L->nd, L->gd, L->s);
     }
     UMX_HIP(hipMemcpyAsync(&L->nnz, L->rp.p + L->n, sizeof(int64_t), hipMemcpyDeviceToHost, L->s));
-    L->tally.alloc(static_cast<size_t>(3) * static_cast<size_t>(num_epochs > 0 ? num_epochs : 1));
+    L->tally.alloc(static_cast<size_t>(3) * static_cast<size_t>(L->total_epochs));
     UMX_HIP(hipMemsetAsync(L->tally.p, 0, sizeof(unsigned long long) * L->tally.n, L->s));
     L->cursor.alloc(2);
     UMX_HIP(hipMemsetAsync(L->cursor.p, 0, sizeof(unsigned long long) * 2, L->s));
+    L->epoch_ctr.alloc(1);
+    umx::set_epoch_counter(L->epoch_ctr.p, 0, L->s);
+    // first schedule entry of every instance
+    DevBuf<int64_t> eoff;
+    eoff.alloc(static_cast<size_t>(ni + 1));
+    k_gather_rows<<<(ni + 1 + 63) / 64, 64, 0, L->s>>>(L->rp.p, L->d_inst_off.p, ni + 1, eoff.p);
+    UMX_HIP(hipGetLastError());
+    L->inst_eoff.assign(static_cast<size_t>(ni + 1), 0);
+    UMX_HIP(hipMemcpyAsync(L->inst_eoff.data(), eoff.p, sizeof(int64_t) * (ni + 1), hipMemcpyDeviceToHost, L->s));
     DevBuf<int64_t> unused;
     L->max_degree = umx::max_row_length(L->rp.p, L->n, unused, L->s);
     UMX_HIP(hipStreamSynchronize(L->s));
+}
+
+void prepare_events(umx_layout* L, int count) {
+    if (!L->timing) return;
+    while (static_cast<int>(L->events.size()) < 2 * count) {
+        hipEvent_t e;
+        UMX_HIP(hipEventCreate(&e));
+        L->events.push_back(e);
+    }
+}
+
+void collect_events(umx_layout* L, int count) {
+    if (!L->timing) return;
+    for (int q = 0; q < count; ++q) {
+        float ms = 0;
+        UMX_HIP(hipEventElapsedTime(&ms, L->events[2 * q], L->events[2 * q + 1]));
+        L->kernel_ms += ms;
+    }
+    L->launches += count;
+}
+
+// One epoch of the sliced-ELL kernel: 'epoch' >= 0, or -1 to take it from
+// the device counter and advance that (captured blocks).
+void launch_sell_once(umx_layout* L, int epoch) {
+    if (L->dtype == UMX_F32) {
+        auto v = view_of<float>(L, L->ef, L->nf, L->gf, L->mf);
+        attach_sell(v, L->sell_f);
+        umx::launch_layout_sell<float>(v, epoch, L->tile_cols, L->waves, L->s);
+    } else {
+        auto v = view_of<double>(L, L->ed, L->nd, L->gd, L->md);
+        attach_sell(v, L->sell_d);
+        umx::launch_layout_sell<double>(v, epoch, L->tile_cols, L->waves, L->s);
+    }
+}
+
+// Identity of what a captured block would launch: the kernel arguments,
+// tile, waves and block length.
+std::vector<unsigned char> launch_key(umx_layout* L, int block) {
+    std::vector<unsigned char> k;
+    auto put = [&](const void* p, size_t n) {
+        const unsigned char* c = static_cast<const unsigned char*>(p);
+        k.insert(k.end(), c, c + n);
+    };
+    if (L->dtype == UMX_F32) {
+        auto v = view_of<float>(L, L->ef, L->nf, L->gf, L->mf);
+        attach_sell(v, L->sell_f);
+        put(&v, sizeof(v));
+    } else {
+        auto v = view_of<double>(L, L->ed, L->nd, L->gd, L->md);
+        attach_sell(v, L->sell_d);
+        put(&v, sizeof(v));
+    }
+    const int extra[3] = {L->tile_cols, L->waves, block};
+    put(extra, sizeof(extra));
+    return k;
+}
+
+// Replays 'block' epochs captured as one HIP graph (captured on first use and
+// whenever the launch changes).
+void replay_block(umx_layout* L, int block) {
+    std::vector<unsigned char> key = launch_key(L, block);
+    if (!L->gexec || L->gexec_key != key) {
+        if (L->gexec) {
+            UMX_HIP(hipGraphExecDestroy(L->gexec));
+            L->gexec = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        UMX_HIP(hipStreamBeginCapture(L->s, hipStreamCaptureModeThreadLocal));
+        try {
+            for (int q = 0; q < block; ++q) launch_sell_once(L, -1);
+        } catch (...) {
+            (void)hipStreamEndCapture(L->s, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        UMX_HIP(hipStreamEndCapture(L->s, &graph));
+        const hipError_t e = hipGraphInstantiate(&L->gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        UMX_HIP(e);
+        L->gexec_key.swap(key);
+    }
+    UMX_HIP(hipGraphLaunch(L->gexec, L->s));
+}
+
+// Epochs [first, first + count) with the sliced-ELL kernel.
+void run_sell(umx_layout* L, int first, int count) {
+    ensure_items(L);
+    ensure_sell_current(L);
+    ensure_tables(L);
+    if (L->timing) {
+        prepare_events(L, count);
+        for (int q = 0; q < count; ++q) {
+            UMX_HIP(hipEventRecord(L->events[2 * q], L->s));
+            launch_sell_once(L, first + q);
+            UMX_HIP(hipEventRecord(L->events[2 * q + 1], L->s));
+        }
+    } else if (L->graph_block > 0 && L->trace_cap == 0 && count >= L->graph_block) {
+        umx::set_epoch_counter(L->epoch_ctr.p, first, L->s);
+        int done = 0;
+        for (; count - done >= L->graph_block; done += L->graph_block) replay_block(L, L->graph_block);
+        for (; done < count; ++done) launch_sell_once(L, first + done);
+    } else {
+        for (int q = 0; q < count; ++q) launch_sell_once(L, first + q);
+    }
+    UMX_HIP(hipStreamSynchronize(L->s));
+    collect_events(L, count);
+    if (count > 0) L->home = kHomeSell;
 }
 
 void check_epoch_args(int num_epochs, double nsr, int dtype) {
@@ -462,22 +817,30 @@ void check_epoch_args(int num_epochs, double nsr, int dtype) {
 
 extern "C" {
 
-int umx_layout_create(const umx_graph* g, int num_epochs, double negative_sample_rate, umx_layout** out) {
+int umx_layout_create_batch(const umx_graph* g, const int32_t* num_epochs, double negative_sample_rate,
+                            umx_layout** out) {
     return guard([&] {
-        require(g != nullptr && out != nullptr, UMX_ERR_ARGUMENT, "NULL argument");
+        require(g != nullptr && out != nullptr && num_epochs != nullptr, UMX_ERR_ARGUMENT, "NULL argument");
         *out = nullptr;
-        check_epoch_args(num_epochs, negative_sample_rate, g->dtype);
+        const int ni = g->n_inst();
+        std::vector<int32_t> epochs(num_epochs, num_epochs + ni);
+        for (int q = 0; q < ni; ++q) check_epoch_args(epochs[q], negative_sample_rate, g->dtype);
         std::unique_ptr<umx_layout> L(new umx_layout());
         L->dtype = g->dtype;
         L->n = g->n;
-        L->total_epochs = num_epochs;
         L->nsr = g->dtype == UMX_F32 ? static_cast<double>(static_cast<float>(negative_sample_rate)) : negative_sample_rate;
         L->s = make_stream();
         L->max_degree = g->max_degree;
         UMX_HIP(hipStreamSynchronize(g->s));
-        layout_from_graph_arrays(L.get(), g->rp.p, g->idx.p, g->val(), num_epochs, negative_sample_rate);
+        layout_from_graph(L.get(), g, epochs, negative_sample_rate);
         *out = L.release();
     });
+}
+
+int umx_layout_create(const umx_graph* g, int num_epochs, double negative_sample_rate, umx_layout** out) {
+    if (g == nullptr) return umx_layout_create_batch(nullptr, nullptr, negative_sample_rate, out);
+    const std::vector<int32_t> epochs(static_cast<size_t>(g->n_inst()), num_epochs);
+    return umx_layout_create_batch(g, epochs.data(), negative_sample_rate, out);
 }
 
 int umx_layout_create_csr(int dtype, int64_t n, const int64_t* row_ptr, const int32_t* idx, const void* val,
@@ -543,6 +906,7 @@ int umx_layout_set_params(umx_layout* L, int num_dim, double a, double b, double
         L->alpha0 = learning_rate;
         L->seed = seed;
         L->params = true;
+        L->tables_ok = false;
     });
 }
 
@@ -574,53 +938,33 @@ int umx_layout_run(umx_layout* L, int epoch_limit) {
         require(epoch_limit >= L->current_epoch && epoch_limit <= L->total_epochs, UMX_ERR_ARGUMENT,
                 "epoch_limit must lie in [epoch(), num_epochs()]");
         const bool use_sell = L->mode == UMX_LAYOUT_PULL && L->dim <= 4;
-        ensure_items(L);
-        if (use_sell) ensure_sell_current(L);
-        else ensure_csr_current(L);
+        require(use_sell || L->n_inst() == 1, UMX_ERR_STATE,
+                "batched layouts run in pull mode with num_dim <= 4 only");
         const int first = L->current_epoch;
         const int count = epoch_limit - first;
-        if (L->timing && static_cast<int>(L->events.size()) < 2 * count) {
-            while (static_cast<int>(L->events.size()) < 2 * count) {
-                hipEvent_t e;
-                UMX_HIP(hipEventCreate(&e));
-                L->events.push_back(e);
-            }
-        }
-        for (int q = 0; q < count; ++q) {
-            const int ep = first + q;
-            umx::EpochArgs ea;
-            ea.epoch = ep;
-            ea.ekey = umx::counter_epoch_key(L->seed, ep);
-            if (L->timing) UMX_HIP(hipEventRecord(L->events[2 * q], L->s));
-            if (L->dtype == UMX_F32) {
-                ea.alpha = epoch_alpha<float>(L->alpha0, ep, L->total_epochs);
-                auto v = view_of<float>(L, L->ef, L->nf, L->gf, L->mf);
-                if (use_sell) {
-                    attach_sell(v, L->sell_f);
-                    umx::launch_layout_sell<float>(v, ea, L->tile_cols, L->s);
-                } else {
-                    umx::launch_layout_epoch<float>(v, ea, L->s);
-                }
-            } else {
-                ea.alpha = epoch_alpha<double>(L->alpha0, ep, L->total_epochs);
-                auto v = view_of<double>(L, L->ed, L->nd, L->gd, L->md);
-                if (use_sell) {
-                    attach_sell(v, L->sell_d);
-                    umx::launch_layout_sell<double>(v, ea, L->tile_cols, L->s);
-                } else {
-                    umx::launch_layout_epoch<double>(v, ea, L->s);
-                }
-            }
-            if (L->timing) UMX_HIP(hipEventRecord(L->events[2 * q + 1], L->s));
-        }
-        UMX_HIP(hipStreamSynchronize(L->s));
-        if (L->timing) {
+        if (use_sell) {
+            run_sell(L, first, count);
+        } else {
+            ensure_items(L);
+            ensure_csr_current(L);
+            prepare_events(L, count);
             for (int q = 0; q < count; ++q) {
-                float ms = 0;
-                UMX_HIP(hipEventElapsedTime(&ms, L->events[2 * q], L->events[2 * q + 1]));
-                L->kernel_ms += ms;
+                const int ep = first + q;
+                umx::EpochArgs ea;
+                ea.epoch = ep;
+                ea.ekey = umx::counter_epoch_key(L->seed, ep);
+                if (L->timing) UMX_HIP(hipEventRecord(L->events[2 * q], L->s));
+                if (L->dtype == UMX_F32) {
+                    ea.alpha = epoch_alpha<float>(L->alpha0, ep, L->total_epochs);
+                    umx::launch_layout_epoch<float>(view_of<float>(L, L->ef, L->nf, L->gf, L->mf), ea, L->s);
+                } else {
+                    ea.alpha = epoch_alpha<double>(L->alpha0, ep, L->total_epochs);
+                    umx::launch_layout_epoch<double>(view_of<double>(L, L->ed, L->nd, L->gd, L->md), ea, L->s);
+                }
+                if (L->timing) UMX_HIP(hipEventRecord(L->events[2 * q + 1], L->s));
             }
-            L->launches += count;
+            UMX_HIP(hipStreamSynchronize(L->s));
+            collect_events(L, count);
         }
         L->current_epoch = epoch_limit;
         if (count > 0) L->home = use_sell ? kHomeSell : kHomeCsr;
@@ -634,6 +978,7 @@ int umx_layout_run_serial(umx_layout* L, int epoch_limit, const int32_t* samples
         require(epoch_limit >= L->current_epoch && epoch_limit <= L->total_epochs, UMX_ERR_ARGUMENT,
                 "epoch_limit must lie in [epoch(), num_epochs()]");
         require(samples == nullptr || count >= 0, UMX_ERR_ARGUMENT, "negative sample count");
+        require(L->n_inst() == 1, UMX_ERR_STATE, "the serial replay runs plain (single-instance) layouts only");
         ensure_csr_current(L);
         if (epoch_limit > L->current_epoch) L->home = kHomeCsr;
         int32_t* dsamples = nullptr;
@@ -681,12 +1026,9 @@ int umx_layout_set_work_split(umx_layout* L, double fires_per_item, int edges_pe
         require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
         require(std::isfinite(fires_per_item) && fires_per_item > 0 && edges_per_item >= 1, UMX_ERR_ARGUMENT,
                 "fires_per_item must be > 0 and edges_per_item >= 1");
-        ensure_csr_current(L);
+        invalidate_items(L);  // rebuilt at the next run
         L->fires_per_item = fires_per_item;
         L->edges_per_item = edges_per_item;
-        L->n_items = -1;  // rebuilt at the next run
-        L->sell_built = false;
-        L->home = kHomeCsr;
     });
 }
 
@@ -772,6 +1114,203 @@ int umx_layout_timing(const umx_layout* L, double* kernel_ms, int64_t* launches)
         require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
         if (kernel_ms) *kernel_ms = L->kernel_ms;
         if (launches) *launches = L->launches;
+    });
+}
+
+int umx_layout_instances(const umx_layout* L, int32_t* n_inst, int64_t* inst_offsets, int32_t* num_epochs) {
+    return guard([&] {
+        require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
+        if (n_inst) *n_inst = L->n_inst();
+        if (inst_offsets) std::copy(L->inst_off.begin(), L->inst_off.end(), inst_offsets);
+        if (num_epochs) std::copy(L->inst_epochs.begin(), L->inst_epochs.end(), num_epochs);
+    });
+}
+
+int umx_layout_set_instance_seeds(umx_layout* L, const uint64_t* seeds) {
+    return guard([&] {
+        require(L != nullptr && seeds != nullptr, UMX_ERR_ARGUMENT, "NULL argument");
+        L->inst_seeds.assign(seeds, seeds + L->n_inst());
+        L->tables_ok = false;
+    });
+}
+
+int umx_layout_set_placement(umx_layout* L, int waves, int groups) {
+    return guard([&] {
+        require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
+        require(waves == 1 || waves == 2 || waves == 4, UMX_ERR_ARGUMENT, "waves must be 1, 2 or 4");
+        require(groups >= 0 && groups <= 32, UMX_ERR_ARGUMENT, "groups must lie in [0, 32]");
+        if (waves != L->waves || groups != L->groups_req) {
+            invalidate_items(L);
+            L->waves = waves;
+            L->groups_req = groups;
+        }
+    });
+}
+
+int umx_layout_set_graph_block(umx_layout* L, int epochs) {
+    return guard([&] {
+        require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
+        require(epochs >= 0, UMX_ERR_ARGUMENT, "epochs must be >= 0");
+        L->graph_block = epochs;
+    });
+}
+
+}  // extern "C"
+
+namespace {
+
+// State an autotuning trial starts from: embedding, CSR schedule and tallies.
+struct Snapshot {
+    DevBuf<unsigned char> emb, nx, nn, tally;
+    int epoch = 0;
+};
+
+void copy_dev(DevBuf<unsigned char>& to, const void* from, size_t bytes, hipStream_t s) {
+    to.alloc(bytes);
+    if (bytes) UMX_HIP(hipMemcpyAsync(to.p, from, bytes, hipMemcpyDeviceToDevice, s));
+}
+
+void take_snapshot(umx_layout* L, Snapshot& sn) {
+    ensure_csr_current(L);
+    const size_t es = dsize(L->dtype);
+    const bool f = L->dtype == UMX_F32;
+    copy_dev(sn.emb, L->emb(), es * static_cast<size_t>(L->n) * L->dim, L->s);
+    copy_dev(sn.nx, f ? static_cast<void*>(L->nf.p) : L->nd.p, es * static_cast<size_t>(L->nnz), L->s);
+    copy_dev(sn.nn, f ? static_cast<void*>(L->gf.p) : L->gd.p, es * static_cast<size_t>(L->nnz), L->s);
+    copy_dev(sn.tally, L->tally.p, sizeof(unsigned long long) * L->tally.n, L->s);
+    sn.epoch = L->current_epoch;
+    UMX_HIP(hipStreamSynchronize(L->s));
+}
+
+void restore_snapshot(umx_layout* L, const Snapshot& sn) {
+    const size_t es = dsize(L->dtype);
+    const bool f = L->dtype == UMX_F32;
+    auto back = [&](void* to, const DevBuf<unsigned char>& from, size_t bytes) {
+        if (bytes) UMX_HIP(hipMemcpyAsync(to, from.p, bytes, hipMemcpyDeviceToDevice, L->s));
+    };
+    back(L->emb(), sn.emb, es * static_cast<size_t>(L->n) * L->dim);
+    back(f ? static_cast<void*>(L->nf.p) : L->nd.p, sn.nx, es * static_cast<size_t>(L->nnz));
+    back(f ? static_cast<void*>(L->gf.p) : L->gd.p, sn.nn, es * static_cast<size_t>(L->nnz));
+    back(L->tally.p, sn.tally, sizeof(unsigned long long) * L->tally.n);
+    UMX_HIP(hipStreamSynchronize(L->s));
+    L->current_epoch = sn.epoch;
+    L->home = kHomeCsr;
+}
+
+struct TuneChoice {
+    int tile, waves, groups;
+};
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int64_t, int64_t, int, int>, TuneChoice> g_tune_cache;
+
+void apply_choice(umx_layout* L, const TuneChoice& c) {
+    L->tile_cols = c.tile;
+    if (c.waves != L->waves || c.groups != L->groups_req) {
+        invalidate_items(L);
+        L->waves = c.waves;
+        L->groups_req = c.groups;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int umx_layout_autotune(umx_layout* L, int trial_epochs, int* tile_cols, int* waves, int* groups) {
+    return guard([&] {
+        require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
+        require(trial_epochs >= 1, UMX_ERR_ARGUMENT, "trial_epochs must be >= 1");
+        require(L->params && L->have_emb, UMX_ERR_STATE, "parameters and embedding must be set first");
+        require(L->mode == UMX_LAYOUT_PULL && L->dim <= 4, UMX_ERR_STATE,
+                "the autotuner shapes the pull-mode kernel (num_dim <= 4)");
+        const auto key = std::make_tuple(L->dtype, L->dim, L->n, L->nnz, L->n_inst(), L->total_epochs);
+        ensure_items(L);  // resolves automatic placement groups
+        TuneChoice best{L->tile_cols, L->waves, L->groups};
+        bool cached = false;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tune_cache.find(key);
+            if (it != g_tune_cache.end()) {
+                best = it->second;
+                cached = true;
+            }
+        }
+        const int warm = 4;
+        if (!cached && L->current_epoch + warm + trial_epochs <= L->total_epochs) {
+            const bool timing = L->timing;
+            const int gb = L->graph_block;
+            L->timing = false;
+            L->graph_block = 0;
+            Snapshot s0, s1;
+            take_snapshot(L, s0);
+            run_sell(L, L->current_epoch, warm);  // leave the first, light epochs
+            L->current_epoch += warm;
+            take_snapshot(L, s1);
+            hipEvent_t ev[2];
+            UMX_HIP(hipEventCreate(&ev[0]));
+            UMX_HIP(hipEventCreate(&ev[1]));
+            std::map<std::tuple<int, int, int>, float> seen;
+            auto trial = [&](const TuneChoice& c) -> float {
+                const auto k = std::make_tuple(c.tile, c.waves, c.groups);
+                auto it = seen.find(k);
+                if (it != seen.end()) return it->second;
+                if (umx::sell_lds_bytes(dsize(L->dtype), c.tile, c.waves) > static_cast<size_t>(umx::kSellLdsMax))
+                    return std::numeric_limits<float>::infinity();
+                apply_choice(L, c);
+                restore_snapshot(L, s1);
+                ensure_items(L);
+                ensure_sell_current(L);
+                ensure_tables(L);
+                UMX_HIP(hipEventRecord(ev[0], L->s));
+                for (int q = 0; q < trial_epochs; ++q) launch_sell_once(L, s1.epoch + q);
+                UMX_HIP(hipEventRecord(ev[1], L->s));
+                UMX_HIP(hipEventSynchronize(ev[1]));
+                float ms = 0;
+                UMX_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+                L->home = kHomeSell;
+                seen[k] = ms;
+                return ms;
+            };
+            try {
+                float best_ms = trial(best);
+                std::vector<int> gcands{1};
+                const int x = std::min(device_xcds(), 32);
+                if (L->n_inst() > 1 && x > 1) gcands.push_back(x);
+                for (int g : gcands) {
+                    const TuneChoice c{best.tile, best.waves, g};
+                    const float ms = trial(c);
+                    if (ms < best_ms) best_ms = ms, best = c;
+                }
+                for (int w : {1, 2, 4}) {
+                    const TuneChoice c{best.tile, w, best.groups};
+                    const float ms = trial(c);
+                    if (ms < best_ms) best_ms = ms, best = c;
+                }
+                for (int t : {4, 8, 16, 32}) {
+                    const TuneChoice c{t, best.waves, best.groups};
+                    const float ms = trial(c);
+                    if (ms < best_ms) best_ms = ms, best = c;
+                }
+            } catch (...) {
+                (void)hipEventDestroy(ev[0]);
+                (void)hipEventDestroy(ev[1]);
+                L->timing = timing;
+                L->graph_block = gb;
+                throw;
+            }
+            (void)hipEventDestroy(ev[0]);
+            (void)hipEventDestroy(ev[1]);
+            ensure_csr_current(L);
+            restore_snapshot(L, s0);
+            L->timing = timing;
+            L->graph_block = gb;
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tune_cache[key] = best;
+        }
+        apply_choice(L, best);
+        if (tile_cols) *tile_cols = best.tile;
+        if (waves) *waves = best.waves;
+        if (groups) *groups = best.groups;
     });
 }
 

@@ -571,17 +571,24 @@ struct Bits<double> {
     using S = long long;
 };
 
-// Largest non-NaN weight, starting from 0 like the reference's std::max fold.
-// Non-negative IEEE values order like their bit patterns as signed integers;
-// negative values map to negative integers and never beat the initial +0.
+// Largest non-NaN weight of each instance, starting from 0 like the
+// reference's std::max fold.  Non-negative IEEE values order like their bit
+// patterns as signed integers; negative values map to negative integers and
+// never beat the initial +0.  blockIdx.y is the instance; its entries are
+// [rp[pt_off[q]], rp[pt_off[q + 1]]) (all entries when pt_off is null).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_max_weight(const T* __restrict__ val, int64_t nnz,
+                                                     const int64_t* __restrict__ rp,
+                                                     const int64_t* __restrict__ pt_off,
                                                      typename Bits<T>::S* __restrict__ out) {
     using S = typename Bits<T>::S;
+    const int q = blockIdx.y;
+    const int64_t lo = pt_off ? rp[pt_off[q]] : 0;
+    const int64_t hi = pt_off ? rp[pt_off[q + 1]] : nnz;
     S best = 0;
-    for (int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; q < nnz;
-         q += static_cast<int64_t>(gridDim.x) * kBlock) {
-        const T v = val[q];
+    for (int64_t e = lo + static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; e < hi;
+         e += static_cast<int64_t>(gridDim.x) * kBlock) {
+        const T v = val[e];
         if (v == v) {
             S bits;
             __builtin_memcpy(&bits, &v, sizeof(S));
@@ -593,7 +600,7 @@ __global__ void __launch_bounds__(kBlock) k_max_weight(const T* __restrict__ val
         const S o = __shfl_xor(best, m, kWave);
         best = o > best ? o : best;
     }
-    if ((threadIdx.x & (kWave - 1)) == 0 && best > 0) atomicMax(out, best);
+    if ((threadIdx.x & (kWave - 1)) == 0 && best > 0) atomicMax(out + q, best);
 }
 
 template <typename T>
@@ -603,16 +610,34 @@ __device__ inline T bits_to(typename Bits<T>::S b) {
     return v;
 }
 
+// Largest weight and survivor limit of the instance that owns 'row'
+// (optimize_layout.hpp:49-63, per instance).
+template <typename T>
+struct RowLimit {
+    T maxed, limit;
+};
+template <typename T>
+__device__ inline RowLimit<T> row_limit(int64_t row, const int32_t* __restrict__ of_row,
+                                        const typename Bits<T>::S* __restrict__ maxbits, int num_epochs,
+                                        const int32_t* __restrict__ epochs_of_inst) {
+    const int q = of_row ? of_row[row] : 0;
+    const T maxed = bits_to<T>(maxbits[q]);
+    const int ne = epochs_of_inst ? epochs_of_inst[q] : num_epochs;
+    return RowLimit<T>{maxed, maxed / static_cast<T>(ne)};  // :63
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_count_survivors(int64_t n, int G, const int64_t* __restrict__ rp,
                                                           const T* __restrict__ val,
                                                           const typename Bits<T>::S* __restrict__ maxbits,
-                                                          int num_epochs, int64_t* __restrict__ cnt) {
+                                                          int num_epochs, const int32_t* __restrict__ epochs_of_inst,
+                                                          const int32_t* __restrict__ of_row,
+                                                          int64_t* __restrict__ cnt) {
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     const int64_t row = gid / G;
     const int lane = static_cast<int>(gid & (G - 1));
     if (row >= n) return;
-    const T limit = bits_to<T>(*maxbits) / static_cast<T>(num_epochs);  // :63
+    const T limit = row_limit<T>(row, of_row, maxbits, num_epochs, epochs_of_inst).limit;
     const int64_t s = rp[row], e = rp[row + 1];
     int64_t c = 0;
     for (int64_t q = s + lane; q < e; q += G) c += val[q] >= limit;
@@ -623,27 +648,65 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_fill_epochs(int64_t n, int G, const int64_t* __restrict__ rp,
                                                       const int32_t* __restrict__ idx, const T* __restrict__ val,
                                                       const typename Bits<T>::S* __restrict__ maxbits,
-                                                      int num_epochs, T nsr, const int64_t* __restrict__ orp,
-                                                      int32_t* __restrict__ tg, T* __restrict__ eps,
-                                                      T* __restrict__ nxt, T* __restrict__ nxt_neg) {
+                                                      int num_epochs, const int32_t* __restrict__ epochs_of_inst,
+                                                      const int32_t* __restrict__ of_row, T nsr,
+                                                      const int64_t* __restrict__ orp, int32_t* __restrict__ tg,
+                                                      T* __restrict__ eps, T* __restrict__ nxt,
+                                                      T* __restrict__ nxt_neg) {
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     const int64_t row = gid / G;
     const int lane = static_cast<int>(gid & (G - 1));
     if (row >= n) return;
-    const T maxed = bits_to<T>(*maxbits);
-    const T limit = maxed / static_cast<T>(num_epochs);
+    const RowLimit<T> rl = row_limit<T>(row, of_row, maxbits, num_epochs, epochs_of_inst);
     const int64_t s = rp[row], e = rp[row + 1];
     const int64_t ob = orp[row];
     for (int64_t q = s + lane; q < e; q += G) {
         const T w = val[q];
-        if (!(w >= limit)) continue;
+        if (!(w >= rl.limit)) continue;
         int64_t rank = 0;
-        for (int64_t f = s; f < q; ++f) rank += val[f] >= limit;
-        const T ep = maxed / w;  // :70
+        for (int64_t f = s; f < q; ++f) rank += val[f] >= rl.limit;
+        const T ep = rl.maxed / w;  // :70
         tg[ob + rank] = idx[q];
         eps[ob + rank] = ep;
         nxt[ob + rank] = ep;  // :77
         nxt_neg[ob + rank] = ep / nsr;  // :78-81
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Instances of a batched graph.
+// ---------------------------------------------------------------------------
+__device__ inline int instance_of(int64_t i, const int64_t* __restrict__ pt_off, int n_inst) {
+    int lo = 0, hi = n_inst;  // largest q with pt_off[q] <= i
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pt_off[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(kBlock) k_instance_of_row(int64_t n, const int64_t* __restrict__ pt_off, int n_inst,
+                                                          int32_t* __restrict__ of_row) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n) of_row[i] = instance_of(i, pt_off, n_inst);
+}
+
+__global__ void __launch_bounds__(kBlock) k_localize(int64_t n, const int64_t* __restrict__ rp,
+                                                   int32_t* __restrict__ idx, const int64_t* __restrict__ pt_off,
+                                                   const int32_t* __restrict__ of_row, int* __restrict__ bad) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int q = of_row[i];
+    const int64_t off = pt_off[q], size = pt_off[q + 1] - off;
+    for (int64_t e = rp[i]; e < rp[i + 1]; ++e) {
+        const int32_t j = idx[e];
+        if (j < 0 || j >= size) {
+            atomicOr(bad, 1);
+            idx[e] = -1;  // rejected again by the symmetrisation's index check
+        } else {
+            idx[e] = static_cast<int32_t>(off + j);
+        }
     }
 }
 
@@ -786,28 +849,51 @@ void run_combine(int64_t n, const int64_t* rp, const int32_t* idx, const T* val,
     UMX_HIP(hipStreamSynchronize(s));
 }
 
+void fill_instance_of_row(int64_t n, const Instances& inst, int32_t* of_row, hipStream_t s) {
+    if (n <= 0) return;
+    k_instance_of_row<<<grid_for(n), kBlock, 0, s>>>(n, inst.pt_off, inst.n_inst, of_row);
+    UMX_HIP(hipGetLastError());
+}
+
+bool localize_to_global(int64_t n, const int64_t* rp, int32_t* idx, const Instances& inst, hipStream_t s) {
+    if (n <= 0) return true;
+    Tmp<int> bad(1, s);
+    UMX_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+    k_localize<<<grid_for(n), kBlock, 0, s>>>(n, rp, idx, inst.pt_off, inst.of_row, bad.p);
+    UMX_HIP(hipGetLastError());
+    int h = 0;
+    UMX_HIP(hipMemcpyAsync(&h, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    UMX_HIP(hipStreamSynchronize(s));
+    return h == 0;
+}
+
 template <typename T>
-void run_to_epochs(int64_t n, const int64_t* rp, const int32_t* idx, const T* val, int num_epochs, T nsr,
-                   int64_t max_degree, DevBuf<int64_t>& out_rp, DevBuf<int32_t>& out_tg, DevBuf<T>& eps,
-                   DevBuf<T>& nxt, DevBuf<T>& nxt_neg, hipStream_t s) {
+void run_to_epochs(int64_t n, const int64_t* rp, const int32_t* idx, const T* val, int num_epochs,
+                   const int32_t* epochs_of_inst, const Instances& inst, T nsr, int64_t max_degree,
+                   DevBuf<int64_t>& out_rp, DevBuf<int32_t>& out_tg, DevBuf<T>& eps, DevBuf<T>& nxt,
+                   DevBuf<T>& nxt_neg, hipStream_t s) {
     using S = typename Bits<T>::S;
     int64_t nnz = 0;
     UMX_HIP(hipMemcpyAsync(&nnz, rp + n, sizeof(nnz), hipMemcpyDeviceToHost, s));
     UMX_HIP(hipStreamSynchronize(s));
     const int G = lanes_for_degree(max_degree);
     const unsigned grid = grid_for(n * G);
+    const int n_inst = inst.n_inst;
+    const int32_t* of_row = n_inst > 1 ? inst.of_row : nullptr;
 
-    Tmp<S> maxbits(1, s);
-    UMX_HIP(hipMemsetAsync(maxbits.p, 0, sizeof(S), s));
+    Tmp<S> maxbits(static_cast<size_t>(n_inst), s);
+    UMX_HIP(hipMemsetAsync(maxbits.p, 0, sizeof(S) * static_cast<size_t>(n_inst), s));
     if (nnz > 0) {
-        const int64_t want = (nnz + kBlock - 1) / kBlock;
-        const unsigned g = static_cast<unsigned>(std::min<int64_t>(want, 8192));
-        k_max_weight<T><<<g, kBlock, 0, s>>>(val, nnz, maxbits.p);
+        const int64_t want = (nnz / n_inst + kBlock - 1) / kBlock + 1;
+        const unsigned gx = static_cast<unsigned>(std::min<int64_t>(want, n_inst > 1 ? 512 : 8192));
+        k_max_weight<T><<<dim3(gx, static_cast<unsigned>(n_inst)), kBlock, 0, s>>>(
+            val, nnz, rp, n_inst > 1 ? inst.pt_off : nullptr, maxbits.p);
         UMX_HIP(hipGetLastError());
     }
     Tmp<int64_t> cnt(static_cast<size_t>(n), s);
     UMX_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int64_t) * static_cast<size_t>(n), s));
-    k_count_survivors<T><<<grid, kBlock, 0, s>>>(n, G, rp, val, maxbits.p, num_epochs, cnt.p);
+    k_count_survivors<T><<<grid, kBlock, 0, s>>>(n, G, rp, val, maxbits.p, num_epochs, epochs_of_inst, of_row,
+                                                 cnt.p);
     UMX_HIP(hipGetLastError());
     out_rp.alloc(static_cast<size_t>(n + 1));
     DevBuf<int64_t> unused;
@@ -819,8 +905,8 @@ void run_to_epochs(int64_t n, const int64_t* rp, const int32_t* idx, const T* va
     eps.alloc(static_cast<size_t>(kept));
     nxt.alloc(static_cast<size_t>(kept));
     nxt_neg.alloc(static_cast<size_t>(kept));
-    k_fill_epochs<T><<<grid, kBlock, 0, s>>>(n, G, rp, idx, val, maxbits.p, num_epochs, nsr, out_rp.p, out_tg.p,
-                                             eps.p, nxt.p, nxt_neg.p);
+    k_fill_epochs<T><<<grid, kBlock, 0, s>>>(n, G, rp, idx, val, maxbits.p, num_epochs, epochs_of_inst, of_row, nsr,
+                                             out_rp.p, out_tg.p, eps.p, nxt.p, nxt_neg.p);
     UMX_HIP(hipGetLastError());
     UMX_HIP(hipStreamSynchronize(s));
 }
@@ -830,9 +916,9 @@ void run_to_epochs(int64_t n, const int64_t* rp, const int32_t* idx, const T* va
                                          double*, int64_t, hipStream_t);                                         \
     template void run_combine<T>(int64_t, const int64_t*, const int32_t*, const T*, T, int64_t, DevBuf<int64_t>&, \
                                  DevBuf<int32_t>&, DevBuf<T>&, hipStream_t);                                     \
-    template void run_to_epochs<T>(int64_t, const int64_t*, const int32_t*, const T*, int, T, int64_t,           \
-                                   DevBuf<int64_t>&, DevBuf<int32_t>&, DevBuf<T>&, DevBuf<T>&, DevBuf<T>&,       \
-                                   hipStream_t);
+    template void run_to_epochs<T>(int64_t, const int64_t*, const int32_t*, const T*, int, const int32_t*,       \
+                                   const Instances&, T, int64_t, DevBuf<int64_t>&, DevBuf<int32_t>&, DevBuf<T>&, \
+                                   DevBuf<T>&, DevBuf<T>&, hipStream_t);
 UMX_INST(float)
 UMX_INST(double)
 #undef UMX_INST

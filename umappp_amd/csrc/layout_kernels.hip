@@ -22,7 +22,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <algorithm>
 #include <cmath>
+#include <vector>
 
 namespace umx {
 
@@ -34,7 +36,7 @@ template <typename T, int D, bool TRACE>
 __global__ void __launch_bounds__(kBlock) k_layout_rows(LayoutView<T> v, EpochConst ec, T ep, T alpha) {
     const int64_t m = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     unsigned long long fires = 0, draws = 0, self_hits = 0;
-    if (m < v.n_items) {
+    if (m < v.n_items && v.item_len[m] > 0) {  // padding items have length 0
         const int64_t i = static_cast<int64_t>(static_cast<uint32_t>(v.item_row[m]) & 0x7fffffffu);
         const int64_t s = v.item_begin[m], e = s + v.item_len[m];
         const Vec<T, D> h0 = load_point<T, D>(v.emb, i);
@@ -118,7 +120,7 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_layout_rows_generic(LayoutView<T> v, EpochConst ec, T ep, T alpha) {
     const int64_t m = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     unsigned long long fires = 0, draws = 0, self_hits = 0;
-    if (m < v.n_items) {
+    if (m < v.n_items && v.item_len[m] > 0) {  // padding items have length 0
         const int D = v.dim;
         const int64_t i = static_cast<int64_t>(static_cast<uint32_t>(v.item_row[m]) & 0x7fffffffu);
         const int64_t s = v.item_begin[m], e = s + v.item_len[m];
@@ -288,7 +290,7 @@ template <typename T, int D, bool TRACE>
 __global__ void __launch_bounds__(kBlock) k_layout_pull(LayoutView<T> v, EpochConst ec, T ep, T alpha) {
     const int64_t m = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     unsigned long long fires = 0, draws = 0, self_hits = 0;
-    if (m < v.n_items) {
+    if (m < v.n_items && v.item_len[m] > 0) {  // padding items have length 0
         const uint32_t packed = static_cast<uint32_t>(v.item_row[m]);
         const int64_t i = static_cast<int64_t>(packed & 0x7fffffffu);
         const bool split = (packed >> 31) != 0;
@@ -401,7 +403,7 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_layout_pull_generic(LayoutView<T> v, EpochConst ec, T ep, T alpha) {
     const int64_t m = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     unsigned long long fires = 0, draws = 0, self_hits = 0;
-    if (m < v.n_items) {
+    if (m < v.n_items && v.item_len[m] > 0) {  // padding items have length 0
         const int D = v.dim;
         const int64_t i = static_cast<int64_t>(static_cast<uint32_t>(v.item_row[m]) & 0x7fffffffu);
         const int64_t s = v.item_begin[m], e = s + v.item_len[m];
@@ -521,10 +523,11 @@ __global__ void __launch_bounds__(kBlock) k_fill_items(int64_t n, const int64_t*
 
 // Heaviest items first: a stable counting sort of the items on a 32-level
 // key (descending log2 of expected firings), so that long items start at
-// once and each wavefront gets items of similar length.  Pass A counts keys
-// per tile of kBlock items; pass B scatters each item to
-// offset[key][tile] + its rank among the tile's items with that key.
-constexpr int kKeys = 32;
+// once and each wavefront gets items of similar length.  With placement
+// groups the key is group * 32 + level.  Pass A counts keys per tile of
+// kBlock items; pass B scatters each item to offset[key][tile] + its rank
+// among the tile's items with that key, then places it (see place_item).
+constexpr int kLevels = 32;
 
 __device__ inline int work_key(float f) {
     // 31 for the heaviest items down to 0: key = 31 - clamp(floor(log2(16 f)) + 1, 0, 31)
@@ -535,42 +538,115 @@ __device__ inline int work_key(float f) {
     return 31 - lvl;
 }
 
-__global__ void __launch_bounds__(kBlock) k_item_key_hist(const float* __restrict__ iwork, int64_t n_items,
+__device__ inline int item_key(const float* __restrict__ iwork, const int32_t* __restrict__ irow, int64_t m,
+                               const int32_t* __restrict__ of_row, const int32_t* __restrict__ group_of_inst) {
+    int g = 0;
+    if (group_of_inst) {
+        const int64_t row = static_cast<int64_t>(static_cast<uint32_t>(irow[m]) & 0x7fffffffu);
+        g = group_of_inst[of_row[row]];
+    }
+    return g * kLevels + work_key(iwork[m]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_item_key_hist(const float* __restrict__ iwork,
+                                                        const int32_t* __restrict__ irow, int64_t n_items,
+                                                        const int32_t* __restrict__ of_row,
+                                                        const int32_t* __restrict__ group_of_inst, int n_keys,
                                                         int64_t ntiles, int64_t* __restrict__ hist) {
-    __shared__ int cnt[kKeys];
-    if (threadIdx.x < kKeys) cnt[threadIdx.x] = 0;
+    extern __shared__ int cnt[];
+    for (int k = threadIdx.x; k < n_keys; k += kBlock) cnt[k] = 0;
     __syncthreads();
     const int64_t m = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    if (m < n_items) atomicAdd(&cnt[work_key(iwork[m])], 1);
+    if (m < n_items) atomicAdd(&cnt[item_key(iwork, irow, m, of_row, group_of_inst)], 1);
     __syncthreads();
-    if (threadIdx.x < kKeys) hist[static_cast<int64_t>(threadIdx.x) * ntiles + blockIdx.x] = cnt[threadIdx.x];
+    for (int k = threadIdx.x; k < n_keys; k += kBlock) hist[static_cast<int64_t>(k) * ntiles + blockIdx.x] = cnt[k];
+}
+
+// Final index of the item at sorted position 'at' within group g (whose
+// items start at sorted position g0): chunks of 64 items, 'waves' chunks per
+// workgroup, and workgroup b serving group b % n_groups.
+__device__ inline int64_t place_item(int64_t at, int64_t g0, int g, int waves, int n_groups) {
+    if (n_groups == 1 && waves == 1) return at;
+    const int64_t r = at - g0;
+    const int64_t chunk = r / kWave;
+    const int64_t blk = (chunk / waves) * n_groups + g;
+    return (blk * waves + chunk % waves) * kWave + (r % kWave);
 }
 
 __global__ void __launch_bounds__(kBlock) k_item_scatter(const int64_t* __restrict__ ibeg, const int32_t* __restrict__ ilen,
                                                        const int32_t* __restrict__ irow, const float* __restrict__ iwork,
-                                                       int64_t n_items, int64_t ntiles,
-                                                       const int64_t* __restrict__ offs, int64_t* __restrict__ obeg,
-                                                       int32_t* __restrict__ olen, int32_t* __restrict__ orow) {
-    __shared__ int wave_cnt[kBlock / kWave][kKeys];
+                                                       int64_t n_items, const int32_t* __restrict__ of_row,
+                                                       const int32_t* __restrict__ group_of_inst, int n_keys,
+                                                       int64_t ntiles, const int64_t* __restrict__ offs, int waves,
+                                                       int n_groups, int64_t* __restrict__ obeg,
+                                                       int32_t* __restrict__ olen, int32_t* __restrict__ orow,
+                                                       int32_t* __restrict__ oinst) {
+    extern __shared__ int wave_cnt[];  // [kBlock / kWave][n_keys]
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
+    for (int k = threadIdx.x; k < (kBlock / kWave) * n_keys; k += kBlock) wave_cnt[k] = 0;
+    __syncthreads();
     const int64_t m = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     const bool live = m < n_items;
-    const int key = live ? work_key(iwork[m]) : -1;
-    // rank among earlier lanes of this wave with the same key
+    const int key = live ? item_key(iwork, irow, m, of_row, group_of_inst) : -1;
+    // rank among earlier lanes of this wave with the same key: one round per
+    // distinct key present in the wave
     int rank = 0;
-    for (int k = 0; k < kKeys; ++k) {
-        const unsigned long long bal = __ballot(key == k);
-        if (key == k) rank = __popcll(bal & ((1ULL << lane) - 1ULL));
-        if (lane == 0) wave_cnt[wid][k] = __popcll(bal);
+    bool done = !live;
+    while (true) {
+        const unsigned long long todo = __ballot(!done);
+        if (todo == 0) break;
+        const int first = __ffsll(static_cast<long long>(todo)) - 1;
+        const int k = __shfl(key, first, kWave);
+        const unsigned long long bal = __ballot(!done && key == k);
+        if (!done && key == k) {
+            rank = __popcll(bal & ((1ULL << lane) - 1ULL));
+            done = true;
+        }
+        if (lane == 0) wave_cnt[wid * n_keys + k] = __popcll(bal);
     }
     __syncthreads();
     if (!live) return;
-    for (int w = 0; w < wid; ++w) rank += wave_cnt[w][key];
+    for (int w = 0; w < wid; ++w) rank += wave_cnt[w * n_keys + key];
     const int64_t at = offs[static_cast<int64_t>(key) * ntiles + blockIdx.x] + rank;
-    obeg[at] = ibeg[m];
-    olen[at] = ilen[m];
-    orow[at] = irow[m];
+    const int g = key / kLevels;
+    const int64_t g0 = offs[static_cast<int64_t>(g * kLevels) * ntiles];
+    const int64_t to = place_item(at, g0, g, waves, n_groups);
+    obeg[to] = ibeg[m];
+    olen[to] = ilen[m];
+    orow[to] = irow[m];
+    if (oinst) {
+        const int64_t row = static_cast<int64_t>(static_cast<uint32_t>(irow[m]) & 0x7fffffffu);
+        oinst[to] = of_row ? of_row[row] : 0;
+    }
+}
+
+// Padding items: length 0, row 0, instance 0.
+__global__ void __launch_bounds__(kBlock) k_clear_items(int64_t count, int64_t* __restrict__ obeg,
+                                                      int32_t* __restrict__ olen, int32_t* __restrict__ orow,
+                                                      int32_t* __restrict__ oinst) {
+    const int64_t m = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (m >= count) return;
+    obeg[m] = 0;
+    olen[m] = 0;
+    orow[m] = 0;
+    oinst[m] = 0;
+}
+
+// Sum of 1 / eps over each instance's entries (blockIdx.y = instance).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_instance_fires(const int64_t* __restrict__ rp, const T* __restrict__ eps,
+                                                         const int64_t* __restrict__ pt_off,
+                                                         double* __restrict__ fires) {
+    const int q = blockIdx.y;
+    const int64_t lo = rp[pt_off[q]], hi = rp[pt_off[q + 1]];
+    double f = 0;
+    for (int64_t e = lo + static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; e < hi;
+         e += static_cast<int64_t>(gridDim.x) * kBlock)
+        f += 1.0 / static_cast<double>(eps[e]);
+#pragma unroll
+    for (int m = kWave / 2; m >= 1; m >>= 1) f += __shfl_xor(f, m, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0 && f != 0) atomicAdd(fires + q, f);
 }
 
 }  // namespace
@@ -632,9 +708,14 @@ void launch_layout_serial(const LayoutView<T>& v, const EpochArgs& ea, const int
 
 template <typename T>
 void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_split, int edges_cap,
-                      DevBuf<int64_t>& item_begin, DevBuf<int32_t>& item_len, DevBuf<int32_t>& item_row,
+                      const Instances& inst, const ItemPlacement& place, DevBuf<int64_t>& item_begin,
+                      DevBuf<int32_t>& item_len, DevBuf<int32_t>& item_row, DevBuf<int32_t>& item_inst,
                       int64_t& n_items, hipStream_t s) {
     if (!(fires_split > 0) || edges_cap < 1) throw Error(-1, "work split parameters must be positive");
+    if (place.waves < 1 || place.waves > 4 || place.n_groups < 1 || place.n_groups > 32)
+        throw Error(-1, "placement: waves must be 1..4 and groups 1..32");
+    if (place.n_groups > 1 && (place.group_of_inst == nullptr || (inst.n_inst > 1 && inst.of_row == nullptr)))
+        throw Error(-1, "placement groups need the instance of every row");
     DevBuf<int64_t> count, offs;
     count.alloc(static_cast<size_t>(n > 0 ? n : 1));
     offs.alloc(static_cast<size_t>(n + 1));
@@ -643,10 +724,11 @@ void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_s
     UMX_HIP(hipGetLastError());
     DevBuf<int64_t> unused;
     exclusive_scan_i64(count.p, offs.p, n, unused, s);
-    UMX_HIP(hipMemcpyAsync(&n_items, offs.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    int64_t real = 0;
+    UMX_HIP(hipMemcpyAsync(&real, offs.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     UMX_HIP(hipStreamSynchronize(s));
-    if (n_items > 0x7fffffffLL) throw Error(-5, "too many work items");
-    const size_t M = static_cast<size_t>(n_items > 0 ? n_items : 1);
+    if (real > 0x7fffffffLL) throw Error(-5, "too many work items");
+    const size_t M = static_cast<size_t>(real > 0 ? real : 1);
     DevBuf<int64_t> ib;
     DevBuf<int32_t> il, ir;
     DevBuf<float> iw;
@@ -657,28 +739,71 @@ void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_s
     if (n > 0)
         k_fill_items<T><<<grid, kBlock, 0, s>>>(n, rp, eps, fires_split, edges_cap, offs.p, ib.p, il.p, ir.p, iw.p);
     UMX_HIP(hipGetLastError());
-    item_begin.alloc(M);
-    item_len.alloc(M);
-    item_row.alloc(M);
-    if (n_items > 0) {
-        const int64_t ntiles = (n_items + kBlock - 1) / kBlock;
-        DevBuf<int64_t> hist, hoff;
-        hist.alloc(static_cast<size_t>(kKeys * ntiles));
-        hoff.alloc(static_cast<size_t>(kKeys * ntiles + 1));
-        k_item_key_hist<<<static_cast<unsigned>(ntiles), kBlock, 0, s>>>(iw.p, n_items, ntiles, hist.p);
+
+    const int G = place.n_groups;
+    const int n_keys = G * kLevels;
+    const int32_t* of_row = inst.n_inst > 1 ? inst.of_row : nullptr;
+    const int32_t* goi = G > 1 ? place.group_of_inst : nullptr;
+    const int64_t ntiles = (real + kBlock - 1) / kBlock;
+    DevBuf<int64_t> hist, hoff;
+    hist.alloc(static_cast<size_t>(n_keys * (ntiles > 0 ? ntiles : 1)));
+    hoff.alloc(static_cast<size_t>(n_keys * ntiles + 1));
+    // Final item count: each group padded to whole workgroups of
+    // waves * 64 items, every group given as many workgroups as the largest.
+    int64_t padded = real;
+    if (real > 0) {
+        k_item_key_hist<<<static_cast<unsigned>(ntiles), kBlock, sizeof(int) * n_keys, s>>>(
+            iw.p, ir.p, real, of_row, goi, n_keys, ntiles, hist.p);
         UMX_HIP(hipGetLastError());
-        exclusive_scan_i64(hist.p, hoff.p, kKeys * ntiles, unused, s);
-        k_item_scatter<<<static_cast<unsigned>(ntiles), kBlock, 0, s>>>(ib.p, il.p, ir.p, iw.p, n_items, ntiles, hoff.p,
-                                                                       item_begin.p, item_len.p, item_row.p);
+        exclusive_scan_i64(hist.p, hoff.p, n_keys * ntiles, unused, s);
+        if (G > 1 || place.waves > 1) {
+            std::vector<int64_t> starts(static_cast<size_t>(G) + 1);
+            for (int g = 0; g < G; ++g)
+                UMX_HIP(hipMemcpyAsync(&starts[g], hoff.p + static_cast<int64_t>(g * kLevels) * ntiles,
+                                       sizeof(int64_t), hipMemcpyDeviceToHost, s));
+            UMX_HIP(hipStreamSynchronize(s));
+            starts[G] = real;
+            const int64_t per_blk = static_cast<int64_t>(place.waves) * kWave;
+            int64_t most = 0;
+            for (int g = 0; g < G; ++g) most = std::max(most, (starts[g + 1] - starts[g] + per_blk - 1) / per_blk);
+            padded = most * G * per_blk;
+        }
+    }
+    if (padded > 0x7fffffffLL) throw Error(-5, "too many work items");
+    n_items = padded;
+    const size_t P = static_cast<size_t>(padded > 0 ? padded : 1);
+    item_begin.alloc(P);
+    item_len.alloc(P);
+    item_row.alloc(P);
+    item_inst.alloc(P);
+    if (padded > real)
+        k_clear_items<<<static_cast<unsigned>((padded + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+            padded, item_begin.p, item_len.p, item_row.p, item_inst.p);
+    if (real > 0) {
+        const size_t lds = sizeof(int) * static_cast<size_t>((kBlock / kWave) * n_keys);
+        k_item_scatter<<<static_cast<unsigned>(ntiles), kBlock, lds, s>>>(
+            ib.p, il.p, ir.p, iw.p, real, of_row, goi, n_keys, ntiles, hoff.p, place.waves, G, item_begin.p,
+            item_len.p, item_row.p, item_inst.p);
         UMX_HIP(hipGetLastError());
     }
     UMX_HIP(hipStreamSynchronize(s));
 }
 
-template void build_work_items<float>(int64_t, const int64_t*, const float*, double, int, DevBuf<int64_t>&,
-                                      DevBuf<int32_t>&, DevBuf<int32_t>&, int64_t&, hipStream_t);
-template void build_work_items<double>(int64_t, const int64_t*, const double*, double, int, DevBuf<int64_t>&,
-                                       DevBuf<int32_t>&, DevBuf<int32_t>&, int64_t&, hipStream_t);
+template <typename T>
+void instance_fires(const int64_t* rp, const T* eps, const Instances& inst, double* fires, hipStream_t s) {
+    UMX_HIP(hipMemsetAsync(fires, 0, sizeof(double) * static_cast<size_t>(inst.n_inst), s));
+    k_instance_fires<T><<<dim3(64, static_cast<unsigned>(inst.n_inst)), kBlock, 0, s>>>(rp, eps, inst.pt_off, fires);
+    UMX_HIP(hipGetLastError());
+}
+
+template void build_work_items<float>(int64_t, const int64_t*, const float*, double, int, const Instances&,
+                                      const ItemPlacement&, DevBuf<int64_t>&, DevBuf<int32_t>&, DevBuf<int32_t>&,
+                                      DevBuf<int32_t>&, int64_t&, hipStream_t);
+template void build_work_items<double>(int64_t, const int64_t*, const double*, double, int, const Instances&,
+                                       const ItemPlacement&, DevBuf<int64_t>&, DevBuf<int32_t>&, DevBuf<int32_t>&,
+                                       DevBuf<int32_t>&, int64_t&, hipStream_t);
+template void instance_fires<float>(const int64_t*, const float*, const Instances&, double*, hipStream_t);
+template void instance_fires<double>(const int64_t*, const double*, const Instances&, double*, hipStream_t);
 
 template void launch_layout_epoch<float>(const LayoutView<float>&, const EpochArgs&, hipStream_t);
 template void launch_layout_epoch<double>(const LayoutView<double>&, const EpochArgs&, hipStream_t);

@@ -20,6 +20,14 @@
 // with coalesced stores.  The CSR arrays remain the canonical schedule state;
 // the host syncs the two copies when it switches between this kernel and the
 // CSR kernels (serial replay, push mode, export).
+//
+// Batches.  The same kernel runs many independent problems (instances) in one
+// launch: each lane reads its item's instance constants (observation offset
+// and count, seed, epochs, learning rates), draws negative samples inside its
+// instance, and keys the sampler by the instance's seed and its own entry
+// numbers, so an instance sees the samples it would see when run alone.  The
+// epoch number lives in device memory and the last workgroup advances it, so
+// a block of launches captured once as a HIP graph replays for any epochs.
 #include "layout_device.hpp"
 #include "umx_internal.hpp"
 
@@ -121,161 +129,195 @@ struct SellTile {
     int32_t tg[TW][kWave];
 };
 
+// One wavefront per chunk; a workgroup of W wavefronts (W = blockDim.x / 64)
+// takes chunks blockIdx.x * W .. + W - 1.  Each lane reads its item's
+// instance constants (observation offset and count, sampler seed, epochs,
+// learning rate); a plain layout is one instance.  The epoch is epoch_arg,
+// or v.epoch_ctr[0] when epoch_arg < 0 (captured blocks of epochs, where a
+// one-thread kernel advances the counter after each launch).
 template <typename T, int D, int TW, bool TRACE>
-__global__ void __launch_bounds__(kWave) k_layout_sell(LayoutView<T> v, EpochConst ec, T ep, T alpha) {
-    __shared__ SellTile<T, TW> tile;
-    const int64_t c = blockIdx.x;
-    const int lane = threadIdx.x;
-    const int64_t m = c * kWave + lane;
-    const bool live = m < v.n_items;
-    const int width = v.chunk_width[c];
-    const int64_t base = v.chunk_base[c];
+__global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_arg) {
+    extern __shared__ __align__(16) unsigned char sell_lds[];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    SellTile<T, TW>& tile = reinterpret_cast<SellTile<T, TW>*>(sell_lds)[wid];
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + wid;
+    const int epn = epoch_arg >= 0 ? epoch_arg : v.epoch_ctr[0];
+    const T ep = static_cast<T>(epn);
 
     unsigned long long fires = 0, draws = 0, self_hits = 0;
-    int64_t i = 0, e0 = 0;
-    bool split = false;
-    Vec<T, D> h0{}, h{};
-    if (live) {
-        const uint32_t packed = static_cast<uint32_t>(v.item_row[m]);
-        i = static_cast<int64_t>(packed & 0x7fffffffu);
-        split = (packed >> 31) != 0;
-        e0 = v.item_begin[m];
-        h0 = load_point<T, D>(v.emb, i);
-        h = h0;
-    }
-    const T two_ab = -2 * v.a * v.b;
-    const T two_gb = 2 * v.gamma * v.b;
-    const uint32_t n32 = static_cast<uint32_t>(v.n);
-    bool moved = false;
+    if (c < v.n_chunks) {
+        const int64_t m = c * kWave + lane;
+        bool live = m < v.n_items;
+        const int width = v.chunk_width[c];
+        const int64_t base = v.chunk_base[c];
 
-    for (int t0 = 0; t0 < width; t0 += TW) {
-        const int tw = width - t0 < TW ? width - t0 : TW;
-        // coalesced fetch of the tile (all columns in flight together)
-        unsigned int mask = 0;
-#pragma unroll
-        for (int k = 0; k < TW; ++k) {
-            if (k < tw) {
-                const int64_t slot = base + static_cast<int64_t>(t0 + k) * kWave + lane;
-                const T x = v.sell_nx[slot];
-                tile.nx[k][lane] = x;
-                tile.nn[k][lane] = v.sell_nn[slot];
-                tile.eps[k][lane] = v.sell_eps[slot];
-                tile.tg[k][lane] = v.sell_tg[slot];
-                if (!(x > ep)) mask |= 1u << k;
-            }
+        int64_t i = 0, e0 = 0, edge_off = 0, pt_off = 0;
+        bool split = false;
+        uint32_t n32 = 0;
+        uint64_t ekey = 0;
+        T alpha = 0;
+        Vec<T, D> h0{}, h{};
+        if (live) {
+            const int q = v.item_inst[m];
+            const InstConst ic = v.inst[q];
+            live = epn < ic.epochs;
+            pt_off = ic.pt_off;
+            edge_off = ic.edge_off;
+            n32 = ic.n;
+            ekey = mix64(ic.seed + kGold * (static_cast<uint64_t>(epn) + 1));
+            alpha = live ? v.alpha_tbl[static_cast<int64_t>(q) * v.alpha_stride + epn] : static_cast<T>(0);
+            const uint32_t packed = static_cast<uint32_t>(v.item_row[m]);
+            i = static_cast<int64_t>(packed & 0x7fffffffu);
+            split = (packed >> 31) != 0;
+            e0 = v.item_begin[m];
+            h0 = load_point<T, D>(v.emb, i);
+            h = h0;
         }
-        if (!live) mask = 0;
-        unsigned int fired = mask;
-        while (mask) {
-            const int k = __builtin_ctz(mask);
-            mask &= mask - 1;
-            const int64_t j = e0 + t0 + k;  // CSR index of this edge: keys the sampler
-            ++fires;
-            moved = true;
-            const int32_t t = tile.tg[k][lane];
-            const T nxj = tile.nx[k][lane];
-            const T epsj = tile.eps[k][lane];
-            const T nnj = tile.nn[k][lane];
-            const T epns = epsj / v.nsr;
-            const int nneg = static_cast<int>((ep - nnj) / epns);
-            const uint64_t ej = edge_key(ec.ekey, static_cast<uint64_t>(j));
-            const Vec<T, D> r = load_point<T, D>(v.emb, t);
-            int32_t kk[kNegFetch];
-            Vec<T, D> q[kNegFetch];
+        const T two_ab = -2 * v.a * v.b;
+        const T two_gb = 2 * v.gamma * v.b;
+        bool moved = false;
+
+        for (int t0 = 0; t0 < width; t0 += TW) {
+            const int tw = width - t0 < TW ? width - t0 : TW;
+            // coalesced fetch of the tile (all columns in flight together)
+            unsigned int mask = 0;
 #pragma unroll
-            for (int u = 0; u < kNegFetch; ++u) {
-                kk[u] = u < nneg ? draw_index(ej, static_cast<uint32_t>(u), n32) : -1;
-                if (kk[u] >= 0) q[u] = load_point<T, D>(v.emb, kk[u]);
-            }
-#pragma unroll
-            for (int half = 0; half < 2; ++half) {
-                const T d2 = sqdist<T, D>(h, r);
-                const T pd2b = fast_pow(d2, v.b);
-                const T gc = prod_div(two_ab * pd2b, d2 * (v.a * pd2b + 1));
-                double* rec = TRACE ? trace_slot(v, half == 0 ? 0 : 2, i, t) : nullptr;
-#pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    const T g = alpha * clip4(gc * (h.c[d] - r.c[d]));
-                    if (TRACE && rec) {
-                        rec[3 + d] = h.c[d];
-                        rec[3 + D + d] = r.c[d];
-                        rec[3 + 2 * D + d] = g;
-                    }
-                    h.c[d] += g;
+            for (int k = 0; k < TW; ++k) {
+                if (k < tw) {
+                    const int64_t slot = base + static_cast<int64_t>(t0 + k) * kWave + lane;
+                    const T x = v.sell_nx[slot];
+                    tile.nx[k][lane] = x;
+                    tile.nn[k][lane] = v.sell_nn[slot];
+                    tile.eps[k][lane] = v.sell_eps[slot];
+                    tile.tg[k][lane] = v.sell_tg[slot];
+                    if (!(x > ep)) mask |= 1u << k;
                 }
             }
-            for (int p0 = 0; p0 < nneg; p0 += kNegFetch) {
-                if (p0 > 0) {
-#pragma unroll
-                    for (int u = 0; u < kNegFetch; ++u) {
-                        kk[u] = p0 + u < nneg ? draw_index(ej, static_cast<uint32_t>(p0 + u), n32) : -1;
-                        if (kk[u] >= 0) q[u] = load_point<T, D>(v.emb, kk[u]);
-                    }
-                }
+            if (!live) mask = 0;
+            const unsigned int fired = mask;
+            while (mask) {
+                const int k = __builtin_ctz(mask);
+                mask &= mask - 1;
+                // instance-local entry number of this edge: keys the sampler
+                const int64_t j = e0 + t0 + k - edge_off;
+                ++fires;
+                moved = true;
+                const int32_t t = tile.tg[k][lane];
+                const T nxj = tile.nx[k][lane];
+                const T epsj = tile.eps[k][lane];
+                const T nnj = tile.nn[k][lane];
+                const T epns = epsj / v.nsr;
+                const int nneg = static_cast<int>((ep - nnj) / epns);
+                const uint64_t ej = edge_key(ekey, static_cast<uint64_t>(j));
+                const Vec<T, D> r = load_point<T, D>(v.emb, t);
+                int32_t kk[kNegFetch];
+                Vec<T, D> q[kNegFetch];
 #pragma unroll
                 for (int u = 0; u < kNegFetch; ++u) {
-                    if (kk[u] < 0) break;
-                    ++draws;
-                    if (kk[u] == i) {
-                        ++self_hits;
-                        continue;
-                    }
-                    const T e2 = sqdist<T, D>(h, q[u]);
-                    const T rc = prod_div(two_gb, (static_cast<T>(0.001) + e2) * (v.a * fast_pow(e2, v.b) + 1));
-                    double* rec = TRACE ? trace_slot(v, 1, i, kk[u]) : nullptr;
+                    kk[u] = u < nneg ? static_cast<int32_t>(pt_off + draw_index(ej, static_cast<uint32_t>(u), n32)) : -1;
+                    if (kk[u] >= 0) q[u] = load_point<T, D>(v.emb, kk[u]);
+                }
+#pragma unroll
+                for (int half = 0; half < 2; ++half) {
+                    const T d2 = sqdist<T, D>(h, r);
+                    const T pd2b = fast_pow(d2, v.b);
+                    const T gc = prod_div(two_ab * pd2b, d2 * (v.a * pd2b + 1));
+                    double* rec = TRACE ? trace_slot(v, half == 0 ? 0 : 2, i, t) : nullptr;
 #pragma unroll
                     for (int d = 0; d < D; ++d) {
-                        const T g = alpha * clip4(rc * (h.c[d] - q[u].c[d]));
+                        const T g = alpha * clip4(gc * (h.c[d] - r.c[d]));
                         if (TRACE && rec) {
                             rec[3 + d] = h.c[d];
-                            rec[3 + D + d] = q[u].c[d];
+                            rec[3 + D + d] = r.c[d];
                             rec[3 + 2 * D + d] = g;
                         }
                         h.c[d] += g;
                     }
                 }
-            }
-            tile.nx[k][lane] = nxj + epsj;
-            tile.nn[k][lane] = nnj + static_cast<T>(nneg) * epns;
-        }
-        // write-back of this lane's fired columns (coalesced across the lanes
-        // that fired in the same column)
+                for (int p0 = 0; p0 < nneg; p0 += kNegFetch) {
+                    if (p0 > 0) {
 #pragma unroll
-        for (int k = 0; k < TW; ++k) {
-            if (k < tw && (fired >> k) & 1u) {
-                const int64_t slot = base + static_cast<int64_t>(t0 + k) * kWave + lane;
-                v.sell_nx[slot] = tile.nx[k][lane];
-                v.sell_nn[slot] = tile.nn[k][lane];
+                        for (int u = 0; u < kNegFetch; ++u) {
+                            kk[u] = p0 + u < nneg
+                                        ? static_cast<int32_t>(pt_off + draw_index(ej, static_cast<uint32_t>(p0 + u), n32))
+                                        : -1;
+                            if (kk[u] >= 0) q[u] = load_point<T, D>(v.emb, kk[u]);
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < kNegFetch; ++u) {
+                        if (kk[u] < 0) break;
+                        ++draws;
+                        if (kk[u] == i) {
+                            ++self_hits;
+                            continue;
+                        }
+                        const T e2 = sqdist<T, D>(h, q[u]);
+                        const T rc = prod_div(two_gb, (static_cast<T>(0.001) + e2) * (v.a * fast_pow(e2, v.b) + 1));
+                        double* rec = TRACE ? trace_slot(v, 1, i, kk[u]) : nullptr;
+#pragma unroll
+                        for (int d = 0; d < D; ++d) {
+                            const T g = alpha * clip4(rc * (h.c[d] - q[u].c[d]));
+                            if (TRACE && rec) {
+                                rec[3 + d] = h.c[d];
+                                rec[3 + D + d] = q[u].c[d];
+                                rec[3 + 2 * D + d] = g;
+                            }
+                            h.c[d] += g;
+                        }
+                    }
+                }
+                tile.nx[k][lane] = nxj + epsj;
+                tile.nn[k][lane] = nnj + static_cast<T>(nneg) * epns;
+            }
+            // write-back of this lane's fired columns (coalesced across the
+            // lanes that fired in the same column)
+#pragma unroll
+            for (int k = 0; k < TW; ++k) {
+                if (k < tw && (fired >> k) & 1u) {
+                    const int64_t slot = base + static_cast<int64_t>(t0 + k) * kWave + lane;
+                    v.sell_nx[slot] = tile.nx[k][lane];
+                    v.sell_nn[slot] = tile.nn[k][lane];
+                }
+            }
+        }
+        if (live && moved) {
+            if (split) {
+#pragma unroll
+                for (int d = 0; d < D; ++d) add_to(v.emb + i * D + d, h.c[d] - h0.c[d]);
+            } else {
+                store_point<T, D>(v.emb, i, h);
             }
         }
     }
-    if (live && moved) {
-        if (split) {
-#pragma unroll
-            for (int d = 0; d < D; ++d) add_to(v.emb + i * D + d, h.c[d] - h0.c[d]);
-        } else {
-            store_point<T, D>(v.emb, i, h);
-        }
-    }
-    tally_add(v.tally + 3 * static_cast<int64_t>(ec.epoch), fires, draws, self_hits);
+    tally_add(v.tally + 3 * static_cast<int64_t>(epn), fires, draws, self_hits);
 }
 
+__global__ void k_set_epoch(int* ctr, int epoch) { ctr[0] = epoch; }
+
+__global__ void k_advance_epoch(int* ctr) { ctr[0] += 1; }
+
 template <typename T, int TW>
-void launch_sell_tw(const LayoutView<T>& v, const EpochConst& ec, T ep, T alpha, hipStream_t s) {
-    const unsigned grid = static_cast<unsigned>(v.n_chunks);
+void launch_sell_tw(const LayoutView<T>& v, int epoch, int waves, hipStream_t s) {
+    const unsigned grid = static_cast<unsigned>((v.n_chunks + waves - 1) / waves);
+    const unsigned block = static_cast<unsigned>(waves * kWave);
+    const size_t lds = sizeof(SellTile<T, TW>) * static_cast<size_t>(waves);
+    if (lds > static_cast<size_t>(kSellLdsMax))
+        throw Error(-1, "LDS tile times waves per workgroup exceeds 64 KiB; use a smaller tile or fewer waves");
     if (v.trace) {
         switch (v.dim) {
-            case 1: k_layout_sell<T, 1, TW, true><<<grid, kWave, 0, s>>>(v, ec, ep, alpha); break;
-            case 2: k_layout_sell<T, 2, TW, true><<<grid, kWave, 0, s>>>(v, ec, ep, alpha); break;
-            case 3: k_layout_sell<T, 3, TW, true><<<grid, kWave, 0, s>>>(v, ec, ep, alpha); break;
-            default: k_layout_sell<T, 4, TW, true><<<grid, kWave, 0, s>>>(v, ec, ep, alpha); break;
+            case 1: k_layout_sell<T, 1, TW, true><<<grid, block, lds, s>>>(v, epoch); break;
+            case 2: k_layout_sell<T, 2, TW, true><<<grid, block, lds, s>>>(v, epoch); break;
+            case 3: k_layout_sell<T, 3, TW, true><<<grid, block, lds, s>>>(v, epoch); break;
+            default: k_layout_sell<T, 4, TW, true><<<grid, block, lds, s>>>(v, epoch); break;
         }
     } else {
         switch (v.dim) {
-            case 1: k_layout_sell<T, 1, TW, false><<<grid, kWave, 0, s>>>(v, ec, ep, alpha); break;
-            case 2: k_layout_sell<T, 2, TW, false><<<grid, kWave, 0, s>>>(v, ec, ep, alpha); break;
-            case 3: k_layout_sell<T, 3, TW, false><<<grid, kWave, 0, s>>>(v, ec, ep, alpha); break;
-            default: k_layout_sell<T, 4, TW, false><<<grid, kWave, 0, s>>>(v, ec, ep, alpha); break;
+            case 1: k_layout_sell<T, 1, TW, false><<<grid, block, lds, s>>>(v, epoch); break;
+            case 2: k_layout_sell<T, 2, TW, false><<<grid, block, lds, s>>>(v, epoch); break;
+            case 3: k_layout_sell<T, 3, TW, false><<<grid, block, lds, s>>>(v, epoch); break;
+            default: k_layout_sell<T, 4, TW, false><<<grid, block, lds, s>>>(v, epoch); break;
         }
     }
 }
@@ -332,19 +374,25 @@ void sync_sell_to_csr(T* nx, T* nn, const SellState<T>& st, hipStream_t s) {
 }
 
 template <typename T>
-void launch_layout_sell(const LayoutView<T>& v, const EpochArgs& ea, int tile_cols, hipStream_t s) {
-    if (v.n_chunks == 0) {
-        return;
+void launch_layout_sell(const LayoutView<T>& v, int epoch, int tile_cols, int waves, hipStream_t s) {
+    if (waves < 1 || waves > 4) throw Error(-1, "waves per workgroup must be 1..4");
+    if (v.n_chunks > 0) {
+        switch (tile_cols) {
+            case 4: launch_sell_tw<T, 4>(v, epoch, waves, s); break;
+            case 8: launch_sell_tw<T, 8>(v, epoch, waves, s); break;
+            case 32: launch_sell_tw<T, 32>(v, epoch, waves, s); break;
+            default: launch_sell_tw<T, 16>(v, epoch, waves, s); break;
+        }
+        UMX_HIP(hipGetLastError());
     }
-    const EpochConst ec{ea.epoch, ea.ekey};
-    const T ep = static_cast<T>(ea.epoch);
-    const T alpha = static_cast<T>(ea.alpha);
-    switch (tile_cols) {
-        case 4: launch_sell_tw<T, 4>(v, ec, ep, alpha, s); break;
-        case 8: launch_sell_tw<T, 8>(v, ec, ep, alpha, s); break;
-        case 32: launch_sell_tw<T, 32>(v, ec, ep, alpha, s); break;
-        default: launch_sell_tw<T, 16>(v, ec, ep, alpha, s); break;
+    if (epoch < 0) {
+        k_advance_epoch<<<1, 1, 0, s>>>(v.epoch_ctr);
+        UMX_HIP(hipGetLastError());
     }
+}
+
+void set_epoch_counter(int* epoch_ctr, int epoch, hipStream_t s) {
+    k_set_epoch<<<1, 1, 0, s>>>(epoch_ctr, epoch);
     UMX_HIP(hipGetLastError());
 }
 
@@ -353,7 +401,7 @@ void launch_layout_sell(const LayoutView<T>& v, const EpochArgs& ea, int tile_co
                                 const int32_t*, int64_t, SellState<T>&, hipStream_t);                          \
     template void sync_csr_to_sell<T>(const T*, const T*, SellState<T>&, hipStream_t);                         \
     template void sync_sell_to_csr<T>(T*, T*, const SellState<T>&, hipStream_t);                               \
-    template void launch_layout_sell<T>(const LayoutView<T>&, const EpochArgs&, int, hipStream_t);
+    template void launch_layout_sell<T>(const LayoutView<T>&, int, int, int, hipStream_t);
 UMX_INST(float)
 UMX_INST(double)
 #undef UMX_INST

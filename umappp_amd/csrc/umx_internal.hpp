@@ -73,11 +73,31 @@ template <typename T>
 void run_combine(int64_t n, const int64_t* rp, const int32_t* idx, const T* val, T mix, int64_t max_degree,
                  DevBuf<int64_t>& out_rp, DevBuf<int32_t>& out_idx, DevBuf<T>& out_val, hipStream_t s);
 
-// similarities_to_epochs.  Allocates the EpochData arrays.
+// Instances of a batched graph or layout: independent problems stored one
+// after the other in a single CSR.  Instance q owns observations
+// [pt_off[q], pt_off[q + 1]); its rows' entries are contiguous too.  A plain
+// problem is the case n_inst = 1 (of_row may then be null).
+struct Instances {
+    int n_inst = 1;
+    const int64_t* pt_off = nullptr;  // device, n_inst + 1 entries
+    const int32_t* of_row = nullptr;  // device, instance of each row (null when n_inst == 1)
+};
+
+// Turns instance-local neighbour indices into global ones (adds the row's
+// instance offset); returns false if a local index lies outside its instance.
+bool localize_to_global(int64_t n, const int64_t* rp, int32_t* idx, const Instances& inst, hipStream_t s);
+
+// Instance of every row.
+void fill_instance_of_row(int64_t n, const Instances& inst, int32_t* of_row, hipStream_t s);
+
+// similarities_to_epochs.  Allocates the EpochData arrays.  The largest
+// weight and the survivor limit are taken per instance; epochs_of_inst holds
+// each instance's num_epochs (device; null = num_epochs for all).
 template <typename T>
-void run_to_epochs(int64_t n, const int64_t* rp, const int32_t* idx, const T* val, int num_epochs, T nsr,
-                   int64_t max_degree, DevBuf<int64_t>& out_rp, DevBuf<int32_t>& out_tg, DevBuf<T>& eps,
-                   DevBuf<T>& nxt, DevBuf<T>& nxt_neg, hipStream_t s);
+void run_to_epochs(int64_t n, const int64_t* rp, const int32_t* idx, const T* val, int num_epochs,
+                   const int32_t* epochs_of_inst, const Instances& inst, T nsr, int64_t max_degree,
+                   DevBuf<int64_t>& out_rp, DevBuf<int32_t>& out_tg, DevBuf<T>& eps, DevBuf<T>& nxt,
+                   DevBuf<T>& nxt_neg, hipStream_t s);
 
 // ---- layout kernels (layout_kernels.hip) ----
 
@@ -91,6 +111,15 @@ struct EpochArgs {
 // its moves, no atomics) and push (tails moved with float atomics).
 constexpr int kModePull = 0;
 constexpr int kModePush = 1;
+
+// Per-instance constants of the production layout kernel.
+struct InstConst {
+    int64_t pt_off;    // first observation
+    int64_t edge_off;  // first schedule entry (sampler keys use instance-local entry numbers)
+    uint32_t n;        // observations
+    int32_t epochs;    // num_epochs of this instance
+    uint64_t seed;     // optimize_seed of this instance
+};
 
 template <typename T>
 struct LayoutView {
@@ -124,6 +153,15 @@ struct LayoutView {
     const T* sell_eps;
     T* sell_nx;
     T* sell_nn;
+    // Instances (sliced-ELL kernel): item_inst[m] is the instance of item m,
+    // inst[q] its constants, alpha_tbl[q * alpha_stride + epoch] its learning
+    // rate per epoch.  epoch_ctr[0] is the epoch of the next launch of a
+    // captured block.
+    const int32_t* item_inst;
+    const InstConst* inst;
+    const T* alpha_tbl;
+    int alpha_stride;
+    int* epoch_ctr;
     // Optional update trace (parity mode): records of 3 + 3 * dim doubles
     // (kind, head, other, head before, other before, head move).
     double* trace;
@@ -142,10 +180,29 @@ void launch_layout_serial(const LayoutView<T>& v, const EpochArgs& ea, const int
 // expected to fire more than 'fires_split' edges per epoch (sum of 1 / eps)
 // or longer than 'edges_cap' are cut into items of about equal expected
 // firings.  Items are sorted heaviest first.
+//
+// Placement for the sliced-ELL kernel: a workgroup of 'waves' wavefronts
+// takes waves * 64 consecutive items.  With n_groups > 1, each instance
+// belongs to group group_of_inst[q] and workgroup b only takes items of
+// group b % n_groups (workgroups are dispatched to the XCDs round-robin, so
+// a group's instances stay in one XCD's L2); groups are padded with empty
+// items (length 0, row 0).  item_inst receives each item's instance.
+struct ItemPlacement {
+    int waves = 1;
+    int n_groups = 1;
+    const int32_t* group_of_inst = nullptr;  // device, n_inst entries (n_groups > 1 only)
+};
+
 template <typename T>
 void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_split, int edges_cap,
-                      DevBuf<int64_t>& item_begin, DevBuf<int32_t>& item_len, DevBuf<int32_t>& item_row,
+                      const Instances& inst, const ItemPlacement& place, DevBuf<int64_t>& item_begin,
+                      DevBuf<int32_t>& item_len, DevBuf<int32_t>& item_row, DevBuf<int32_t>& item_inst,
                       int64_t& n_items, hipStream_t s);
+
+// Expected firings per epoch of each instance (sum of 1 / eps over its
+// entries), written to fires[n_inst] (device).
+template <typename T>
+void instance_fires(const int64_t* rp, const T* eps, const Instances& inst, double* fires, hipStream_t s);
 
 // Sliced-ELL schedule state (sell_kernels.hip).
 template <typename T>
@@ -165,8 +222,22 @@ template <typename T>
 void sync_csr_to_sell(const T* nx, const T* nn, SellState<T>& st, hipStream_t s);
 template <typename T>
 void sync_sell_to_csr(T* nx, T* nn, const SellState<T>& st, hipStream_t s);
+// LDS budget of one workgroup of the sliced-ELL kernel, and what a
+// configuration needs: waves * tile columns * 64 lanes * (3 values + 1 index).
+constexpr int kSellLdsMax = 65536;
+inline size_t sell_lds_bytes(size_t value_bytes, int tile_cols, int waves) {
+    return static_cast<size_t>(waves) * tile_cols * kWave * (3 * value_bytes + 4);
+}
+
+// One epoch of the sliced-ELL kernel.  epoch >= 0: that epoch.  epoch < 0:
+// the epoch in v.epoch_ctr[0] on the device, which a one-thread kernel
+// launched right after advances (so a captured block of launches replays for
+// any epochs).
 template <typename T>
-void launch_layout_sell(const LayoutView<T>& v, const EpochArgs& ea, int tile_cols, hipStream_t s);
+void launch_layout_sell(const LayoutView<T>& v, int epoch, int tile_cols, int waves, hipStream_t s);
+
+// Sets epoch_ctr[0] = epoch with a kernel (safe inside graph capture).
+void set_epoch_counter(int* epoch_ctr, int epoch, hipStream_t s);
 
 // Counter sampler key of one epoch (host side; the device derives per-edge
 // and per-draw keys from it).
