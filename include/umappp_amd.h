@@ -245,6 +245,39 @@ int umx_layout_timing(const umx_layout* L, double* kernel_ms, int64_t* launches)
 void umx_layout_destroy(umx_layout* L);
 
 /* ------------------------------------------------------------------ */
+/* UMAP cross-entropy loss (forward / backward of the autograd loss)    */
+/* ------------------------------------------------------------------ */
+
+/* No counterpart in the reference, which only applies per-edge SGD steps
+ * (optimize_layout.hpp:145-176); this is the loss those steps descend.  Over
+ * an edge list heads[e] -> tails[e] (int32[nnz]) of an embedding
+ * (float[n * num_dim], point-major):
+ *
+ *   L = sum_e c_e [ log(1 + a s^b)
+ *                   + gamma * sum_{p < m_e, k != head} (log(1 + a s_k^b) - log a - b log s_k) ]
+ *
+ * s = max(FLT_EPSILON, squared distance).  c_e = coef[e] (NULL: 1),
+ * m_e = num_neg[e] (NULL: m).  Negative p of edge e is the layout's
+ * counter-sampler draw for (seed, step, key, p), key = keys[e] (int64; NULL:
+ * e), so the samples are recomputed, never stored; a draw equal to the head
+ * is skipped.  ab = device double[2] {a, b}.  All pointers are GPU pointers;
+ * work is queued on 'stream' (a hipStream_t; NULL = the default stream) and
+ * the call returns once it is done.  An edge index outside [0, n) is
+ * UMX_ERR_GRAPH.  Arithmetic is in double.
+ *
+ * forward: *loss (device double) = L.
+ * backward: grad_embedding (float[n * num_dim]) = grad_out * dL/dy and
+ * grad_ab (double[2]) = grad_out * (dL/da, dL/db), grad_out a device double
+ * (NULL: 1).  Both outputs are overwritten. */
+int umx_loss_forward(int64_t n, int num_dim, const float* embedding, int64_t nnz, const int32_t* heads,
+                     const int32_t* tails, const int64_t* keys, const float* coef, const int32_t* num_neg, int m,
+                     const double* ab, double gamma, uint64_t seed, int step, double* loss, void* stream);
+int umx_loss_backward(int64_t n, int num_dim, const float* embedding, int64_t nnz, const int32_t* heads,
+                      const int32_t* tails, const int64_t* keys, const float* coef, const int32_t* num_neg, int m,
+                      const double* ab, double gamma, uint64_t seed, int step, const double* grad_out,
+                      float* grad_embedding, double* grad_ab, void* stream);
+
+/* ------------------------------------------------------------------ */
 /* Host helpers (no GPU work)                                          */
 /* ------------------------------------------------------------------ */
 

@@ -29,6 +29,8 @@ RNG_MT, RNG_COUNTER, RNG_REPLAY = 0, 1, 2
 
 _lib = None
 _vp = C.c_void_p
+_LOSS_ARGS = [C.c_int64, C.c_int, _vp, C.c_int64, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_double, C.c_double,
+              C.c_double, C.c_uint64, C.c_int]
 
 
 def build() -> Path:
@@ -69,6 +71,9 @@ def lib():
             "orc_gen_normal": (None, [C.c_uint64, C.c_int64, _vp]),
             "orc_gen_uniform01": (None, [C.c_uint64, C.c_int64, _vp]),
             "orc_knn_bruteforce": (None, [C.c_int64, C.c_int, _vp, C.c_int, C.c_int, _vp, _vp]),
+            "orc_loss": (C.c_double, _LOSS_ARGS),
+            "orc_loss_grad": (None, _LOSS_ARGS + [_vp, _vp]),
+            "orc_loss_fd": (None, _LOSS_ARGS + [C.c_int64, _vp, _vp, C.c_double, _vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -256,3 +261,40 @@ def knn_bruteforce(x: np.ndarray, k: int, nthreads=8):
     dist = np.empty((n, k))
     lib().orc_knn_bruteforce(n, dim, _p(x), k, nthreads, _p(idx), _p(dist))
     return idx, dist
+
+
+class LossProblem:
+    """Inputs of the project's UMAP cross-entropy loss (edge list + negative
+    sampling), held as contiguous arrays for the oracle calls."""
+
+    def __init__(self, emb, heads, tails, a, b, gamma=1.0, m=5, seed=0, step=0, keys=None, coef=None, num_neg=None):
+        self.emb = np.ascontiguousarray(emb, dtype=np.float64)
+        self.n, self.dim = self.emb.shape
+        self.heads = np.ascontiguousarray(heads, dtype=np.int32)
+        self.tails = np.ascontiguousarray(tails, dtype=np.int32)
+        self.keys = None if keys is None else np.ascontiguousarray(keys, dtype=np.int64)
+        self.coef = None if coef is None else np.ascontiguousarray(coef, dtype=np.float64)
+        self.num_neg = None if num_neg is None else np.ascontiguousarray(num_neg, dtype=np.int32)
+        self.a, self.b, self.gamma, self.m, self.seed, self.step = float(a), float(b), float(gamma), int(m), int(seed), int(step)
+
+    def _args(self, emb=None, a=None, b=None):
+        e = self.emb if emb is None else np.ascontiguousarray(emb, dtype=np.float64)
+        return [self.n, self.dim, _p(e), self.heads.size, _p(self.heads), _p(self.tails), _p(self.keys), _p(self.coef),
+                _p(self.num_neg), self.m, self.a if a is None else float(a), self.b if b is None else float(b),
+                self.gamma, C.c_uint64(self.seed), self.step]
+
+    def loss(self, emb=None, a=None, b=None) -> float:
+        return float(lib().orc_loss(*self._args(emb, a, b)))
+
+    def grad(self):
+        g = np.zeros((self.n, self.dim))
+        gab = np.zeros(2)
+        lib().orc_loss_grad(*self._args(), _p(g), _p(gab))
+        return g, gab
+
+    def fd(self, points, dims, h=1e-5):
+        pts = np.ascontiguousarray(points, dtype=np.int64)
+        ds = np.ascontiguousarray(dims, dtype=np.int32)
+        out = np.zeros(pts.size)
+        lib().orc_loss_fd(*self._args(), pts.size, _p(pts), _p(ds), float(h), _p(out))
+        return out

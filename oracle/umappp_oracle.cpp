@@ -24,6 +24,12 @@
 //   orc_layout_* (parallel)     pinned (bit-equal to serial, reference test :106-125)
 //   discrete_uniform (rng kind 0) parity unpinned: aarand::discrete_uniform is not in
 //                               the image; this is a documented rejection sampler.
+//   orc_loss / orc_loss_grad /  the project's UMAP cross-entropy loss (no reference
+//   orc_loss_fd                 counterpart): pinned by central finite differences of
+//                               the loss against the closed-form gradient, by the
+//                               reference's per-edge update directions
+//                               (optimize_layout.hpp:145-176) and by hand-computed
+//                               single-term values (tests/test_oracle_loss.py).
 
 #include <algorithm>
 #include <atomic>
@@ -728,6 +734,77 @@ struct OrcLayout {
 // ---------------------------------------------------------------------------
 // C entry points (dtype: 0 = float, 1 = double)
 // ---------------------------------------------------------------------------
+namespace {
+// ---------------------------------------------------------------------------
+// UMAP cross-entropy loss over an edge list (the project's autograd loss).
+//   L = sum_e c_e [ log(1 + a s^b)
+//                   + gamma * sum_{p < m_e, k != head} (log(1 + a s_k^b) - log a - b log s_k) ]
+// s = max(FLT_EPSILON, |y_head - y_other|^2) as in optimize_layout.hpp:94-103.
+// The attractive term's gradient with respect to the head is minus the
+// reference's attractive step direction (optimize_layout.hpp:145-158); the
+// repulsive term's is that of :160-176 with s in place of 0.001 + s.
+// Negative p of edge e is the counter sampler's draw for (seed, step, key_e,
+// p), key_e = keys[e] (or e); a draw equal to the head is skipped.
+// ---------------------------------------------------------------------------
+struct LossIn {
+    int64_t n;
+    int dim;
+    const double* emb;
+    int64_t nnz;
+    const int32_t* heads;
+    const int32_t* tails;
+    const int64_t* keys;
+    const double* coef;
+    const int32_t* nneg;
+    int m;
+    double a, b, gamma;
+    uint64_t seed;
+    int step;
+};
+
+double loss_sqd(const LossIn& L, int64_t i, int64_t j) {
+    double s = 0;
+    for (int d = 0; d < L.dim; ++d) {
+        const double q = L.emb[i * L.dim + d] - L.emb[j * L.dim + d];
+        s += q * q;
+    }
+    return std::max(s, static_cast<double>(std::numeric_limits<float>::epsilon()));
+}
+double loss_attr(const LossIn& L, double s) { return std::log1p(L.a * std::pow(s, L.b)); }
+double loss_rep(const LossIn& L, double s) {
+    return L.gamma * (std::log1p(L.a * std::pow(s, L.b)) - std::log(L.a) - L.b * std::log(s));
+}
+int64_t loss_draw(const LossIn& L, int64_t e, int p) {
+    const uint64_t key = L.keys ? static_cast<uint64_t>(L.keys[e]) : static_cast<uint64_t>(e);
+    return counter_draw(counter_edge_key(counter_epoch_key(L.seed, L.step), key), static_cast<uint32_t>(p),
+                        static_cast<uint32_t>(L.n));
+}
+int loss_m(const LossIn& L, int64_t e) { return L.nneg ? L.nneg[e] : L.m; }
+double loss_c(const LossIn& L, int64_t e) { return L.coef ? L.coef[e] : 1.0; }
+
+double loss_total(const LossIn& L) {
+    double tot = 0;
+    for (int64_t e = 0; e < L.nnz; ++e) {
+        const int64_t i = L.heads[e];
+        double t = loss_attr(L, loss_sqd(L, i, L.tails[e]));
+        for (int p = 0; p < loss_m(L, e); ++p) {
+            const int64_t k = loss_draw(L, e, p);
+            if (k == i) continue;
+            t += loss_rep(L, loss_sqd(L, i, k));
+        }
+        tot += loss_c(L, e) * t;
+    }
+    return tot;
+}
+
+LossIn loss_in(int64_t n, int dim, const double* emb, int64_t nnz, const int32_t* heads, const int32_t* tails,
+               const int64_t* keys, const double* coef, const int32_t* nneg, int m, double a, double b,
+               double gamma, uint64_t seed, int step) {
+    return LossIn{n, dim, emb, nnz, heads, tails, keys, coef, nneg, m, a, b, gamma, seed, step};
+}
+
+}  // namespace
+
 extern "C" {
 
 int orc_neighbor_similarities(int dtype, int64_t n, const int64_t* rp, void* val, double lc, double bw, double mkds,
@@ -896,6 +973,123 @@ void orc_gen_normal(uint64_t seed, int64_t count, double* out) { gen_normal(seed
 void orc_gen_uniform01(uint64_t seed, int64_t count, double* out) { gen_uniform01(seed, count, out); }
 void orc_knn_bruteforce(int64_t n, int dim, const double* x, int k, int nthreads, int32_t* idx, double* dist) {
     knn_bruteforce(n, dim, x, k, nthreads, idx, dist);
+}
+
+// Loss value (double).
+double orc_loss(int64_t n, int dim, const double* emb, int64_t nnz, const int32_t* heads, const int32_t* tails,
+                const int64_t* keys, const double* coef, const int32_t* nneg, int m, double a, double b, double gamma,
+                uint64_t seed, int step) {
+    return loss_total(loss_in(n, dim, emb, nnz, heads, tails, keys, coef, nneg, m, a, b, gamma, seed, step));
+}
+
+// Closed-form gradient of orc_loss with respect to the embedding (grad,
+// n * dim) and to (a, b) (gab[2]).  A clamped distance (s = FLT_EPSILON)
+// contributes nothing to the embedding gradient.
+void orc_loss_grad(int64_t n, int dim, const double* emb, int64_t nnz, const int32_t* heads, const int32_t* tails,
+                   const int64_t* keys, const double* coef, const int32_t* nneg, int m, double a, double b,
+                   double gamma, uint64_t seed, int step, double* grad, double* gab) {
+    const LossIn L = loss_in(n, dim, emb, nnz, heads, tails, keys, coef, nneg, m, a, b, gamma, seed, step);
+    std::fill(grad, grad + n * dim, 0.0);
+    gab[0] = gab[1] = 0;
+    const double feps = std::numeric_limits<float>::epsilon();
+    auto pair = [&](int64_t i, int64_t j, double c, bool rep) {
+        double s = 0;
+        for (int d = 0; d < dim; ++d) {
+            const double q = emb[i * dim + d] - emb[j * dim + d];
+            s += q * q;
+        }
+        const bool clamped = !(s > feps);
+        if (clamped) s = feps;
+        const double sb = std::pow(s, b), den = 1.0 + a * sb, ls = std::log(s);
+        double dlds;
+        if (!rep) {
+            dlds = c * a * b * sb / (s * den);
+            gab[0] += c * sb / den;
+            gab[1] += c * a * sb * ls / den;
+        } else {
+            dlds = -c * gamma * b / (s * den);
+            gab[0] += c * gamma * (sb / den - 1.0 / a);
+            gab[1] += -c * gamma * ls / den;
+        }
+        if (clamped) return;
+        for (int d = 0; d < dim; ++d) {
+            const double g = 2.0 * dlds * (emb[i * dim + d] - emb[j * dim + d]);
+            grad[i * dim + d] += g;
+            grad[j * dim + d] -= g;
+        }
+    };
+    for (int64_t e = 0; e < nnz; ++e) {
+        const int64_t i = heads[e];
+        const double c = loss_c(L, e);
+        pair(i, tails[e], c, false);
+        for (int p = 0; p < loss_m(L, e); ++p) {
+            const int64_t k = loss_draw(L, e, p);
+            if (k != i) pair(i, k, c, true);
+        }
+    }
+}
+
+// Central finite differences of orc_loss, step h, with respect to
+// emb[points[q] * dim + dims[q]] (out[q]).  Only the terms that involve the
+// perturbed observation change, so each difference sums those terms (found
+// once through per-observation incidence lists) at y + h and y - h.
+void orc_loss_fd(int64_t n, int dim, const double* emb, int64_t nnz, const int32_t* heads, const int32_t* tails,
+                 const int64_t* keys, const double* coef, const int32_t* nneg, int m, double a, double b,
+                 double gamma, uint64_t seed, int step, int64_t count, const int64_t* points, const int32_t* dims,
+                 double h, double* out) {
+    const LossIn L0 = loss_in(n, dim, emb, nnz, heads, tails, keys, coef, nneg, m, a, b, gamma, seed, step);
+    // incidence: attractive terms (edge e) and repulsive terms (edge e, draw p)
+    std::vector<int64_t> cnt(static_cast<size_t>(n) + 1, 0);
+    std::vector<std::pair<int64_t, int32_t>> terms;  // (edge, -1 attractive | draw p)
+    std::vector<int64_t> owner;
+    for (int64_t e = 0; e < nnz; ++e) {
+        const int64_t i = heads[e];
+        terms.emplace_back(e, -1);
+        owner.push_back(i);
+        terms.emplace_back(e, -1);
+        owner.push_back(tails[e]);
+        for (int p = 0; p < loss_m(L0, e); ++p) {
+            const int64_t k = loss_draw(L0, e, p);
+            if (k == i) continue;
+            terms.emplace_back(e, p);
+            owner.push_back(i);
+            terms.emplace_back(e, p);
+            owner.push_back(k);
+        }
+    }
+    for (int64_t o : owner) ++cnt[o + 1];
+    for (int64_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    std::vector<std::pair<int64_t, int32_t>> by(terms.size());
+    {
+        std::vector<int64_t> at(cnt.begin(), cnt.end() - 1);
+        for (size_t t = 0; t < terms.size(); ++t) by[at[owner[t]]++] = terms[t];
+    }
+    std::vector<double> y(emb, emb + n * dim);
+    LossIn L = L0;
+    L.emb = y.data();
+    auto local = [&](int64_t pt) {
+        double tot = 0;
+        for (int64_t t = cnt[pt]; t < cnt[pt + 1]; ++t) {
+            const int64_t e = by[t].first;
+            const int p = by[t].second;
+            const int64_t i = heads[e];
+            const double c = loss_c(L, e);
+            if (p < 0) tot += c * loss_attr(L, loss_sqd(L, i, tails[e]));
+            else tot += c * loss_rep(L, loss_sqd(L, i, loss_draw(L, e, p)));
+        }
+        return tot;
+    };
+    for (int64_t q = 0; q < count; ++q) {
+        const int64_t pt = points[q];
+        const int64_t at = pt * dim + dims[q];
+        const double y0 = y[at];
+        y[at] = y0 + h;
+        const double up = local(pt);
+        y[at] = y0 - h;
+        const double dn = local(pt);
+        y[at] = y0;
+        out[q] = (up - dn) / (2.0 * h);
+    }
 }
 
 }  // extern "C"

@@ -120,7 +120,9 @@ def lib():
         p = library_path()
         if not p.exists():
             raise LibraryNotBuilt(f"{p} is not built; run __graft_entry__.build() or python -m umappp_amd.build")
-        L = C.CDLL(str(p))
+        # global symbol scope: the libtorch extension of the autograd loss
+        # (umappp_amd.autograd) resolves its umx_* calls against this library
+        L = C.CDLL(str(p), mode=C.RTLD_GLOBAL)
         _declare(L)
         _lib_handle = L
     return _lib_handle

@@ -1,0 +1,169 @@
+"""UMAP cross-entropy loss as a torch autograd function, and a layout driven by it.
+
+:func:`umap_loss` is a ``torch::autograd::Function`` (libtorch C++ extension
+``umappp_amd_torch``) whose forward and backward are HIP kernels of
+libumappp_amd (``umx_loss_forward`` / ``umx_loss_backward``).  Over an edge
+list ``heads[e] -> tails[e]`` of an embedding ``y`` (float32, n x dim)::
+
+    L = sum_e c_e [ log(1 + a s^b)
+                    + gamma * sum_{p < m_e, k != head} (log(1 + a s_k^b) - log a - b log s_k) ]
+
+with ``s = max(FLT_EPSILON, |y_head - y_other|^2)``.  Gradients reach ``y``
+and the curve parameters ``a`` and ``b``.  Negative sample ``p`` of edge ``e``
+is the layout's counter-sampler draw for ``(seed, step, keys[e], p)``; the
+backward pass recomputes it.  There is no reference counterpart: the
+reference applies per-edge SGD steps (``optimize_layout.hpp:145-176``), and
+minus this loss's per-term gradient with respect to the head is exactly the
+reference's attractive step direction, and its repulsive one with ``s`` in
+place of ``0.001 + s``.
+
+:class:`AutogradLayout` runs the reference's epoch schedule
+(``optimize_layout.hpp:116-185``: which edges fire in an epoch and how many
+negative samples each draws) but moves the embedding by the gradient of the
+epoch's loss, computed through :func:`umap_loss`.
+"""
+from __future__ import annotations
+
+import importlib.util
+from pathlib import Path
+
+import numpy as np
+
+from . import LibraryNotBuilt, lib
+
+_PKG = Path(__file__).resolve().parent
+_EXT_NAME = "umappp_amd_torch"
+_EXT_PATH = _PKG / "_torch_build" / f"{_EXT_NAME}.so"
+_ext = None
+
+
+def extension():
+    """Loads the libtorch extension (raises LibraryNotBuilt when absent)."""
+    global _ext
+    if _ext is None:
+        import torch  # noqa: F401  (the extension needs libtorch loaded)
+
+        lib()  # global scope: the extension calls the library's C ABI
+        if not _EXT_PATH.exists():
+            raise LibraryNotBuilt(f"{_EXT_PATH} is not built; run __graft_entry__.build()")
+        spec = importlib.util.spec_from_file_location(_EXT_NAME, str(_EXT_PATH))
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        _ext = mod
+    return _ext
+
+
+def _scalar(x, dev):
+    import torch
+
+    if isinstance(x, torch.Tensor):
+        return x
+    return torch.tensor(float(x), dtype=torch.float64, device=dev)
+
+
+def umap_loss(emb, a, b, heads, tails, *, keys=None, coef=None, num_neg=None, m=5, gamma=1.0, seed=0, step=0):
+    """UMAP cross-entropy of ``emb`` over the edge list (see the module docstring).
+
+    ``emb``: float32 (n, dim) GPU tensor; ``a``, ``b``: floats or float64
+    scalar tensors (gradients flow to tensors that require them);
+    ``heads``/``tails``: int32 GPU tensors; ``keys`` (int64, sampler key per
+    edge; default the edge's position), ``coef`` (float32) and ``num_neg``
+    (int32, negatives per edge; default ``m``) are optional.  Returns a
+    float64 scalar tensor.
+    """
+    import torch
+
+    dev = emb.device
+    empty = lambda dt: torch.empty(0, dtype=dt, device=dev)  # noqa: E731
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    return extension().umap_loss(
+        emb, _scalar(a, dev), _scalar(b, dev), heads, tails,
+        keys if keys is not None else empty(torch.int64),
+        coef if coef is not None else empty(torch.float32),
+        num_neg if num_neg is not None else empty(torch.int32),
+        int(m), float(gamma), int(seed) & (2**63 - 1), int(step), int(stream))
+
+
+def edge_heads(row_ptr):
+    """Head of every entry of a CSR (int32), from its row offsets."""
+    import torch
+
+    rp = torch.as_tensor(row_ptr)
+    counts = (rp[1:] - rp[:-1]).to(torch.int64)
+    return torch.repeat_interleave(torch.arange(counts.numel(), device=rp.device, dtype=torch.int32), counts)
+
+
+class AutogradLayout:
+    """Layout epochs whose moves come from the autograd loss.
+
+    The schedule is the reference's (``optimize_layout.hpp``): edge ``j`` fires
+    in epoch ``n`` when ``epoch_of_next_sample[j] <= n`` (:141-143) and then
+    draws ``num_neg = (int)((n - epoch_of_next_negative_sample[j]) /
+    (epochs_per_sample[j] / negative_sample_rate))`` negative samples
+    (:160-161), after which both schedule values advance (:178-179).  The
+    epoch's loss is :func:`umap_loss` over the firing edges with those sample
+    counts (keyed by the edge's schedule index and the epoch, so the samples
+    are the plain layout kernel's), and the embedding moves by
+    ``alpha_n * clamp(-grad, -4, 4)`` with the reference's learning-rate decay
+    (:131-133) and clip value (:105-110).
+
+    The loss's repulsive terms move both the head and the sampled point, where
+    the reference moves only the head; since every observation is sampled as
+    often as it samples, the loss uses ``gamma / 2`` so that the expected
+    repulsion per observation matches the reference's.
+    """
+
+    def __init__(self, row_ptr, targets, epochs_per_sample, num_epochs, negative_sample_rate=5.0, a=None, b=None,
+                 gamma=1.0, learning_rate=1.0, seed=1234567890, device="cuda"):
+        import torch
+
+        self.dev = torch.device(device)
+        self.rp = torch.as_tensor(np.asarray(row_ptr, dtype=np.int64), device=self.dev)
+        self.tails = torch.as_tensor(np.asarray(targets, dtype=np.int32), device=self.dev)
+        self.heads = edge_heads(self.rp).contiguous()
+        self.eps = torch.as_tensor(np.asarray(epochs_per_sample, dtype=np.float32), device=self.dev)
+        self.nsr = float(np.float32(negative_sample_rate))
+        self.next = self.eps.clone()
+        self.next_neg = self.eps / np.float32(self.nsr)
+        self.keys = torch.arange(self.eps.numel(), dtype=torch.int64, device=self.dev)
+        self.total = int(num_epochs)
+        self.current = 0
+        self.a = torch.tensor(float(a), dtype=torch.float64, device=self.dev)
+        self.b = torch.tensor(float(b), dtype=torch.float64, device=self.dev)
+        self.gamma = float(gamma)
+        self.alpha0 = float(learning_rate)
+        self.seed = int(seed)
+        self.fires = []
+
+    def _alpha(self, n):
+        # (float)(alpha0 * (1.0 - epoch / num_epochs)): the quotient in float,
+        # the rest in double (optimize_layout.hpp:131-133)
+        x = float(np.float32(n) / np.float32(self.total))
+        return float(np.float32(float(np.float32(self.alpha0)) * (1.0 - x)))
+
+    def run(self, emb, epoch_limit=None):
+        """Runs epochs [current, epoch_limit) on ``emb`` (float32 (n, dim) GPU
+        tensor, updated in place)."""
+        import torch
+
+        limit = self.total if epoch_limit is None else int(epoch_limit)
+        epns_all = self.eps / np.float32(self.nsr)
+        for n in range(self.current, limit):
+            ep = torch.tensor(np.float32(n), device=self.dev)
+            fire = self.next <= ep
+            idx = torch.nonzero(fire).squeeze(1)
+            self.fires.append(int(idx.numel()))
+            if idx.numel():
+                epns = epns_all[idx]
+                nneg = torch.trunc((ep - self.next_neg[idx]) / epns).to(torch.int32)
+                y = emb.detach().requires_grad_(True)
+                loss = umap_loss(y, self.a, self.b, self.heads[idx].contiguous(), self.tails[idx].contiguous(),
+                                 keys=self.keys[idx].contiguous(), num_neg=nneg.contiguous(),
+                                 gamma=0.5 * self.gamma, seed=self.seed, step=n)
+                (g,) = torch.autograd.grad(loss, [y])
+                with torch.no_grad():
+                    emb.add_(torch.clamp(-g, -4.0, 4.0), alpha=self._alpha(n))
+                self.next[idx] += self.eps[idx]
+                self.next_neg[idx] += nneg.to(torch.float32) * epns
+            self.current = n + 1
+        return emb

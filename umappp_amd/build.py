@@ -18,8 +18,10 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "libumappp_amd.so"
+TORCH_DIR = PKG / "_torch_build"
+TORCH_NAME = "umappp_amd_torch"
 
-SOURCES = ["graph_kernels.hip", "layout_kernels.hip", "sell_kernels.hip", "capi.hip"]
+SOURCES = ["graph_kernels.hip", "layout_kernels.hip", "sell_kernels.hip", "loss_kernels.hip", "capi.hip"]
 HOST_SOURCES = ["host_api.cpp"]
 CXX = os.environ.get("CXX", "g++")
 
@@ -68,11 +70,38 @@ def build(verbose: bool = False) -> Path:
         for fut in [ex.submit(_run, c) for c in cmds]:
             fut.result()
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-soname,libumappp_amd.so", "-o", str(tmp),
+          *map(str, objs)])
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_torch(verbose)
     return LIB
+
+
+def build_torch(verbose: bool = False) -> Path:
+    """Builds the libtorch extension (umappp_amd/_torch_build/umappp_amd_torch.so)
+    holding the autograd loss.  It calls libumappp_amd's C ABI, whose symbols
+    it takes from the library that umappp_amd loads first (RTLD_GLOBAL)."""
+    import ctypes
+
+    import torch  # noqa: F401  (the extension links against this torch)
+    from torch.utils import cpp_extension
+
+    ctypes.CDLL(str(LIB), mode=ctypes.RTLD_GLOBAL)  # the extension's umx_* symbols
+    TORCH_DIR.mkdir(exist_ok=True)
+    for old in (TORCH_DIR / "torch_loss.o", TORCH_DIR / f"{TORCH_NAME}.so"):  # always compile afresh
+        if old.exists():
+            old.unlink()
+    cpp_extension.load(name=TORCH_NAME, sources=[str(CSRC / "torch_loss.cpp")],
+                       extra_include_paths=[str(ROOT / "include")], extra_cflags=["-O2"],
+                       build_directory=str(TORCH_DIR), with_cuda=False, verbose=verbose, is_python_module=True)
+    out = TORCH_DIR / f"{TORCH_NAME}.so"
+    if not out.exists():
+        raise RuntimeError(f"torch extension was not produced at {out}")
+    if verbose:
+        print(f"built {out}")
+    return out
 
 
 if __name__ == "__main__":

@@ -15,6 +15,7 @@
 #include <mutex>
 #include <new>
 #include <tuple>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -27,19 +28,7 @@ thread_local std::string g_last_error;
 
 template <typename F>
 int guard(F&& f) {
-    try {
-        f();
-        return UMX_OK;
-    } catch (const Error& e) {
-        g_last_error = e.what();
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        g_last_error = "host allocation failed";
-        return UMX_ERR_DEVICE;
-    } catch (const std::exception& e) {
-        g_last_error = e.what();
-        return UMX_ERR_DEVICE;
-    }
+    return umx::guarded(std::forward<F>(f));
 }
 
 void require(bool ok, int code, const char* msg) {
@@ -75,6 +64,8 @@ void check_rowptr_device(const int64_t* rp, int64_t n, int64_t nnz, hipStream_t 
 size_t dsize(int dtype) { return dtype == UMX_F32 ? sizeof(float) : sizeof(double); }
 
 }  // namespace
+
+void umx::set_last_error(const std::string& msg) { g_last_error = msg; }
 
 // ---------------------------------------------------------------------------
 // Graph handle

@@ -1,0 +1,370 @@
+// UMAP cross-entropy loss and its gradient (forward / backward kernels of the
+// autograd loss).  No reference counterpart: the reference only applies
+// per-edge SGD steps (optimize_layout.hpp:145-176); this is the loss whose
+// gradient those steps follow.  Over an edge list (head, tail) with
+// per-edge coefficients c_e and m_e negative samples:
+//
+//   L = sum_e c_e [ log(1 + a s^b)
+//                   + gamma * sum_{p < m_e, k != head} (log(1 + a s_k^b) - log a - b log s_k) ]
+//
+// s = max(FLT_EPSILON, |y_head - y_other|^2).  Minus the gradient with
+// respect to the head is the reference's attractive step direction, and its
+// repulsive direction with s in place of 0.001 + s.  Negative p of edge e is
+// the layout's counter-sampler draw for (seed, step, key_e, p): the backward
+// pass recomputes the samples instead of storing them.
+//
+// Arithmetic is in double (the embedding is float): the loss is a sum of
+// tens of millions of terms and the gradient of a point often nearly cancels
+// over its terms.  One thread per edge; the head's gradient is accumulated
+// in registers, the tail's and the negative samples' with double atomics
+// into a scratch buffer, which a last kernel scales by the upstream gradient
+// and rounds to float.
+#include "layout_device.hpp"
+#include "umappp_amd.h"
+#include "umx_internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <string>
+#include <type_traits>
+
+namespace umx {
+namespace {
+
+using namespace umx::dev;
+
+constexpr int kMaxLossDim = 16;
+constexpr int kLossBlock = 256;
+
+struct LossView {
+    int64_t n;
+    int dim;
+    const float* emb;
+    int64_t nnz;
+    const int32_t* heads;
+    const int32_t* tails;
+    const int64_t* keys;
+    const float* coef;
+    const int32_t* nneg;
+    int m;
+    const double* ab;  // device: a, b
+    double gamma;
+    uint64_t ekey;     // counter-sampler key of 'step'
+    int* bad;          // set when an index lies outside [0, n)
+};
+
+template <int D>
+struct DVec {
+    double c[D];
+};
+
+template <int D>
+__device__ inline DVec<D> load_d(const float* emb, int64_t i, int dim) {
+    DVec<D> v;
+#pragma unroll
+    for (int d = 0; d < D; ++d) v.c[d] = d < dim ? static_cast<double>(emb[i * dim + d]) : 0.0;
+    return v;
+}
+
+template <int D>
+__device__ inline double sqd(const DVec<D>& x, const DVec<D>& y) {
+    double s = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const double q = x.c[d] - y.c[d];
+        s += q * q;
+    }
+    return s;
+}
+
+__device__ inline double block_sum(double v, double* lds) {
+#pragma unroll
+    for (int o = kWave / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
+    const int w = threadIdx.x / kWave;
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) lds[w] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0)
+        for (int q = 0; q < kLossBlock / kWave; ++q) t += lds[q];
+    return t;
+}
+
+__device__ inline bool edge_ok(const LossView& v, int64_t e, int64_t& i, int64_t& j) {
+    i = v.heads[e];
+    j = v.tails[e];
+    if (i < 0 || i >= v.n || j < 0 || j >= v.n) {
+        atomicOr(v.bad, 1);
+        return false;
+    }
+    return true;
+}
+
+// D = compile-time bound on the embedding dimension (1, 2, 3, 4 or 16).
+template <int D>
+__global__ void __launch_bounds__(kLossBlock) k_loss_forward(LossView v, double* __restrict__ loss) {
+    __shared__ double lds[kLossBlock / kWave];
+    const double a = v.ab[0], b = v.ab[1];
+    const double feps = FLT_EPSILON;
+    const double log_a = log(a);
+    const uint32_t n32 = static_cast<uint32_t>(v.n);
+    double acc = 0;
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * kLossBlock + threadIdx.x; e < v.nnz;
+         e += static_cast<int64_t>(gridDim.x) * kLossBlock) {
+        int64_t i, j;
+        if (!edge_ok(v, e, i, j)) continue;
+        const DVec<D> yi = load_d<D>(v.emb, i, v.dim);
+        const DVec<D> yj = load_d<D>(v.emb, j, v.dim);
+        double s = sqd<D>(yi, yj);
+        s = s > feps ? s : feps;
+        double t = log1p(a * exp(b * log(s)));
+        const int m = v.nneg ? v.nneg[e] : v.m;
+        const uint64_t ej = edge_key(v.ekey, v.keys ? static_cast<uint64_t>(v.keys[e]) : static_cast<uint64_t>(e));
+        for (int p = 0; p < m; ++p) {
+            const int64_t k = draw_index(ej, static_cast<uint32_t>(p), n32);
+            if (k == i) continue;
+            const DVec<D> yk = load_d<D>(v.emb, k, v.dim);
+            double s2 = sqd<D>(yi, yk);
+            s2 = s2 > feps ? s2 : feps;
+            const double ls = log(s2);
+            t += v.gamma * (log1p(a * exp(b * ls)) - log_a - b * ls);
+        }
+        acc += (v.coef ? static_cast<double>(v.coef[e]) : 1.0) * t;
+    }
+    const double tot = block_sum(acc, lds);
+    if (threadIdx.x == 0 && tot != 0) unsafeAtomicAdd(loss, tot);
+}
+
+template <int D>
+__global__ void __launch_bounds__(kLossBlock) k_loss_backward(LossView v, double* __restrict__ g64,
+                                                            double* __restrict__ gab) {
+    __shared__ double lds[kLossBlock / kWave];
+    const double a = v.ab[0], b = v.ab[1];
+    const double feps = FLT_EPSILON;
+    const uint32_t n32 = static_cast<uint32_t>(v.n);
+    const int dim = v.dim;
+    double da = 0, db = 0;
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * kLossBlock + threadIdx.x; e < v.nnz;
+         e += static_cast<int64_t>(gridDim.x) * kLossBlock) {
+        int64_t i, j;
+        if (!edge_ok(v, e, i, j)) continue;
+        const double c = v.coef ? static_cast<double>(v.coef[e]) : 1.0;
+        const DVec<D> yi = load_d<D>(v.emb, i, dim);
+        DVec<D> gi{};
+        {
+            const DVec<D> yj = load_d<D>(v.emb, j, dim);
+            double s = sqd<D>(yi, yj);
+            const bool clamped = !(s > feps);
+            if (clamped) s = feps;
+            const double ls = log(s), sb = exp(b * ls), den = 1.0 + a * sb;
+            da += c * sb / den;
+            db += c * a * sb * ls / den;
+            if (!clamped) {
+                const double f = 2.0 * c * a * b * sb / (s * den);
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    if (d >= dim) break;
+                    const double g = f * (yi.c[d] - yj.c[d]);
+                    gi.c[d] += g;
+                    unsafeAtomicAdd(g64 + j * dim + d, -g);
+                }
+            }
+        }
+        const int m = v.nneg ? v.nneg[e] : v.m;
+        const uint64_t ej = edge_key(v.ekey, v.keys ? static_cast<uint64_t>(v.keys[e]) : static_cast<uint64_t>(e));
+        for (int p = 0; p < m; ++p) {
+            const int64_t k = draw_index(ej, static_cast<uint32_t>(p), n32);
+            if (k == i) continue;
+            const DVec<D> yk = load_d<D>(v.emb, k, dim);
+            double s = sqd<D>(yi, yk);
+            const bool clamped = !(s > feps);
+            if (clamped) s = feps;
+            const double ls = log(s), sb = exp(b * ls), den = 1.0 + a * sb;
+            da += c * v.gamma * (sb / den - 1.0 / a);
+            db += -c * v.gamma * ls / den;
+            if (!clamped) {
+                const double f = -2.0 * c * v.gamma * b / (s * den);
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    if (d >= dim) break;
+                    const double g = f * (yi.c[d] - yk.c[d]);
+                    gi.c[d] += g;
+                    unsafeAtomicAdd(g64 + k * dim + d, -g);
+                }
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            if (d >= dim) break;
+            unsafeAtomicAdd(g64 + i * dim + d, gi.c[d]);
+        }
+    }
+    const double ta = block_sum(da, lds);
+    const double tb = block_sum(db, lds);
+    if (threadIdx.x == 0) {
+        if (ta != 0) unsafeAtomicAdd(gab, ta);
+        if (tb != 0) unsafeAtomicAdd(gab + 1, tb);
+    }
+}
+
+__global__ void __launch_bounds__(kLossBlock) k_loss_finish(int64_t count, const double* __restrict__ g64,
+                                                          const double* __restrict__ grad_out,
+                                                          float* __restrict__ grad_emb, double* __restrict__ gab) {
+    const double go = grad_out ? *grad_out : 1.0;
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * kLossBlock + threadIdx.x;
+    if (q < count) grad_emb[q] = static_cast<float>(go * g64[q]);
+    if (q == 0) {
+        gab[0] *= go;
+        gab[1] *= go;
+    }
+}
+
+__global__ void k_zero_f64(double* p, int64_t count) {
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * kLossBlock + threadIdx.x;
+    if (q < count) p[q] = 0.0;
+}
+__global__ void k_zero_i32(int* p) { *p = 0; }
+
+unsigned loss_grid(int64_t nnz) {
+    const int64_t want = (nnz + kLossBlock - 1) / kLossBlock;
+    return static_cast<unsigned>(want < 1 ? 1 : (want > 16384 ? 16384 : want));
+}
+
+template <typename F>
+void dispatch_dim(int dim, F&& f) {
+    switch (dim) {
+        case 1: f(std::integral_constant<int, 1>{}); break;
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 3: f(std::integral_constant<int, 3>{}); break;
+        case 4: f(std::integral_constant<int, 4>{}); break;
+        default: f(std::integral_constant<int, kMaxLossDim>{}); break;
+    }
+}
+
+struct LossArgs {
+    int64_t n;
+    int dim;
+    const float* emb;
+    int64_t nnz;
+    const int32_t* heads;
+    const int32_t* tails;
+    const int64_t* keys;
+    const float* coef;
+    const int32_t* num_neg;
+    int m;
+    const double* ab;
+    double gamma;
+    uint64_t seed;
+    int step;
+};
+
+void check_args(const LossArgs& x) {
+    if (x.n < 0 || x.n >= 0x7fffffffLL) throw Error(UMX_ERR_OVERFLOW, "n must fit a 32-bit index");
+    if (x.dim < 1 || x.dim > kMaxLossDim) throw Error(UMX_ERR_ARGUMENT, "num_dim must lie in [1, 16]");
+    if (x.nnz < 0 || x.m < 0) throw Error(UMX_ERR_ARGUMENT, "negative edge or sample count");
+    if (x.ab == nullptr) throw Error(UMX_ERR_ARGUMENT, "ab is NULL");
+    if (x.nnz > 0 && (x.emb == nullptr || x.heads == nullptr || x.tails == nullptr))
+        throw Error(UMX_ERR_ARGUMENT, "emb/heads/tails is NULL");
+    if (x.nnz > 0 && x.n == 0) throw Error(UMX_ERR_GRAPH, "edges over an empty embedding");
+}
+
+LossView view_of(const LossArgs& x, int* bad) {
+    LossView v;
+    v.n = x.n;
+    v.dim = x.dim;
+    v.emb = x.emb;
+    v.nnz = x.nnz;
+    v.heads = x.heads;
+    v.tails = x.tails;
+    v.keys = x.keys;
+    v.coef = x.coef;
+    v.nneg = x.num_neg;
+    v.m = x.m;
+    v.ab = x.ab;
+    v.gamma = x.gamma;
+    v.ekey = counter_epoch_key(x.seed, x.step);
+    v.bad = bad;
+    return v;
+}
+
+// Error flag of a call: a device int, cleared, checked once the stream is done.
+struct BadFlag {
+    int* p = nullptr;
+    hipStream_t s;
+    explicit BadFlag(hipStream_t st) : s(st) {
+        UMX_HIP(hipMallocAsync(reinterpret_cast<void**>(&p), sizeof(int), s));
+        k_zero_i32<<<1, 1, 0, s>>>(p);
+        UMX_HIP(hipGetLastError());
+    }
+    ~BadFlag() {
+        if (p) (void)hipFreeAsync(p, s);
+    }
+    void check() {
+        int h = 0;
+        UMX_HIP(hipMemcpyAsync(&h, p, sizeof(int), hipMemcpyDeviceToHost, s));
+        UMX_HIP(hipStreamSynchronize(s));
+        if (h) throw Error(UMX_ERR_GRAPH, "an edge index lies outside [0, n)");
+    }
+};
+
+void loss_forward(const LossArgs& x, double* loss, hipStream_t s) {
+    check_args(x);
+    if (loss == nullptr) throw Error(UMX_ERR_ARGUMENT, "loss is NULL");
+    BadFlag bad(s);
+    k_zero_f64<<<1, kLossBlock, 0, s>>>(loss, 1);
+    const LossView v = view_of(x, bad.p);
+    if (x.nnz > 0)
+        dispatch_dim(x.dim, [&](auto Dc) { k_loss_forward<decltype(Dc)::value><<<loss_grid(x.nnz), kLossBlock, 0, s>>>(v, loss); });
+    UMX_HIP(hipGetLastError());
+    bad.check();
+}
+
+void loss_backward(const LossArgs& x, const double* grad_out, float* grad_emb, double* grad_ab, hipStream_t s) {
+    check_args(x);
+    if (grad_ab == nullptr || (x.n > 0 && grad_emb == nullptr)) throw Error(UMX_ERR_ARGUMENT, "output is NULL");
+    BadFlag bad(s);
+    const int64_t count = x.n * x.dim;
+    double* g64 = nullptr;
+    UMX_HIP(hipMallocAsync(reinterpret_cast<void**>(&g64), sizeof(double) * static_cast<size_t>(count > 0 ? count : 1), s));
+    const unsigned zg = static_cast<unsigned>((count + kLossBlock - 1) / kLossBlock);
+    if (count > 0) k_zero_f64<<<zg, kLossBlock, 0, s>>>(g64, count);
+    k_zero_f64<<<1, kLossBlock, 0, s>>>(grad_ab, 2);
+    const LossView v = view_of(x, bad.p);
+    if (x.nnz > 0)
+        dispatch_dim(x.dim, [&](auto Dc) {
+            k_loss_backward<decltype(Dc)::value><<<loss_grid(x.nnz), kLossBlock, 0, s>>>(v, g64, grad_ab);
+        });
+    k_loss_finish<<<count > 0 ? zg : 1, kLossBlock, 0, s>>>(count, g64, grad_out, grad_emb, grad_ab);
+    const hipError_t e = hipGetLastError();
+    (void)hipFreeAsync(g64, s);
+    UMX_HIP(e);
+    bad.check();
+}
+
+}  // namespace
+}  // namespace umx
+
+extern "C" {
+
+int umx_loss_forward(int64_t n, int num_dim, const float* embedding, int64_t nnz, const int32_t* heads,
+                     const int32_t* tails, const int64_t* keys, const float* coef, const int32_t* num_neg, int m,
+                     const double* ab, double gamma, uint64_t seed, int step, double* loss, void* stream) {
+    return umx::guarded([&] {
+        const umx::LossArgs x{n, num_dim, embedding, nnz, heads, tails, keys, coef, num_neg, m, ab, gamma, seed, step};
+        umx::loss_forward(x, loss, static_cast<hipStream_t>(stream));
+    });
+}
+
+int umx_loss_backward(int64_t n, int num_dim, const float* embedding, int64_t nnz, const int32_t* heads,
+                      const int32_t* tails, const int64_t* keys, const float* coef, const int32_t* num_neg, int m,
+                      const double* ab, double gamma, uint64_t seed, int step, const double* grad_out,
+                      float* grad_embedding, double* grad_ab, void* stream) {
+    return umx::guarded([&] {
+        const umx::LossArgs x{n, num_dim, embedding, nnz, heads, tails, keys, coef, num_neg, m, ab, gamma, seed, step};
+        umx::loss_backward(x, grad_out, grad_embedding, grad_ab, static_cast<hipStream_t>(stream));
+    });
+}
+
+}  // extern "C"

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <new>
 #include <stdexcept>
 #include <string>
 
@@ -15,6 +16,27 @@ struct Error : std::runtime_error {
     int code;
     Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
 };
+
+// Thread-local message of the last failed C ABI call (umx_last_error).
+void set_last_error(const std::string& msg);
+
+// Runs f, turning exceptions into a umx_status (0 on success).
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return -3;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return -3;
+    }
+}
 
 #define UMX_HIP(call)                                                                                    \
     do {                                                                                                 \
