@@ -38,6 +38,10 @@ CONFIGS = {
                label="1M x 50 f32, 100-component mixture, exact kNN k=15, D=2, 200 epochs, nsr 5"),
     "c2": dict(n=100_000, dim=50, ncomp=30, seed=2, k=15, epochs=200,
                label="100k x 50 f32, 30-component mixture, exact kNN k=15, D=2, 200 epochs, nsr 5"),
+    "c5": dict(n=200_000, dim=50, ncomp=30, seed=5, k=30, epochs=200, out_dim=3, m=5,
+               label="200k x 50 f32, 30-component mixture (seed 5), exact kNN k=30, fuzzy graph, D=3, "
+                     "UMAP cross-entropy over all edges + 5 negatives per edge (counter sampler), "
+                     "forward + backward through torch autograd, then 200 autograd-driven epochs"),
     "c4": dict(count=64, lo=2_000, hi=200_000, dim=30, ncomp=10, seed=4, k=15, epochs=200,
                label="64 independent UMAPs, sizes log-uniform in [2k, 200k] (seed 4), each 30 dims f32 from "
                      "its own 10-component mixture (seed 4000+i), exact kNN k=15, D=2, 200 epochs, nsr 5"),
@@ -104,6 +108,8 @@ def main():
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU layout sample time per mode")
+    ap.add_argument("--skip-sequential", action="store_true",
+                    help="c4 only: time the batched run alone (for profiling the batched kernel)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device-to-device copy three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -132,6 +138,9 @@ def main():
     dev = torch.device("cuda:0")
     if args.config == "c4":
         main_batched(args, umx, torch, dev)
+        return
+    if args.config == "c5":
+        main_autograd(args, umx, torch, dev)
         return
     cfg = CONFIGS[args.config]
     n, k, T = cfg["n"], cfg["k"], cfg["epochs"]
@@ -253,7 +262,7 @@ def preservation(hi_idx, y, k, torch):
     """Mean fraction of each observation's k high-dimensional neighbours that
     are among its k nearest in the embedding."""
     lo = knn_low(y, k, torch)
-    hits = (hi_idx.long()[:, :, None] == lo[:, None, :]).any(2).sum(1)
+    hits = (hi_idx[:, :k].long()[:, :, None] == lo[:, None, :]).any(2).sum(1)
     return float(hits.double().mean()) / k
 
 
@@ -322,16 +331,19 @@ def main_batched(args, umx, torch, dev):
 
     for w in range(args.warmup):
         log(f"warmup {w + 1}/{args.warmup}")
-        plain_runs()
+        if not args.skip_sequential:
+            plain_runs()
         batched_run()
     torch.cuda.synchronize()
-    seq_steps = max(1, min(args.steps, 3))
-    t0 = time.perf_counter()
-    for _ in range(seq_steps):
-        U_plain = plain_runs()
-    torch.cuda.synchronize()
-    seq_ms = 1000.0 * (time.perf_counter() - t0) / seq_steps
-    log(f"64 sequential plain runs: {seq_ms:.1f} ms")
+    seq_ms, U_plain = None, None
+    if not args.skip_sequential:
+        seq_steps = max(1, min(args.steps, 3))
+        t0 = time.perf_counter()
+        for _ in range(seq_steps):
+            U_plain = plain_runs()
+        torch.cuda.synchronize()
+        seq_ms = 1000.0 * (time.perf_counter() - t0) / seq_steps
+        log(f"64 sequential plain runs: {seq_ms:.1f} ms")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         lay, shape = batched_run()
@@ -358,7 +370,8 @@ def main_batched(args, umx, torch, dev):
     for q in range(len(sizes)):
         r0, r1 = int(off[q]), int(off[q + 1])
         pres_b.append(preservation(idx_l[q], out_batch[r0:r1], k, torch))
-        pres_p.append(preservation(idx_l[q], out_plain[r0:r1], k, torch))
+        if seq_ms is not None:
+            pres_p.append(preservation(idx_l[q], out_plain[r0:r1], k, torch))
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -380,11 +393,138 @@ def main_batched(args, umx, torch, dev):
                      "kernel": "k_layout_sell<float,2,*,false> (batched)",
                      "avg_launch_us": round(avg_launch_s * 1e6, 2), "algorithmic_bytes_per_launch": round(avg_bytes)},
         "cpu_baseline": None,
-        "sequential_plain_ms": round(seq_ms, 3),
+        "sequential_plain_ms": None if seq_ms is None else round(seq_ms, 3),
         "sequential_plain_edge_updates": U_plain,
-        "speedup_vs_sequential": round(seq_ms / ms_per_step, 2),
+        "speedup_vs_sequential": None if seq_ms is None else round(seq_ms / ms_per_step, 2),
         "knn15_preservation_batched": round(float(np.mean(pres_b)), 4),
-        "knn15_preservation_plain": round(float(np.mean(pres_p)), 4),
+        "knn15_preservation_plain": round(float(np.mean(pres_p)), 4) if pres_p else None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def main_autograd(args, umx, torch, dev):
+    """Config C5: the UMAP cross-entropy loss and its gradient (HIP forward and
+    backward behind the libtorch autograd Function) over the whole fuzzy graph,
+    then 200 layout epochs driven through the autograd path beside the plain
+    layout on the same data."""
+    from umappp_amd.autograd import AutogradLayout, umap_loss
+
+    cfg = CONFIGS["c5"]
+    n, k, T, D, m = cfg["n"], cfg["k"], cfg["epochs"], cfg["out_dim"], cfg["m"]
+    a, b = umx.find_ab(1.0, 0.1)
+    log(f"generating {cfg['label']}")
+    x = make_mixture(cfg["seed"], n, cfg["dim"], cfg["ncomp"])
+    kidx, kdist = knn_torch(x, k, dev)
+    rp = torch.arange(n + 1, dtype=torch.int64, device=dev) * k
+    g = umx.Graph((rp.data_ptr(), kidx.data_ptr(), kdist.data_ptr()), device_ptrs=True, n=n, dtype=np.float32)
+    g.neighbor_similarities(1.0, 1.0, 1e-3, True)
+    g.combine(1.0)
+    nl = g.export()
+    E = int(nl.row_ptr[-1])
+    heads = torch.from_numpy(np.repeat(np.arange(n, dtype=np.int32), np.diff(nl.row_ptr))).to(dev)
+    tails = torch.from_numpy(nl.idx).to(dev)
+    w = torch.from_numpy(nl.val.astype(np.float32)).to(dev)
+    emb0 = np.random.default_rng(cfg["seed"] + 100).uniform(-10, 10, size=(n, D)).astype(np.float32)
+    y = torch.from_numpy(emb0).to(dev).requires_grad_(True)
+    at = torch.tensor(a, dtype=torch.float64, device=dev, requires_grad=True)
+    bt = torch.tensor(b, dtype=torch.float64, device=dev, requires_grad=True)
+
+    def pass_once():
+        y.grad = None
+        at.grad = None
+        bt.grad = None
+        loss = umap_loss(y, at, bt, heads, tails, coef=w, m=m, seed=2024, step=0)
+        loss.backward()
+        return loss
+
+    for _ in range(max(1, args.warmup)):
+        pass_once()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    fwd_ms = bwd_ms = 0.0
+    steps = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        y.grad = None
+        ev[0].record()
+        loss = umap_loss(y, at, bt, heads, tails, coef=w, m=m, seed=2024, step=0)
+        ev[1].record()
+        ev[2].record()
+        loss.backward()
+        ev[3].record()
+        torch.cuda.synchronize()
+        fwd_ms += ev[0].elapsed_time(ev[1])
+        bwd_ms += ev[2].elapsed_time(ev[3])
+    pass_ms = 1000.0 * (time.perf_counter() - t0) / steps
+    fwd_ms /= steps
+    bwd_ms /= steps
+    log(f"loss pass over {E} edges: forward {fwd_ms:.2f} ms, backward {bwd_ms:.2f} ms, wall {pass_ms:.2f} ms")
+
+    # 200 epochs: plain layout kernel vs autograd-driven epochs, same data and start
+    lay = umx.Layout(graph=g, num_epochs=T, negative_sample_rate=5.0)
+    ed = lay.epoch_data()
+    lay.set_params(D, a, b, 1.0, 1.0, 99)
+    lay.set_embedding(emb0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lay.run(T)
+    plain_s = time.perf_counter() - t0
+    plain = torch.from_numpy(lay.get_embedding()).to(dev)
+    al = AutogradLayout(ed.cumulative_num_edges, ed.edge_targets, ed.epochs_per_sample, T, 5.0, a, b, seed=99)
+    ya = torch.from_numpy(emb0.copy()).to(dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    al.run(ya)
+    torch.cuda.synchronize()
+    auto_s = time.perf_counter() - t0
+    U = int(sum(al.fires))
+    log(f"200 epochs: plain {plain_s * 1000:.1f} ms, autograd-driven {auto_s * 1000:.1f} ms ({U} edge-updates)")
+    pres_plain = preservation(kidx, plain, 15, torch)
+    pres_auto = preservation(kidx, ya, 15, torch)
+
+    # roofline of the backward pass: per edge, the head/tail indices and the
+    # weight (12 B), (2 + m) point gathers of D floats and (2 + m) * D double
+    # gradient updates (read + write)
+    bytes_bwd = E * (12.0 + (2 + m) * D * 4.0 + (2 + m) * D * 16.0) + n * D * (8.0 + 8.0 + 4.0)
+    achieved = bytes_bwd / (bwd_ms / 1000.0) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+
+        P = oracle.LossProblem(emb0.astype(np.float64), heads.cpu().numpy(), nl.idx, a, b, 1.0, m=m, seed=2024,
+                               step=0, coef=nl.val.astype(np.float32).astype(np.float64))
+        t0 = time.perf_counter()
+        P.grad()
+        cpu_s = time.perf_counter() - t0
+        cpu = {"value": round(E / cpu_s, 1), "unit": "edges/s (loss gradient, all edges + 5 negatives)", "cores": 1,
+               "kind": "port", "sample": f"the oracle's FP64 closed-form loss gradient over the full graph "
+                                         f"({E} edges) in {cpu_s:.2f}s on one host core",
+               "host_cpu": _cpu_model()}
+    out = {
+        "metric": METRIC,
+        "value": round(U / auto_s, 1),
+        "unit": "edge-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(auto_s * 1000.0, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 embedding, f64 loss arithmetic",
+        "data": "synthetic",
+        "config": {"workload": f"differentiable loss, {cfg['label']}", "n": n, "k": k, "num_dim": D, "epochs": T,
+                   "stored_edges": E, "negatives_per_edge": m, "edge_updates_autograd_layout": U},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "umx_loss_backward (k_loss_backward<3> + zero/finish)",
+                     "avg_launch_us": round(bwd_ms * 1000.0, 2), "algorithmic_bytes_per_launch": round(bytes_bwd)},
+        "cpu_baseline": cpu,
+        "loss_pass_ms": {"forward": round(fwd_ms, 3), "backward": round(bwd_ms, 3), "wall": round(pass_ms, 3)},
+        "loss_edges_per_s": round(E / ((fwd_ms + bwd_ms) / 1000.0), 1),
+        "layout_200_epochs_ms": {"plain": round(plain_s * 1000, 3), "autograd": round(auto_s * 1000, 3)},
+        "knn15_preservation": {"plain": round(pres_plain, 4), "autograd": round(pres_auto, 4)},
     }
     print(json.dumps(out), flush=True)
 

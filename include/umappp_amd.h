@@ -268,14 +268,18 @@ void umx_layout_destroy(umx_layout* L);
  * forward: *loss (device double) = L.
  * backward: grad_embedding (float[n * num_dim]) = grad_out * dL/dy and
  * grad_ab (double[2]) = grad_out * (dL/da, dL/db), grad_out a device double
- * (NULL: 1).  Both outputs are overwritten. */
+ * (NULL: 1).  Both outputs are overwritten.  clip > 0 clips every term's
+ * gradient with respect to its head to [-clip, clip] per coordinate before
+ * the sum (per-sample clipping; the term's other point receives minus the
+ * clipped value); clip = 4 gives the reference's clipped steps
+ * (optimize_layout.hpp:105-110, 145-176).  clip = 0 is the exact gradient. */
 int umx_loss_forward(int64_t n, int num_dim, const float* embedding, int64_t nnz, const int32_t* heads,
                      const int32_t* tails, const int64_t* keys, const float* coef, const int32_t* num_neg, int m,
                      const double* ab, double gamma, uint64_t seed, int step, double* loss, void* stream);
 int umx_loss_backward(int64_t n, int num_dim, const float* embedding, int64_t nnz, const int32_t* heads,
                       const int32_t* tails, const int64_t* keys, const float* coef, const int32_t* num_neg, int m,
-                      const double* ab, double gamma, uint64_t seed, int step, const double* grad_out,
-                      float* grad_embedding, double* grad_ab, void* stream);
+                      const double* ab, double gamma, uint64_t seed, int step, double clip,
+                      const double* grad_out, float* grad_embedding, double* grad_ab, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Host helpers (no GPU work)                                          */

@@ -72,7 +72,7 @@ def lib():
             "orc_gen_uniform01": (None, [C.c_uint64, C.c_int64, _vp]),
             "orc_knn_bruteforce": (None, [C.c_int64, C.c_int, _vp, C.c_int, C.c_int, _vp, _vp]),
             "orc_loss": (C.c_double, _LOSS_ARGS),
-            "orc_loss_grad": (None, _LOSS_ARGS + [_vp, _vp]),
+            "orc_loss_grad": (None, _LOSS_ARGS + [C.c_double, _vp, _vp]),
             "orc_loss_fd": (None, _LOSS_ARGS + [C.c_int64, _vp, _vp, C.c_double, _vp]),
         }
         for name, (res, args) in sig.items():
@@ -286,10 +286,11 @@ class LossProblem:
     def loss(self, emb=None, a=None, b=None) -> float:
         return float(lib().orc_loss(*self._args(emb, a, b)))
 
-    def grad(self):
+    def grad(self, clip=0.0):
+        """Closed-form gradient; clip > 0 clips each term's head gradient per coordinate first."""
         g = np.zeros((self.n, self.dim))
         gab = np.zeros(2)
-        lib().orc_loss_grad(*self._args(), _p(g), _p(gab))
+        lib().orc_loss_grad(*self._args(), float(clip), _p(g), _p(gab))
         return g, gab
 
     def fd(self, points, dims, h=1e-5):

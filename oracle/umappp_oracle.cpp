@@ -984,10 +984,14 @@ double orc_loss(int64_t n, int dim, const double* emb, int64_t nnz, const int32_
 
 // Closed-form gradient of orc_loss with respect to the embedding (grad,
 // n * dim) and to (a, b) (gab[2]).  A clamped distance (s = FLT_EPSILON)
-// contributes nothing to the embedding gradient.
+// contributes nothing to the embedding gradient.  clip > 0: every term's
+// gradient with respect to its head is clipped to [-clip, clip] per
+// coordinate before the sum, the other point receiving minus the clipped
+// value (per-sample clipping; clip = 4 is the reference's clip,
+// optimize_layout.hpp:105-110).
 void orc_loss_grad(int64_t n, int dim, const double* emb, int64_t nnz, const int32_t* heads, const int32_t* tails,
                    const int64_t* keys, const double* coef, const int32_t* nneg, int m, double a, double b,
-                   double gamma, uint64_t seed, int step, double* grad, double* gab) {
+                   double gamma, uint64_t seed, int step, double clip, double* grad, double* gab) {
     const LossIn L = loss_in(n, dim, emb, nnz, heads, tails, keys, coef, nneg, m, a, b, gamma, seed, step);
     std::fill(grad, grad + n * dim, 0.0);
     gab[0] = gab[1] = 0;
@@ -1013,7 +1017,8 @@ void orc_loss_grad(int64_t n, int dim, const double* emb, int64_t nnz, const int
         }
         if (clamped) return;
         for (int d = 0; d < dim; ++d) {
-            const double g = 2.0 * dlds * (emb[i * dim + d] - emb[j * dim + d]);
+            double g = 2.0 * dlds * (emb[i * dim + d] - emb[j * dim + d]);
+            if (clip > 0) g = std::min(std::max(g, -clip), clip);
             grad[i * dim + d] += g;
             grad[j * dim + d] -= g;
         }

@@ -75,6 +75,50 @@ def test_loss_and_gradient_match_oracle(torch_dev, dim):
     assert abs(float(b.grad) - gab_ref[1]) <= 1e-9 * abs(gab_ref[1])
 
 
+def test_gradient_with_shuffled_edges(torch_dev):
+    """Edges in random order (equal heads in separate runs of a wavefront):
+    same gradient as the oracle."""
+    torch, dev = torch_dev
+    from umappp_amd.autograd import umap_loss
+
+    emb, heads, tails, coef, nneg, keys = _random_problem(31, n=200, deg=12)
+    perm = np.random.default_rng(1).permutation(heads.size)
+    heads, tails, coef, nneg, keys = heads[perm], tails[perm], coef[perm], nneg[perm], keys[perm]
+    heads[::7] = heads[0]  # a heavy head spread across many wavefronts
+    ok = heads != tails
+    heads, tails, coef, nneg, keys = heads[ok], tails[ok], coef[ok], nneg[ok], keys[ok]
+    y, h, t, c, nn, k = _to(torch, dev, emb, heads, tails, coef, nneg, keys)
+    y.requires_grad_(True)
+    umap_loss(y, A, B, h, t, keys=k, coef=c, num_neg=nn, seed=8).backward()
+    P = oracle.LossProblem(emb.astype(np.float64), heads, tails, A, B, 1.0, seed=8, keys=keys,
+                           coef=coef.astype(np.float64), num_neg=nneg)
+    g_ref, _ = P.grad()
+    g = y.grad.cpu().numpy().astype(np.float64)
+    scale = np.max(np.abs(g_ref))
+    assert np.all(np.abs(g - g_ref) <= 1e-6 * np.abs(g_ref) + 1e-7 * scale)
+
+
+def test_clipped_gradient_matches_oracle(torch_dev):
+    """Per-term clipping (clip = 4) on a steep curve with close points, where
+    the clip engages: equal to the oracle's clipped gradient."""
+    torch, dev = torch_dev
+    from umappp_amd.autograd import umap_loss
+
+    emb, heads, tails, coef, nneg, keys = _random_problem(21, n=300, dim=2)
+    emb = (emb * 0.05).astype(np.float32)  # crowd the points together
+    y, h, t, c, nn, k = _to(torch, dev, emb, heads, tails, coef, nneg, keys)
+    y.requires_grad_(True)
+    umap_loss(y, 2.0, 0.3, h, t, keys=k, coef=c, num_neg=nn, seed=3, clip=4.0).backward()
+    P = oracle.LossProblem(emb.astype(np.float64), heads, tails, 2.0, 0.3, 1.0, seed=3, keys=keys,
+                           coef=coef.astype(np.float64), num_neg=nneg)
+    g_ref, _ = P.grad(clip=4.0)
+    g_free, _ = P.grad()
+    assert not np.allclose(g_ref, g_free)  # the clip engaged
+    g = y.grad.cpu().numpy().astype(np.float64)
+    scale = np.max(np.abs(g_ref))
+    assert np.all(np.abs(g - g_ref) <= 1e-6 * np.abs(g_ref) + 1e-7 * scale)
+
+
 def test_ab_gradient_matches_finite_differences(torch_dev):
     torch, dev = torch_dev
     from umappp_amd.autograd import umap_loss

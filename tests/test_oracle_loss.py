@@ -131,3 +131,25 @@ def test_negative_samples_are_counter_draws():
             s = max(np.sum((P.emb[i] - P.emb[k]) ** 2), np.finfo(np.float32).eps)
             tot += GAMMA * (np.log1p(A * s ** B) - np.log(A) - B * np.log(s))
     assert abs(tot - P.loss()) <= 1e-10 * abs(tot)
+
+
+def test_clipped_gradient_is_reference_clipped_step():
+    """With clip = 4, minus an attractive term's clipped head gradient is the
+    reference's clipped step at alpha = 1, also where the clip engages; a
+    clip above every term's size changes nothing."""
+    rng = np.random.default_rng(8)
+    engaged = 0
+    a, b = 2.0, 0.3  # steep near 0: the clip engages for close points
+    for _ in range(50):
+        yi = rng.uniform(-1, 1, size=2)
+        yj = yi + rng.normal(scale=rng.choice([1e-3, 0.1, 3.0, 30.0]), size=2)
+        P = oracle.LossProblem(np.array([yi, yj]), [0], [1], a, b, GAMMA, m=0)
+        g, _ = P.grad(clip=4.0)
+        step = oracle.step_delta(0, yi, yj, a, b, GAMMA, 1.0, np.float64)
+        assert np.allclose(-g[0], step, rtol=1e-12, atol=1e-15)
+        engaged += int(np.any(np.abs(step) == 4.0))
+    assert engaged > 0
+    P = _problem(seed=6)
+    g0, gab0 = P.grad()
+    g1, gab1 = P.grad(clip=1e30)
+    assert np.array_equal(g0, g1) and np.array_equal(gab0, gab1)

@@ -61,15 +61,19 @@ def _scalar(x, dev):
     return torch.tensor(float(x), dtype=torch.float64, device=dev)
 
 
-def umap_loss(emb, a, b, heads, tails, *, keys=None, coef=None, num_neg=None, m=5, gamma=1.0, seed=0, step=0):
+def umap_loss(emb, a, b, heads, tails, *, keys=None, coef=None, num_neg=None, m=5, gamma=1.0, seed=0, step=0,
+              clip=0.0):
     """UMAP cross-entropy of ``emb`` over the edge list (see the module docstring).
 
     ``emb``: float32 (n, dim) GPU tensor; ``a``, ``b``: floats or float64
     scalar tensors (gradients flow to tensors that require them);
     ``heads``/``tails``: int32 GPU tensors; ``keys`` (int64, sampler key per
     edge; default the edge's position), ``coef`` (float32) and ``num_neg``
-    (int32, negatives per edge; default ``m``) are optional.  Returns a
-    float64 scalar tensor.
+    (int32, negatives per edge; default ``m``) are optional.  ``clip > 0``
+    makes the backward pass clip every term's gradient with respect to its
+    head to ``[-clip, clip]`` per coordinate before summing (per-sample
+    clipping; 4 is the reference's clip); ``clip = 0`` gives the exact
+    gradient.  Returns a float64 scalar tensor.
     """
     import torch
 
@@ -81,7 +85,7 @@ def umap_loss(emb, a, b, heads, tails, *, keys=None, coef=None, num_neg=None, m=
         keys if keys is not None else empty(torch.int64),
         coef if coef is not None else empty(torch.float32),
         num_neg if num_neg is not None else empty(torch.int32),
-        int(m), float(gamma), int(seed) & (2**63 - 1), int(step), int(stream))
+        int(m), float(gamma), int(seed) & (2**63 - 1), int(step), float(clip), int(stream))
 
 
 def edge_heads(row_ptr):
@@ -104,8 +108,10 @@ class AutogradLayout:
     epoch's loss is :func:`umap_loss` over the firing edges with those sample
     counts (keyed by the edge's schedule index and the epoch, so the samples
     are the plain layout kernel's), and the embedding moves by
-    ``alpha_n * clamp(-grad, -4, 4)`` with the reference's learning-rate decay
-    (:131-133) and clip value (:105-110).
+    ``-alpha_n * grad`` with the reference's learning-rate decay (:131-133),
+    the gradient's terms clipped one by one at the reference's clip value
+    (:105-110) in the backward pass, so that a step is the reference's epoch
+    of clipped per-edge steps, all taken from the epoch's starting positions.
 
     The loss's repulsive terms move both the head and the sampled point, where
     the reference moves only the head; since every observation is sampled as
@@ -159,10 +165,10 @@ class AutogradLayout:
                 y = emb.detach().requires_grad_(True)
                 loss = umap_loss(y, self.a, self.b, self.heads[idx].contiguous(), self.tails[idx].contiguous(),
                                  keys=self.keys[idx].contiguous(), num_neg=nneg.contiguous(),
-                                 gamma=0.5 * self.gamma, seed=self.seed, step=n)
+                                 gamma=0.5 * self.gamma, seed=self.seed, step=n, clip=4.0)
                 (g,) = torch.autograd.grad(loss, [y])
                 with torch.no_grad():
-                    emb.add_(torch.clamp(-g, -4.0, 4.0), alpha=self._alpha(n))
+                    emb.add_(g, alpha=-self._alpha(n))
                 self.next[idx] += self.eps[idx]
                 self.next_neg[idx] += nneg.to(torch.float32) * epns
             self.current = n + 1

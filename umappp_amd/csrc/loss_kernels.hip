@@ -16,9 +16,17 @@
 // Arithmetic is in double (the embedding is float): the loss is a sum of
 // tens of millions of terms and the gradient of a point often nearly cancels
 // over its terms.  One thread per edge; the head's gradient is accumulated
-// in registers, the tail's and the negative samples' with double atomics
-// into a scratch buffer, which a last kernel scales by the upstream gradient
-// and rounds to float.
+// in registers and summed over the wavefront's lanes with the same head
+// before one atomic per run, the tail's and the negative samples' go out with
+// double atomics into a scratch buffer, which a last kernel scales by the
+// upstream gradient and rounds to float.  An edge's negative samples are
+// gathered in batches ahead of their arithmetic and atomics.
+//
+// Optionally (clip > 0) the backward pass clips each term's gradient with
+// respect to the head to [-clip, clip] per coordinate before summing
+// (per-sample gradient clipping; the other endpoint receives minus the
+// clipped value).  With clip = 4 a gradient step of size alpha is the
+// reference's epoch of clipped per-edge steps taken from one snapshot.
 #include "layout_device.hpp"
 #include "umappp_amd.h"
 #include "umx_internal.hpp"
@@ -102,10 +110,32 @@ __device__ inline bool edge_ok(const LossView& v, int64_t e, int64_t& i, int64_t
     return true;
 }
 
+// Negative samples of one edge fetched per batch: all draws of a batch and
+// their points are loaded before any is used, so the gathers overlap instead
+// of waiting on one another (and on the previous sample's atomics).
+template <int D>
+struct Batch {
+    static constexpr int kCount = D <= 4 ? 8 : 2;
+};
+
+template <int D>
+__device__ inline int gather_batch(const LossView& v, uint64_t ej, uint32_t n32, int64_t i, int p0, int m,
+                                   int64_t (&kk)[Batch<D>::kCount], DVec<D> (&yk)[Batch<D>::kCount]) {
+    int got = 0;
+#pragma unroll
+    for (int u = 0; u < Batch<D>::kCount; ++u) {
+        kk[u] = p0 + u < m ? static_cast<int64_t>(draw_index(ej, static_cast<uint32_t>(p0 + u), n32)) : -1;
+        if (kk[u] >= 0 && kk[u] != i) yk[u] = load_d<D>(v.emb, kk[u], v.dim);
+        got += kk[u] >= 0;
+    }
+    return got;
+}
+
 // D = compile-time bound on the embedding dimension (1, 2, 3, 4 or 16).
 template <int D>
 __global__ void __launch_bounds__(kLossBlock) k_loss_forward(LossView v, double* __restrict__ loss) {
     __shared__ double lds[kLossBlock / kWave];
+    constexpr int PF = Batch<D>::kCount;
     const double a = v.ab[0], b = v.ab[1];
     const double feps = FLT_EPSILON;
     const double log_a = log(a);
@@ -115,21 +145,27 @@ __global__ void __launch_bounds__(kLossBlock) k_loss_forward(LossView v, double*
          e += static_cast<int64_t>(gridDim.x) * kLossBlock) {
         int64_t i, j;
         if (!edge_ok(v, e, i, j)) continue;
+        const int m = v.nneg ? v.nneg[e] : v.m;
+        const uint64_t ej = edge_key(v.ekey, v.keys ? static_cast<uint64_t>(v.keys[e]) : static_cast<uint64_t>(e));
         const DVec<D> yi = load_d<D>(v.emb, i, v.dim);
         const DVec<D> yj = load_d<D>(v.emb, j, v.dim);
+        int64_t kk[PF];
+        DVec<D> yk[PF];
+        gather_batch<D>(v, ej, n32, i, 0, m, kk, yk);
         double s = sqd<D>(yi, yj);
         s = s > feps ? s : feps;
         double t = log1p(a * exp(b * log(s)));
-        const int m = v.nneg ? v.nneg[e] : v.m;
-        const uint64_t ej = edge_key(v.ekey, v.keys ? static_cast<uint64_t>(v.keys[e]) : static_cast<uint64_t>(e));
-        for (int p = 0; p < m; ++p) {
-            const int64_t k = draw_index(ej, static_cast<uint32_t>(p), n32);
-            if (k == i) continue;
-            const DVec<D> yk = load_d<D>(v.emb, k, v.dim);
-            double s2 = sqd<D>(yi, yk);
-            s2 = s2 > feps ? s2 : feps;
-            const double ls = log(s2);
-            t += v.gamma * (log1p(a * exp(b * ls)) - log_a - b * ls);
+        for (int p0 = 0; p0 < m; p0 += PF) {
+            if (p0 > 0) gather_batch<D>(v, ej, n32, i, p0, m, kk, yk);
+#pragma unroll
+            for (int u = 0; u < PF; ++u) {
+                if (kk[u] < 0) break;
+                if (kk[u] == i) continue;
+                double s2 = sqd<D>(yi, yk[u]);
+                s2 = s2 > feps ? s2 : feps;
+                const double ls = log(s2);
+                t += v.gamma * (log1p(a * exp(b * ls)) - log_a - b * ls);
+            }
         }
         acc += (v.coef ? static_cast<double>(v.coef[e]) : 1.0) * t;
     }
@@ -137,24 +173,80 @@ __global__ void __launch_bounds__(kLossBlock) k_loss_forward(LossView v, double*
     if (threadIdx.x == 0 && tot != 0) unsafeAtomicAdd(loss, tot);
 }
 
+// Per-term gradient with respect to the head, clipped to [-clip, clip] per
+// coordinate when clip > 0 (the reference's per-step clip,
+// optimize_layout.hpp:105-110, applied to minus the step).
+__device__ inline double clip_term(double g, double clip) {
+    if (clip > 0) g = g < -clip ? -clip : (g > clip ? clip : g);
+    return g;
+}
+
+// Adds a lane's head gradient to g64 with one atomic per run of equal heads
+// in the wavefront (edges in CSR order share heads across ~k neighbouring
+// lanes, and atomics to one address serialise).  Runs are maximal groups of
+// adjacent lanes with one head; a segmented suffix sum leaves each run's
+// total in its first lane, which adds it.  Equal heads in separate runs
+// simply add separately.
 template <int D>
-__global__ void __launch_bounds__(kLossBlock) k_loss_backward(LossView v, double* __restrict__ g64,
+__device__ inline void add_head_segmented(double* __restrict__ g64, int64_t i, int dim, DVec<D> gi) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t prev = __shfl_up(i, 1, kWave);
+    const bool start = lane == 0 || prev != i;
+    const unsigned long long starts = __ballot(start);
+    const unsigned long long later = lane + 1 < kWave ? (starts >> (lane + 1)) << (lane + 1) : 0ULL;
+    const int run_end = later ? __ffsll(static_cast<long long>(later)) - 1 : kWave;  // first lane of the next run
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        // lane holds the sum over [lane, min(lane + o, run_end)); lane + o's
+        // partial covers the next o lanes of the same run
+        const bool take = lane + o < run_end;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const double x = __shfl_down(gi.c[d], o, kWave);
+            if (take) gi.c[d] += x;
+        }
+    }
+    if (i >= 0 && start) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            if (d >= dim) break;
+            unsafeAtomicAdd(g64 + i * dim + d, gi.c[d]);
+        }
+    }
+}
+
+template <int D>
+__global__ void __launch_bounds__(kLossBlock) k_loss_backward(LossView v, double clip, double* __restrict__ g64,
                                                             double* __restrict__ gab) {
     __shared__ double lds[kLossBlock / kWave];
+    constexpr int PF = Batch<D>::kCount;
     const double a = v.ab[0], b = v.ab[1];
     const double feps = FLT_EPSILON;
     const uint32_t n32 = static_cast<uint32_t>(v.n);
     const int dim = v.dim;
     double da = 0, db = 0;
-    for (int64_t e = static_cast<int64_t>(blockIdx.x) * kLossBlock + threadIdx.x; e < v.nnz;
-         e += static_cast<int64_t>(gridDim.x) * kLossBlock) {
-        int64_t i, j;
-        if (!edge_ok(v, e, i, j)) continue;
+    // every lane of a wavefront runs the same number of trips (the head
+    // reduction below is a wavefront collective); lanes past the end or with
+    // a bad edge take part with head -1 and a zero gradient
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kLossBlock;
+    const int64_t first = static_cast<int64_t>(blockIdx.x) * kLossBlock + (threadIdx.x & ~(kWave - 1));
+    for (int64_t e0 = first; e0 < v.nnz; e0 += stride) {
+        const int64_t e = e0 + (threadIdx.x & (kWave - 1));
+        int64_t i = -1, j = -1;
+        if (e >= v.nnz || !edge_ok(v, e, i, j)) {
+            add_head_segmented<D>(g64, -1, dim, DVec<D>{});
+            continue;
+        }
         const double c = v.coef ? static_cast<double>(v.coef[e]) : 1.0;
+        const int m = v.nneg ? v.nneg[e] : v.m;
+        const uint64_t ej = edge_key(v.ekey, v.keys ? static_cast<uint64_t>(v.keys[e]) : static_cast<uint64_t>(e));
         const DVec<D> yi = load_d<D>(v.emb, i, dim);
+        const DVec<D> yj = load_d<D>(v.emb, j, dim);
+        int64_t kk[PF];
+        DVec<D> yk[PF];
+        gather_batch<D>(v, ej, n32, i, 0, m, kk, yk);
         DVec<D> gi{};
         {
-            const DVec<D> yj = load_d<D>(v.emb, j, dim);
             double s = sqd<D>(yi, yj);
             const bool clamped = !(s > feps);
             if (clamped) s = feps;
@@ -166,40 +258,37 @@ __global__ void __launch_bounds__(kLossBlock) k_loss_backward(LossView v, double
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
                     if (d >= dim) break;
-                    const double g = f * (yi.c[d] - yj.c[d]);
+                    const double g = clip_term(f * (yi.c[d] - yj.c[d]), clip);
                     gi.c[d] += g;
                     unsafeAtomicAdd(g64 + j * dim + d, -g);
                 }
             }
         }
-        const int m = v.nneg ? v.nneg[e] : v.m;
-        const uint64_t ej = edge_key(v.ekey, v.keys ? static_cast<uint64_t>(v.keys[e]) : static_cast<uint64_t>(e));
-        for (int p = 0; p < m; ++p) {
-            const int64_t k = draw_index(ej, static_cast<uint32_t>(p), n32);
-            if (k == i) continue;
-            const DVec<D> yk = load_d<D>(v.emb, k, dim);
-            double s = sqd<D>(yi, yk);
-            const bool clamped = !(s > feps);
-            if (clamped) s = feps;
-            const double ls = log(s), sb = exp(b * ls), den = 1.0 + a * sb;
-            da += c * v.gamma * (sb / den - 1.0 / a);
-            db += -c * v.gamma * ls / den;
-            if (!clamped) {
-                const double f = -2.0 * c * v.gamma * b / (s * den);
+        for (int p0 = 0; p0 < m; p0 += PF) {
+            if (p0 > 0) gather_batch<D>(v, ej, n32, i, p0, m, kk, yk);
 #pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    if (d >= dim) break;
-                    const double g = f * (yi.c[d] - yk.c[d]);
-                    gi.c[d] += g;
-                    unsafeAtomicAdd(g64 + k * dim + d, -g);
+            for (int u = 0; u < PF; ++u) {
+                if (kk[u] < 0) break;
+                if (kk[u] == i) continue;
+                double s = sqd<D>(yi, yk[u]);
+                const bool clamped = !(s > feps);
+                if (clamped) s = feps;
+                const double ls = log(s), sb = exp(b * ls), den = 1.0 + a * sb;
+                da += c * v.gamma * (sb / den - 1.0 / a);
+                db += -c * v.gamma * ls / den;
+                if (!clamped) {
+                    const double f = -2.0 * c * v.gamma * b / (s * den);
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        if (d >= dim) break;
+                        const double g = clip_term(f * (yi.c[d] - yk[u].c[d]), clip);
+                        gi.c[d] += g;
+                        unsafeAtomicAdd(g64 + kk[u] * dim + d, -g);
+                    }
                 }
             }
         }
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            if (d >= dim) break;
-            unsafeAtomicAdd(g64 + i * dim + d, gi.c[d]);
-        }
+        add_head_segmented<D>(g64, i, dim, gi);
     }
     const double ta = block_sum(da, lds);
     const double tb = block_sum(db, lds);
@@ -321,9 +410,11 @@ void loss_forward(const LossArgs& x, double* loss, hipStream_t s) {
     bad.check();
 }
 
-void loss_backward(const LossArgs& x, const double* grad_out, float* grad_emb, double* grad_ab, hipStream_t s) {
+void loss_backward(const LossArgs& x, double clip, const double* grad_out, float* grad_emb, double* grad_ab,
+                   hipStream_t s) {
     check_args(x);
     if (grad_ab == nullptr || (x.n > 0 && grad_emb == nullptr)) throw Error(UMX_ERR_ARGUMENT, "output is NULL");
+    if (!(clip >= 0)) throw Error(UMX_ERR_ARGUMENT, "clip must be >= 0");
     BadFlag bad(s);
     const int64_t count = x.n * x.dim;
     double* g64 = nullptr;
@@ -334,7 +425,7 @@ void loss_backward(const LossArgs& x, const double* grad_out, float* grad_emb, d
     const LossView v = view_of(x, bad.p);
     if (x.nnz > 0)
         dispatch_dim(x.dim, [&](auto Dc) {
-            k_loss_backward<decltype(Dc)::value><<<loss_grid(x.nnz), kLossBlock, 0, s>>>(v, g64, grad_ab);
+            k_loss_backward<decltype(Dc)::value><<<loss_grid(x.nnz), kLossBlock, 0, s>>>(v, clip, g64, grad_ab);
         });
     k_loss_finish<<<count > 0 ? zg : 1, kLossBlock, 0, s>>>(count, g64, grad_out, grad_emb, grad_ab);
     const hipError_t e = hipGetLastError();
@@ -359,11 +450,11 @@ int umx_loss_forward(int64_t n, int num_dim, const float* embedding, int64_t nnz
 
 int umx_loss_backward(int64_t n, int num_dim, const float* embedding, int64_t nnz, const int32_t* heads,
                       const int32_t* tails, const int64_t* keys, const float* coef, const int32_t* num_neg, int m,
-                      const double* ab, double gamma, uint64_t seed, int step, const double* grad_out,
+                      const double* ab, double gamma, uint64_t seed, int step, double clip, const double* grad_out,
                       float* grad_embedding, double* grad_ab, void* stream) {
     return umx::guarded([&] {
         const umx::LossArgs x{n, num_dim, embedding, nnz, heads, tails, keys, coef, num_neg, m, ab, gamma, seed, step};
-        umx::loss_backward(x, grad_out, grad_embedding, grad_ab, static_cast<hipStream_t>(stream));
+        umx::loss_backward(x, clip, grad_out, grad_embedding, grad_ab, static_cast<hipStream_t>(stream));
     });
 }
 

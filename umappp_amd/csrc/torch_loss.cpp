@@ -44,7 +44,7 @@ public:
     // int32 (nnz) or empty.
     static Tensor forward(AutogradContext* ctx, Tensor emb, Tensor a, Tensor b, Tensor heads, Tensor tails,
                           Tensor keys, Tensor coef, Tensor num_neg, int64_t m, double gamma, int64_t seed,
-                          int64_t step, int64_t stream) {
+                          int64_t step, double clip, int64_t stream) {
         check_input(emb, torch::kFloat32, "embedding");
         TORCH_CHECK(emb.dim() == 2, "embedding must be (n, dim)");
         check_input(heads, torch::kInt32, "heads");
@@ -65,6 +65,7 @@ public:
         ctx->saved_data["gamma"] = gamma;
         ctx->saved_data["seed"] = seed;
         ctx->saved_data["step"] = step;
+        ctx->saved_data["clip"] = clip;
         ctx->saved_data["stream"] = stream;
         return loss;
     }
@@ -82,17 +83,19 @@ public:
                                 static_cast<int>(ctx->saved_data["m"].toInt()), ab.data_ptr<double>(),
                                 ctx->saved_data["gamma"].toDouble(),
                                 static_cast<uint64_t>(ctx->saved_data["seed"].toInt()),
-                                static_cast<int>(ctx->saved_data["step"].toInt()), go.data_ptr<double>(),
+                                static_cast<int>(ctx->saved_data["step"].toInt()),
+                                ctx->saved_data["clip"].toDouble(), go.data_ptr<double>(),
                                 grad_emb.data_ptr<float>(), grad_ab.data_ptr<double>(),
                                 reinterpret_cast<void*>(ctx->saved_data["stream"].toInt())));
         return {grad_emb, grad_ab[0].to(a.options()).reshape(a.sizes()), grad_ab[1].to(b.options()).reshape(b.sizes()),
-                Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor()};
+                Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor(),
+                Tensor()};
     }
 };
 
 Tensor umap_loss(Tensor emb, Tensor a, Tensor b, Tensor heads, Tensor tails, Tensor keys, Tensor coef,
-                 Tensor num_neg, int64_t m, double gamma, int64_t seed, int64_t step, int64_t stream) {
-    return UmapLoss::apply(emb, a, b, heads, tails, keys, coef, num_neg, m, gamma, seed, step, stream);
+                 Tensor num_neg, int64_t m, double gamma, int64_t seed, int64_t step, double clip, int64_t stream) {
+    return UmapLoss::apply(emb, a, b, heads, tails, keys, coef, num_neg, m, gamma, seed, step, clip, stream);
 }
 
 }  // namespace
