@@ -236,7 +236,7 @@ def test_c5_gradient_vs_fp64_finite_differences(torch_dev, umx):
 
 def test_autograd_layout_quality(torch_dev, umx):
     """10,000-point mixture: 200 epochs driven through the autograd loss keep
-    the plain layout's 15-NN preservation and trustworthiness within 0.02."""
+    the plain layout's 15-NN preservation and trustworthiness within 0.01."""
     torch, dev = torch_dev
     from _quality import knn_preservation, trustworthiness
     from umappp_amd.autograd import AutogradLayout
@@ -267,4 +267,4 @@ def test_autograd_layout_quality(torch_dev, umx):
     ta = trustworthiness(x, auto, 15)
     print(f"autograd layout vs plain: preservation {qa:.4f} vs {qp:.4f}, trustworthiness {ta:.4f} vs {tp:.4f}")
     assert np.all(np.isfinite(auto))
-    assert abs(qa - qp) <= 0.02 and abs(ta - tp) <= 0.02
+    assert abs(qa - qp) <= 0.01 and abs(ta - tp) <= 0.01
