@@ -1,6 +1,7 @@
 // Drop-in for umappp/initialize.hpp (reference :27-119): similarities, fuzzy
-// union and schedule are built on the GPU; the initial coordinates, (a, b)
-// and the epoch count are host steps as in the reference.
+// union, spectral initialisation and schedule are built on the GPU; random
+// initial coordinates, (a, b) and the epoch count are host steps as in the
+// reference.
 #ifndef UMAPPP_AMD_INITIALIZE_HPP
 #define UMAPPP_AMD_INITIALIZE_HPP
 
@@ -57,8 +58,13 @@ Status<Index_, Float_> initialize(NeighborList<Index_, Float_> x, const std::siz
 
     bool use_random = options.initialize_method == InitializeMethod::RANDOM;
     if (options.initialize_method == InitializeMethod::SPECTRAL) {
-        throw std::runtime_error("spectral initialisation is not available in this build; "
-                                 "use InitializeMethod::RANDOM or InitializeMethod::NONE");
+        // GPU spectral_init on the symmetrised graph (reference initialize.hpp:79-93)
+        int ok = 0;
+        detail::check(umx_graph_spectral_init(g, static_cast<int>(num_dim), options.initialize_spectral_scale,
+                                              options.initialize_spectral_jitter ? 1 : 0,
+                                              options.initialize_spectral_jitter_sd, options.initialize_seed,
+                                              embedding, 0, &ok, nullptr));
+        use_random = options.initialize_random_on_spectral_fail && !ok;
     }
     if (use_random) random_init<Index_, Float_>(static_cast<Index_>(n), num_dim, embedding, options.initialize_seed,
                                                 options.initialize_random_scale);

@@ -104,6 +104,41 @@ int umx_graph_export(const umx_graph* g, int64_t* row_ptr, int32_t* idx, void* v
 int umx_graph_export_rho_sigma(const umx_graph* g, double* rho, double* sigma);
 void umx_graph_destroy(umx_graph* g);
 
+/* Exact k-nearest neighbours by brute force on the GPU: the producer of the
+ * NeighborList that initialize() takes (the reference gets it from knncolle,
+ * initialize.hpp:138-177).  data: n observations of 'dim' values each
+ * (point-major, the reference's "data_dim x num_obs" column-major layout),
+ * dtype UMX_F32 or UMX_F64.  Writes idx_out (int32[n * k]) and dist_out
+ * (Euclidean distances, the data's dtype) row by row in increasing distance,
+ * the observation itself excluded, ties to the smaller index; squared
+ * distances are summed in double in increasing dimension order.  1 <= k <=
+ * 64, k < n.  device_ptrs = 1 if all three pointers are GPU memory. */
+int umx_knn_bruteforce(int dtype, int64_t n, int dim, const void* data, int k, int32_t* idx_out, void* dist_out,
+                       int device_ptrs);
+/* A resident kNN graph straight from data (umx_knn_bruteforce without the
+ * round trip through host memory); ready for umx_graph_neighbor_similarities.
+ * Replaces initialize()'s data + knncolle::Builder overload
+ * (initialize.hpp:165-177) together with the calls that follow. */
+int umx_graph_create_knn(int dtype, int64_t n, int dim, const void* data, int k, int device_ptrs, umx_graph** out);
+
+/* Connected components of the (symmetrised) resident graph; the reference's
+ * has_multiple_components (spectral_init.hpp:142-168) is *count != 1. */
+int umx_graph_components(const umx_graph* g, int64_t* count);
+/* spectral_init (spectral_init.hpp:170-207) on the resident symmetrised
+ * graph: the eigenvectors of the normalized Laplacian for its num_dim
+ * smallest positive eigenvalues (Lanczos in double precision on 2I - L),
+ * scaled so that the largest |coordinate| is 'scale', each vector's sign
+ * fixed by making its largest-magnitude entry positive, plus N(0, jitter_sd)
+ * noise when 'jitter' is set (a counter-based generator keyed by 'seed';
+ * the reference's aarand normals are not reproduced).  Writes 'embedding'
+ * (n * num_dim, point-major, the graph's dtype; device_ptr = 1 for GPU
+ * memory) and sets *success = 1; *success = 0 leaves it untouched, as the
+ * reference does when the graph has several components or the eigensolver
+ * does not converge.  eigenvalues (may be NULL) receives the num_dim + 1
+ * largest eigenvalues of 2I - L found. */
+int umx_graph_spectral_init(const umx_graph* g, int num_dim, double scale, int jitter, double jitter_sd,
+                            uint64_t seed, void* embedding, int device_ptr, int* success, double* eigenvalues);
+
 /* Batched graph (no counterpart in the reference, which runs one problem per
  * initialize() call): n_inst independent kNN graphs stored one after the
  * other.  Instance q owns observations [inst_offsets[q], inst_offsets[q+1])
@@ -289,6 +324,11 @@ int umx_loss_backward(int64_t n, int num_dim, const float* embedding, int64_t nn
 int umx_find_ab(double spread, double min_dist, double* a, double* b);
 /* choose_num_epochs (initialize.hpp:27-47); num_epochs < 0 means unset. */
 int umx_choose_num_epochs(int num_epochs, int64_t size);
+/* Symmetric tridiagonal eigen-decomposition used by umx_graph_spectral_init
+ * (host code): diagonal d[k] (overwritten with the eigenvalues, unsorted),
+ * off-diagonal e[k - 1], eigenvectors into the columns of z (row-major
+ * k x k).  UMX_ERR_STATE if the QL iteration does not converge. */
+int umx_tridiagonal_eigen(int k, double* d, const double* e, double* z);
 
 #ifdef __cplusplus
 }

@@ -299,3 +299,54 @@ class LossProblem:
         out = np.zeros(pts.size)
         lib().orc_loss_fd(*self._args(), pts.size, _p(pts), _p(ds), float(h), _p(out))
         return out
+
+
+# ---------------------------------------------------------------------------
+# Spectral initialisation (dense restatement for small n; numpy)
+# ---------------------------------------------------------------------------
+def has_multiple_components(row_ptr, idx):
+    """spectral_init.hpp:142-168: depth-first walk from observation 0 over the
+    symmetric graph; True if it does not reach every observation."""
+    rp = np.asarray(row_ptr, dtype=np.int64)
+    n = rp.size - 1
+    if n == 0:
+        return False
+    seen = np.zeros(n, dtype=bool)
+    seen[0] = True
+    stack, count = [0], 1
+    while stack:
+        c = stack.pop()
+        for t in idx[rp[c]:rp[c + 1]]:
+            if not seen[t]:
+                seen[t] = True
+                count += 1
+                stack.append(int(t))
+    return count != n
+
+
+def spectral_init_dense(row_ptr, idx, val, num_dim, scale=10.0):
+    """spectral_init.hpp:24-140 restated densely: M = 2I - L = I + D^-1/2 W D^-1/2
+    (degrees in double, :34-49), the eigenvectors of its num_dim + 1 largest
+    eigenvalues without the first (:122-126), scaled so that the largest
+    magnitude over all of them is 'scale' (:128-137).  The reference's signs
+    come from irlba; here each vector's largest-magnitude entry is made
+    positive.  Returns (ok, Y (n, num_dim), eigenvalues of M descending)."""
+    rp = np.asarray(row_ptr, dtype=np.int64)
+    n = rp.size - 1
+    if has_multiple_components(rp, idx) or n <= num_dim + 1:
+        return False, None, None
+    W = np.zeros((n, n))
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    W[rows, np.asarray(idx)] = np.asarray(val, dtype=np.float64)
+    isd = 1.0 / np.sqrt(W.sum(axis=1))
+    M = np.eye(n) + isd[:, None] * W * isd[None, :]
+    w, V = np.linalg.eigh(M)
+    order = np.argsort(-w)
+    w, V = w[order], V[:, order]
+    Y = V[:, 1:num_dim + 1].copy()
+    for c in range(num_dim):
+        j = np.argmax(np.abs(Y[:, c]))
+        if Y[j, c] < 0:
+            Y[:, c] = -Y[:, c]
+    Y *= scale / np.max(np.abs(Y))
+    return True, Y, w[:num_dim + 1]

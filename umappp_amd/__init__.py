@@ -28,7 +28,7 @@ import numpy as np
 __all__ = [
     "LibraryNotBuilt", "UmxError", "NeighborList", "NeighborSimilaritiesOptions", "EpochData", "Options",
     "InitializeMethod", "Status", "neighbor_similarities", "combine_neighbor_sets", "similarities_to_epochs",
-    "initialize", "find_ab", "choose_num_epochs", "Graph", "Layout", "library_path", "lib",
+    "initialize", "find_ab", "choose_num_epochs", "Graph", "Layout", "library_path", "lib", "find_knn",
 ]
 
 _HERE = Path(__file__).resolve().parent
@@ -95,6 +95,12 @@ def _declare(L):
         "umx_layout_set_timing": (C.c_int, [_vp, C.c_int]),
         "umx_layout_timing": (C.c_int, [_vp, _f64p, _i64p]),
         "umx_layout_destroy": (None, [_vp]),
+        "umx_knn_bruteforce": (C.c_int, [C.c_int, C.c_int64, C.c_int, _vp, C.c_int, _vp, _vp, C.c_int]),
+        "umx_graph_create_knn": (C.c_int, [C.c_int, C.c_int64, C.c_int, _vp, C.c_int, C.c_int, C.POINTER(_vp)]),
+        "umx_graph_components": (C.c_int, [_vp, _i64p]),
+        "umx_graph_spectral_init": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, C.c_double, C.c_uint64, _vp,
+                                              C.c_int, C.POINTER(C.c_int), _vp]),
+        "umx_tridiagonal_eigen": (C.c_int, [C.c_int, _vp, _vp, _vp]),
         "umx_graph_create_batch": (C.c_int, [C.c_int, C.c_int32, _vp, _vp, _vp, _vp, C.c_int, C.POINTER(_vp)]),
         "umx_graph_instances": (C.c_int, [_vp, _i32p, _vp]),
         "umx_layout_create_batch": (C.c_int, [_vp, _vp, C.c_double, C.POINTER(_vp)]),
@@ -246,6 +252,21 @@ class Graph:
             self._h = None
 
     @classmethod
+    def from_data(cls, data: np.ndarray, k: int) -> "Graph":
+        """Exact kNN graph of ``data`` (n x dim, float32 or float64) computed on
+        the GPU (umx_graph_create_knn); distances in the data's type."""
+        x = np.ascontiguousarray(data)
+        if x.dtype not in (np.float32, np.float64):
+            x = x.astype(np.float64)
+        n, dim = x.shape
+        g = cls.__new__(cls)
+        h = C.c_void_p()
+        _check(lib().umx_graph_create_knn(_dtype_code(x.dtype), n, dim, _ptr(x), int(k), 0, C.byref(h)))
+        g._h = h
+        g.dtype = x.dtype.type
+        return g
+
+    @classmethod
     def batch(cls, lists: Sequence[NeighborList]) -> "Graph":
         """Batched graph of independent kNN graphs (all of one float type)."""
         sizes = [x.size() for x in lists]
@@ -291,6 +312,25 @@ class Graph:
         val = np.empty(nnz, dtype=self.dtype)
         _check(lib().umx_graph_export(self._h, _ptr(rp), _ptr(idx), _ptr(val)))
         return NeighborList(rp, idx, val)
+
+    def components(self) -> int:
+        """Number of connected components (has_multiple_components is != 1)."""
+        c = C.c_int64()
+        _check(lib().umx_graph_components(self._h, C.byref(c)))
+        return c.value
+
+    def spectral_init(self, num_dim, scale=10.0, jitter=False, jitter_sd=1e-4, seed=1234567890, out=None):
+        """spectral_init (spectral_init.hpp:170-207) on the symmetrised graph.
+
+        Returns (success, embedding (n, num_dim) or None, eigenvalues of 2I - L).
+        With ``out`` given, it is filled in place on success."""
+        n, _ = self.info()
+        emb = out if out is not None else np.empty((n, int(num_dim)), dtype=self.dtype)
+        ok = C.c_int()
+        ev = np.full(int(num_dim) + 1, np.nan)
+        _check(lib().umx_graph_spectral_init(self._h, int(num_dim), float(scale), 1 if jitter else 0, float(jitter_sd),
+                                             C.c_uint64(int(seed) & (2**64 - 1)), _ptr(emb), 0, C.byref(ok), _ptr(ev)))
+        return bool(ok.value), (emb if ok.value else None), ev
 
     def rho_sigma(self):
         n, _ = self.info()
@@ -486,6 +526,20 @@ class NeighborSimilaritiesOptions:
     num_threads: int = 1
 
 
+def find_knn(data: np.ndarray, k: int):
+    """Exact k nearest neighbours (self excluded, ties to the smaller index) of
+    every row of ``data`` on the GPU: (indices int32 (n, k), distances (n, k)
+    in the data's float type), each row by increasing distance."""
+    x = np.ascontiguousarray(data)
+    if x.dtype not in (np.float32, np.float64):
+        x = x.astype(np.float64)
+    n, dim = x.shape
+    idx = np.empty((n, int(k)), dtype=np.int32)
+    dist = np.empty((n, int(k)), dtype=x.dtype)
+    _check(lib().umx_knn_bruteforce(_dtype_code(x.dtype), n, dim, _ptr(x), int(k), _ptr(idx), _ptr(dist), 0))
+    return idx, dist
+
+
 def neighbor_similarities(x: NeighborList, options: NeighborSimilaritiesOptions = None, use_newton: bool = True,
                           return_rho_sigma: bool = False):
     """In-place distances -> fuzzy weights (neighbor_similarities.hpp:53)."""
@@ -611,8 +665,10 @@ class Status:
 
 def initialize(x: NeighborList, num_dim: int, embedding: np.ndarray, options: Options = None) -> Status:
     """initialize.hpp:69-119 on the GPU: similarities, union, initial
-    coordinates, (a, b), epoch count and schedule.  SPECTRAL is not available
-    yet; RANDOM fills uniform(-scale, scale) draws; NONE keeps ``embedding``."""
+    coordinates, (a, b), epoch count and schedule.  SPECTRAL runs the GPU
+    spectral initialisation (falling back to RANDOM when it fails and
+    ``initialize_random_on_spectral_fail`` is set, initialize.hpp:79-93);
+    RANDOM fills uniform(-scale, scale) draws; NONE keeps ``embedding``."""
     import copy
 
     opt = copy.deepcopy(options) if options is not None else Options()
@@ -620,9 +676,16 @@ def initialize(x: NeighborList, num_dim: int, embedding: np.ndarray, options: Op
     g.neighbor_similarities(opt.local_connectivity, opt.bandwidth, 1e-3, True)
     g.combine(opt.mix_ratio)
     method = opt.initialize_method
+    use_random = method == InitializeMethod.RANDOM
     if method == InitializeMethod.SPECTRAL:
-        raise NotImplementedError("spectral initialisation is not implemented; use RANDOM or NONE")
-    if method == InitializeMethod.RANDOM:
+        flat = embedding.reshape(-1, num_dim)
+        work = np.empty(flat.shape, dtype=g.dtype)
+        ok, _, _ = g.spectral_init(num_dim, opt.initialize_spectral_scale, opt.initialize_spectral_jitter,
+                                   opt.initialize_spectral_jitter_sd, opt.initialize_seed, out=work)
+        if ok:
+            flat[:] = work.astype(embedding.dtype)
+        use_random = opt.initialize_random_on_spectral_fail and not ok
+    if use_random:
         rng = np.random.default_rng(opt.initialize_seed)
         embedding.reshape(-1)[:] = rng.uniform(-opt.initialize_random_scale, opt.initialize_random_scale,
                                                size=embedding.size).astype(embedding.dtype)

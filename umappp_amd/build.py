@@ -21,7 +21,7 @@ LIB = PKG / "libumappp_amd.so"
 TORCH_DIR = PKG / "_torch_build"
 TORCH_NAME = "umappp_amd_torch"
 
-SOURCES = ["graph_kernels.hip", "layout_kernels.hip", "sell_kernels.hip", "loss_kernels.hip", "capi.hip"]
+SOURCES = ["graph_kernels.hip", "layout_kernels.hip", "sell_kernels.hip", "loss_kernels.hip", "spectral.hip", "knn.hip", "capi.hip"]
 HOST_SOURCES = ["host_api.cpp"]
 CXX = os.environ.get("CXX", "g++")
 

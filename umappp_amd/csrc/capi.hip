@@ -67,6 +67,39 @@ size_t dsize(int dtype) { return dtype == UMX_F32 ? sizeof(float) : sizeof(doubl
 
 void umx::set_last_error(const std::string& msg) { g_last_error = msg; }
 
+namespace {
+
+// Keeps the default pool's freed memory for reuse (once per device).
+void keep_pool_memory() {
+    static std::mutex mu;
+    static std::vector<int> done;
+    int dev = 0;
+    UMX_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    if (std::find(done.begin(), done.end(), dev) != done.end()) return;
+    hipMemPool_t pool = nullptr;
+    UMX_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
+    uint64_t keep = std::numeric_limits<uint64_t>::max();
+    UMX_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+    done.push_back(dev);
+}
+
+}  // namespace
+
+void umx::prepare_pool() { keep_pool_memory(); }
+
+void* umx::device_alloc(size_t bytes) {
+    keep_pool_memory();
+    void* p = nullptr;
+    UMX_HIP(hipMallocAsync(&p, bytes, nullptr));
+    UMX_HIP(hipStreamSynchronize(nullptr));
+    return p;
+}
+
+void umx::device_free(void* p) {
+    if (p) (void)hipFreeAsync(p, nullptr);
+}
+
 // ---------------------------------------------------------------------------
 // Graph handle
 // ---------------------------------------------------------------------------
@@ -194,6 +227,135 @@ int umx_graph_create_batch(int dtype, int32_t n_inst, const int64_t* inst_offset
             require(umx::localize_to_global(n, g->rp.p, g->idx.p, g->instances(), g->s), UMX_ERR_GRAPH,
                     "neighbour index outside its instance");
         *out = g.release();
+    });
+}
+
+int umx_knn_bruteforce(int dtype, int64_t n, int dim, const void* data, int k, int32_t* idx_out, void* dist_out,
+                       int device_ptrs) {
+    return guard([&] {
+        require(dtype == UMX_F32 || dtype == UMX_F64, UMX_ERR_ARGUMENT, "dtype must be UMX_F32 or UMX_F64");
+        require(n >= 0 && n < std::numeric_limits<int32_t>::max(), UMX_ERR_OVERFLOW, "n must fit a 32-bit index");
+        require(dim >= 1 && k >= 1 && k <= 64, UMX_ERR_ARGUMENT, "need dim >= 1 and 1 <= k <= 64");
+        require(n == 0 || k < n, UMX_ERR_ARGUMENT, "k must be smaller than the number of observations");
+        require(n == 0 || (data && idx_out && dist_out), UMX_ERR_ARGUMENT, "NULL argument");
+        if (n == 0) return;
+        hipStream_t s = make_stream();
+        struct StreamGuard {
+            hipStream_t s;
+            ~StreamGuard() { (void)hipStreamDestroy(s); }
+        } sg{s};
+        const size_t es = dsize(dtype);
+        const size_t xbytes = es * static_cast<size_t>(n) * dim, nk = static_cast<size_t>(n) * k;
+        DevBuf<unsigned char> dx, di, dd;
+        const void* x = data;
+        int32_t* idx = idx_out;
+        void* dist = dist_out;
+        if (!device_ptrs) {
+            dx.alloc(xbytes);
+            di.alloc(nk * sizeof(int32_t));
+            dd.alloc(nk * es);
+            UMX_HIP(hipMemcpyAsync(dx.p, data, xbytes, hipMemcpyHostToDevice, s));
+            x = dx.p;
+            idx = reinterpret_cast<int32_t*>(di.p);
+            dist = dd.p;
+        }
+        if (dtype == UMX_F32)
+            umx::knn_bruteforce<float>(n, dim, static_cast<const float*>(x), k, idx, static_cast<float*>(dist), s);
+        else
+            umx::knn_bruteforce<double>(n, dim, static_cast<const double*>(x), k, idx, static_cast<double*>(dist), s);
+        if (!device_ptrs) {
+            UMX_HIP(hipMemcpyAsync(idx_out, idx, nk * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            UMX_HIP(hipMemcpyAsync(dist_out, dist, nk * es, hipMemcpyDeviceToHost, s));
+        }
+        UMX_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int umx_graph_create_knn(int dtype, int64_t n, int dim, const void* data, int k, int device_ptrs, umx_graph** out) {
+    return guard([&] {
+        require(out != nullptr, UMX_ERR_ARGUMENT, "out is NULL");
+        *out = nullptr;
+        require(dtype == UMX_F32 || dtype == UMX_F64, UMX_ERR_ARGUMENT, "dtype must be UMX_F32 or UMX_F64");
+        require(n >= 2 && n < std::numeric_limits<int32_t>::max(), UMX_ERR_ARGUMENT,
+                "need 2 <= n < 2^31 observations");
+        require(dim >= 1 && k >= 1 && k <= 64 && k < n, UMX_ERR_ARGUMENT, "need dim >= 1 and 1 <= k <= min(64, n - 1)");
+        require(data != nullptr, UMX_ERR_ARGUMENT, "data is NULL");
+        std::unique_ptr<umx_graph> g(new umx_graph());
+        g->dtype = dtype;
+        g->n = n;
+        g->s = make_stream();
+        const size_t es = dsize(dtype);
+        const size_t xbytes = es * static_cast<size_t>(n) * dim;
+        DevBuf<unsigned char> dx;
+        const void* x = data;
+        if (!device_ptrs) {
+            dx.alloc(xbytes);
+            UMX_HIP(hipMemcpyAsync(dx.p, data, xbytes, hipMemcpyHostToDevice, g->s));
+            x = dx.p;
+        }
+        g->nnz = n * k;
+        g->rp.alloc(static_cast<size_t>(n + 1));
+        g->idx.alloc(static_cast<size_t>(g->nnz));
+        std::vector<int64_t> hrp(static_cast<size_t>(n + 1));
+        for (int64_t i = 0; i <= n; ++i) hrp[i] = i * k;
+        UMX_HIP(hipMemcpyAsync(g->rp.p, hrp.data(), sizeof(int64_t) * hrp.size(), hipMemcpyHostToDevice, g->s));
+        if (dtype == UMX_F32) {
+            g->vf.alloc(static_cast<size_t>(g->nnz));
+            umx::knn_bruteforce<float>(n, dim, static_cast<const float*>(x), k, g->idx.p, g->vf.p, g->s);
+        } else {
+            g->vd.alloc(static_cast<size_t>(g->nnz));
+            umx::knn_bruteforce<double>(n, dim, static_cast<const double*>(x), k, g->idx.p, g->vd.p, g->s);
+        }
+        g->max_degree = k;
+        UMX_HIP(hipStreamSynchronize(g->s));
+        const int64_t off[2] = {0, n};
+        graph_set_instances(g.get(), off, 1);
+        *out = g.release();
+    });
+}
+
+int umx_graph_components(const umx_graph* g, int64_t* count) {
+    return guard([&] {
+        require(g != nullptr && count != nullptr, UMX_ERR_ARGUMENT, "NULL argument");
+        *count = umx::count_components(g->n, g->rp.p, g->idx.p, g->s);
+    });
+}
+
+int umx_graph_spectral_init(const umx_graph* g, int num_dim, double scale, int jitter, double jitter_sd,
+                            uint64_t seed, void* embedding, int device_ptr, int* success, double* eigenvalues) {
+    return guard([&] {
+        require(g != nullptr && embedding != nullptr && success != nullptr, UMX_ERR_ARGUMENT, "NULL argument");
+        require(num_dim >= 1, UMX_ERR_ARGUMENT, "num_dim must be >= 1");
+        require(std::isfinite(scale) && std::isfinite(jitter_sd), UMX_ERR_ARGUMENT, "scale and jitter_sd must be finite");
+        *success = 0;
+        umx::SpectralParams prm;
+        prm.scale = scale;
+        prm.jitter = jitter != 0;
+        prm.jitter_sd = jitter_sd;
+        prm.jitter_seed = seed;
+        umx::SpectralReport rep;
+        const size_t count = static_cast<size_t>(g->n) * static_cast<size_t>(num_dim);
+        const size_t bytes = dsize(g->dtype) * count;
+        DevBuf<unsigned char> tmp;
+        void* out = embedding;
+        if (!device_ptr) {
+            tmp.alloc(bytes > 0 ? bytes : 1);
+            out = tmp.p;
+        }
+        bool ok;
+        if (g->dtype == UMX_F32)
+            ok = umx::spectral_embedding<float>(g->n, g->rp.p, g->idx.p, g->vf.p, static_cast<int>(g->max_degree),
+                                                num_dim, prm, static_cast<float*>(out), &rep, g->s);
+        else
+            ok = umx::spectral_embedding<double>(g->n, g->rp.p, g->idx.p, g->vd.p, static_cast<int>(g->max_degree),
+                                                 num_dim, prm, static_cast<double*>(out), &rep, g->s);
+        if (ok && !device_ptr && bytes)
+            UMX_HIP(hipMemcpyAsync(embedding, out, bytes, hipMemcpyDeviceToHost, g->s));
+        UMX_HIP(hipStreamSynchronize(g->s));
+        if (eigenvalues)
+            for (size_t c = 0; c < rep.eigenvalues.size() && c < static_cast<size_t>(num_dim) + 1; ++c)
+                eigenvalues[c] = rep.eigenvalues[c];
+        *success = ok ? 1 : 0;
     });
 }
 

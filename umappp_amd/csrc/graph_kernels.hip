@@ -33,6 +33,7 @@ struct Tmp {
     hipStream_t s;
     Tmp(size_t count, hipStream_t st) : s(st) {
         if (count == 0) count = 1;
+        prepare_pool();
         UMX_HIP(hipMallocAsync(reinterpret_cast<void**>(&p), count * sizeof(T), s));
     }
     ~Tmp() {
@@ -514,11 +515,114 @@ __global__ void __launch_bounds__(kBlock) k_stage_mirrors(int64_t n, int G, cons
     }
 }
 
-// Final rows: kept entries (already in index order) merged with the staged
-// mirrored entries (arbitrary order) by rank, so the output order is exactly
-// ascending index whatever order the staging atomics produced.
+// Sorts each row's staged mirrored entries by index (the staging atomics leave
+// them in arbitrary order; keys are unique within a row, one per source row).
+// Rows with at most kSmallMirrors entries: a group of kSmallMirrors lanes per
+// row ranks its keys with shuffles.  Longer rows are listed for
+// k_sort_mirrors_big.
+constexpr int kSmallMirrors = 32;
+constexpr int kBigSortCap = 8192;  // LDS bitonic sort capacity per row
+
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_merge_rows(int64_t n, int G, const int64_t* __restrict__ rp,
+__global__ void __launch_bounds__(kBlock) k_sort_mirrors_small(int64_t n, const int64_t* __restrict__ inc_rp,
+                                                             int32_t* __restrict__ src, T* __restrict__ val,
+                                                             int64_t* __restrict__ big_rows,
+                                                             unsigned long long* __restrict__ n_big) {
+    constexpr int G = kSmallMirrors;
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t row = gid / G;
+    const int lane = static_cast<int>(gid & (G - 1));
+    // every lane of the wavefront runs the shuffles below; rows past the end
+    // take part with no entries
+    const int64_t ss = row < n ? inc_rp[row] : 0;
+    const int64_t m = row < n ? inc_rp[row + 1] - ss : 0;
+    const bool small = m <= G;
+    if (!small && lane == 0) big_rows[atomicAdd(n_big, 1ULL)] = row;
+    const bool mine = small && lane < m;
+    const int32_t key = mine ? src[ss + lane] : 0x7fffffff;
+    const T v = mine ? val[ss + lane] : static_cast<T>(0);
+    int rank = 0;
+    const int base = (threadIdx.x & (kWave - 1)) & ~(G - 1);
+#pragma unroll
+    for (int o = 0; o < G; ++o) {
+        const int32_t other = __shfl(key, base + o, kWave);
+        rank += other < key;
+    }
+    if (mine) {
+        src[ss + rank] = key;
+        val[ss + rank] = v;
+    }
+}
+
+// One workgroup per long row: bitonic sort of (key, value) in LDS.  Rows
+// beyond kBigSortCap entries are ranked in global memory instead
+// (quadratic, for extreme hubs only).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_sort_mirrors_big(const int64_t* __restrict__ big_rows,
+                                                           const int64_t* __restrict__ inc_rp,
+                                                           int32_t* __restrict__ src, T* __restrict__ val,
+                                                           int32_t* __restrict__ tmp_src, T* __restrict__ tmp_val) {
+    __shared__ int32_t keys[kBigSortCap];
+    __shared__ int32_t pos[kBigSortCap];
+    const int64_t row = big_rows[blockIdx.x];
+    const int64_t ss = inc_rp[row];
+    const int64_t m = inc_rp[row + 1] - ss;
+    if (m <= kBigSortCap) {
+        int p2 = 1;
+        while (p2 < m) p2 <<= 1;
+        for (int q = threadIdx.x; q < p2; q += kBlock) {
+            keys[q] = q < m ? src[ss + q] : 0x7fffffff;
+            pos[q] = q;
+        }
+        __syncthreads();
+        for (int k = 2; k <= p2; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int q = threadIdx.x; q < p2; q += kBlock) {
+                    const int o = q ^ j;
+                    if (o > q) {
+                        const bool up = (q & k) == 0;
+                        const int32_t a = keys[q], b = keys[o];
+                        if ((a > b) == up) {
+                            keys[q] = b;
+                            keys[o] = a;
+                            const int32_t t = pos[q];
+                            pos[q] = pos[o];
+                            pos[o] = t;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // permute values through the scratch copy, then write back
+        for (int q = threadIdx.x; q < m; q += kBlock) tmp_val[ss + q] = val[ss + pos[q]];
+        __syncthreads();
+        for (int q = threadIdx.x; q < m; q += kBlock) {
+            src[ss + q] = keys[q];
+            val[ss + q] = tmp_val[ss + q];
+        }
+    } else {
+        for (int64_t q = threadIdx.x; q < m; q += kBlock) {
+            const int32_t key = src[ss + q];
+            int64_t rank = 0;
+            for (int64_t f = 0; f < m; ++f) rank += src[ss + f] < key;
+            tmp_src[ss + rank] = key;
+            tmp_val[ss + rank] = val[ss + q];
+        }
+        __syncthreads();
+        for (int64_t q = threadIdx.x; q < m; q += kBlock) {
+            src[ss + q] = tmp_src[ss + q];
+            val[ss + q] = tmp_val[ss + q];
+        }
+    }
+}
+
+// Final rows: kept entries (already in index order) merged with the mirrored
+// entries (sorted per row above) by rank, so the output order is exactly
+// ascending index.  With any mix other than 0 every original entry is kept;
+// with mix = 0 there are no mirrors.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_merge_rows(int64_t n, int G, int mode, const int64_t* __restrict__ rp,
                                                      const int32_t* __restrict__ sidx, const T* __restrict__ nval,
                                                      const unsigned char* __restrict__ flags,
                                                      const int64_t* __restrict__ inc_rp,
@@ -535,23 +639,32 @@ __global__ void __launch_bounds__(kBlock) k_merge_rows(int64_t n, int G, const i
     for (int64_t q = s + lane; q < e; q += G) {
         if (!(flags[q] & 1)) continue;
         const int32_t key = sidx[q];
-        int64_t rank = 0;
-        for (int64_t f = s; f < q; ++f) rank += flags[f] & 1;
-        for (int64_t f = ss; f < se; ++f) rank += stage_src[f] < key;
+        int64_t rank;
+        if (mode == MIX_INTERSECT) {
+            rank = 0;
+            for (int64_t f = s; f < q; ++f) rank += flags[f] & 1;
+        } else {
+            rank = q - s;
+            int64_t lo = ss, hi = se;  // mirrors with a smaller index
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if (stage_src[mid] < key) lo = mid + 1;
+                else hi = mid;
+            }
+            rank += lo - ss;
+        }
         oidx[ob + rank] = key;
         oval[ob + rank] = nval[q];
     }
     for (int64_t q = ss + lane; q < se; q += G) {
         const int32_t key = stage_src[q];
-        // every original entry is kept whenever mirrors exist (mix != 0)
-        int64_t lo = s, hi = e;
+        int64_t lo = s, hi = e;  // original entries with a smaller index
         while (lo < hi) {
             const int64_t mid = (lo + hi) >> 1;
             if (sidx[mid] < key) lo = mid + 1;
             else hi = mid;
         }
-        int64_t rank = lo - s;
-        for (int64_t f = ss; f < se; ++f) rank += stage_src[f] < key;
+        const int64_t rank = (lo - s) + (q - ss);
         oidx[ob + rank] = key;
         oval[ob + rank] = stage_val[q];
     }
@@ -626,6 +739,16 @@ __device__ inline RowLimit<T> row_limit(int64_t row, const int32_t* __restrict__
     return RowLimit<T>{maxed, maxed / static_cast<T>(ne)};  // :63
 }
 
+// Bits of the G-lane group of the calling lane in a wavefront ballot.
+__device__ inline unsigned long long group_bits(unsigned long long ballot, int G) {
+    const int base = (threadIdx.x & (kWave - 1)) & ~(G - 1);
+    const unsigned long long mask = G >= kWave ? ~0ULL : ((1ULL << G) - 1ULL);
+    return (ballot >> base) & mask;
+}
+
+// Survivors per row: G lanes per row walk it in chunks of G entries and count
+// with ballots.  Every lane of a group runs the same trips (the trip count
+// depends on the row only), which the ballots need.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_count_survivors(int64_t n, int G, const int64_t* __restrict__ rp,
                                                           const T* __restrict__ val,
@@ -640,8 +763,12 @@ __global__ void __launch_bounds__(kBlock) k_count_survivors(int64_t n, int G, co
     const T limit = row_limit<T>(row, of_row, maxbits, num_epochs, epochs_of_inst).limit;
     const int64_t s = rp[row], e = rp[row + 1];
     int64_t c = 0;
-    for (int64_t q = s + lane; q < e; q += G) c += val[q] >= limit;
-    if (c) atomicAdd(reinterpret_cast<unsigned long long*>(cnt + row), static_cast<unsigned long long>(c));
+    for (int64_t b0 = s; b0 < e; b0 += G) {
+        const int64_t q = b0 + lane;
+        const bool keep = q < e && val[q] >= limit;
+        c += __popcll(group_bits(__ballot(keep), G));
+    }
+    if (lane == 0) cnt[row] = c;
 }
 
 template <typename T>
@@ -659,17 +786,21 @@ __global__ void __launch_bounds__(kBlock) k_fill_epochs(int64_t n, int G, const 
     if (row >= n) return;
     const RowLimit<T> rl = row_limit<T>(row, of_row, maxbits, num_epochs, epochs_of_inst);
     const int64_t s = rp[row], e = rp[row + 1];
-    const int64_t ob = orp[row];
-    for (int64_t q = s + lane; q < e; q += G) {
-        const T w = val[q];
-        if (!(w >= rl.limit)) continue;
-        int64_t rank = 0;
-        for (int64_t f = s; f < q; ++f) rank += val[f] >= rl.limit;
-        const T ep = rl.maxed / w;  // :70
-        tg[ob + rank] = idx[q];
-        eps[ob + rank] = ep;
-        nxt[ob + rank] = ep;  // :77
-        nxt_neg[ob + rank] = ep / nsr;  // :78-81
+    int64_t at = orp[row];
+    for (int64_t b0 = s; b0 < e; b0 += G) {
+        const int64_t q = b0 + lane;
+        const T w = q < e ? val[q] : static_cast<T>(0);
+        const bool keep = q < e && w >= rl.limit;
+        const unsigned long long bits = group_bits(__ballot(keep), G);
+        if (keep) {
+            const int64_t o = at + __popcll(bits & ((1ULL << lane) - 1ULL));
+            const T ep = rl.maxed / w;  // :70
+            tg[o] = idx[q];
+            eps[o] = ep;
+            nxt[o] = ep;  // :77
+            nxt_neg[o] = ep / nsr;  // :78-81
+        }
+        at += __popcll(bits);
     }
 }
 
@@ -841,10 +972,28 @@ void run_combine(int64_t n, const int64_t* rp, const int32_t* idx, const T* val,
                                                    stage_val.p);
         UMX_HIP(hipGetLastError());
     }
+    if (n_inc > 0) {
+        Tmp<int64_t> big_rows(static_cast<size_t>(n), s);
+        Tmp<unsigned long long> n_big(1, s);
+        UMX_HIP(hipMemsetAsync(n_big.p, 0, sizeof(unsigned long long), s));
+        k_sort_mirrors_small<T><<<grid_for(n * kSmallMirrors), kBlock, 0, s>>>(n, inc_rp.p, stage_src.p, stage_val.p,
+                                                                               big_rows.p, n_big.p);
+        UMX_HIP(hipGetLastError());
+        unsigned long long hbig = 0;
+        UMX_HIP(hipMemcpyAsync(&hbig, n_big.p, sizeof(hbig), hipMemcpyDeviceToHost, s));
+        UMX_HIP(hipStreamSynchronize(s));
+        if (hbig > 0) {
+            Tmp<int32_t> tsrc(static_cast<size_t>(n_inc), s);
+            Tmp<T> tval(static_cast<size_t>(n_inc), s);
+            k_sort_mirrors_big<T><<<static_cast<unsigned>(hbig), kBlock, 0, s>>>(big_rows.p, inc_rp.p, stage_src.p,
+                                                                                stage_val.p, tsrc.p, tval.p);
+            UMX_HIP(hipGetLastError());
+        }
+    }
     out_idx.alloc(static_cast<size_t>(nnz_out));
     out_val.alloc(static_cast<size_t>(nnz_out));
-    k_merge_rows<T><<<grid, kBlock, 0, s>>>(n, G, rp, sidx.p, nval.p, flags.p, inc_rp.p, stage_src.p, stage_val.p,
-                                            out_rp.p, out_idx.p, out_val.p);
+    k_merge_rows<T><<<grid, kBlock, 0, s>>>(n, G, mode, rp, sidx.p, nval.p, flags.p, inc_rp.p, stage_src.p,
+                                            stage_val.p, out_rp.p, out_idx.p, out_val.p);
     UMX_HIP(hipGetLastError());
     UMX_HIP(hipStreamSynchronize(s));
 }
@@ -876,7 +1025,9 @@ void run_to_epochs(int64_t n, const int64_t* rp, const int32_t* idx, const T* va
     int64_t nnz = 0;
     UMX_HIP(hipMemcpyAsync(&nnz, rp + n, sizeof(nnz), hipMemcpyDeviceToHost, s));
     UMX_HIP(hipStreamSynchronize(s));
-    const int G = lanes_for_degree(max_degree);
+    (void)max_degree;
+    // lanes per row from the average degree: long rows loop over chunks
+    const int G = lanes_for_degree(n > 0 ? (2 * nnz + n - 1) / n : 1);
     const unsigned grid = grid_for(n * G);
     const int n_inst = inst.n_inst;
     const int32_t* of_row = n_inst > 1 ? inst.of_row : nullptr;

@@ -383,6 +383,7 @@ struct BadFlag {
     int* p = nullptr;
     hipStream_t s;
     explicit BadFlag(hipStream_t st) : s(st) {
+        prepare_pool();
         UMX_HIP(hipMallocAsync(reinterpret_cast<void**>(&p), sizeof(int), s));
         k_zero_i32<<<1, 1, 0, s>>>(p);
         UMX_HIP(hipGetLastError());

@@ -7,6 +7,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 namespace umx {
 
@@ -47,6 +48,17 @@ int guarded(F&& f) {
 
 constexpr int kWave = 64;
 
+// Device memory from the device's stream-ordered pool (hipMallocAsync), which
+// keeps freed memory for reuse instead of returning it to the driver, so the
+// many buffers of a graph build cost no driver allocation after the first
+// build.  device_alloc returns memory that is ready on every stream;
+// device_free may only be called once no queued work uses the memory.
+void* device_alloc(size_t bytes);
+void device_free(void* p);
+// Sets the current device's pool to keep freed memory (idempotent); for
+// code that calls hipMallocAsync itself.
+void prepare_pool();
+
 // Device buffer owned by a handle.
 template <typename T>
 struct DevBuf {
@@ -57,14 +69,14 @@ struct DevBuf {
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) device_free(p);
         p = nullptr;
         n = 0;
     }
     void alloc(size_t count) {
         release();
         if (count == 0) count = 1;
-        UMX_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+        p = static_cast<T*>(device_alloc(count * sizeof(T)));
         n = count;
     }
     void swap(DevBuf& o) {
@@ -120,6 +132,49 @@ void run_to_epochs(int64_t n, const int64_t* rp, const int32_t* idx, const T* va
                    const int32_t* epochs_of_inst, const Instances& inst, T nsr, int64_t max_degree,
                    DevBuf<int64_t>& out_rp, DevBuf<int32_t>& out_tg, DevBuf<T>& eps, DevBuf<T>& nxt,
                    DevBuf<T>& nxt_neg, hipStream_t s);
+
+// ---- spectral initialisation (spectral.hip) ----
+
+struct SpectralParams {
+    double scale = 10.0;        // largest |coordinate| (Options::initialize_spectral_scale)
+    bool jitter = false;
+    double jitter_sd = 1e-4;
+    uint64_t jitter_seed = 0;
+    double tol = 1e-6;          // Ritz residual / Ritz value for convergence
+    int start_dim = 32;         // first Krylov dimension (doubled until converged)
+    int max_dim = 512;          // largest Krylov dimension
+    uint64_t start_seed = 0x5eedULL;
+};
+
+struct SpectralReport {
+    int64_t components = 0;
+    int krylov_dim = 0;
+    double worst_residual = 0;
+    std::vector<double> eigenvalues;  // of 2I - L, descending, num_dim + 1 of them
+};
+
+// Number of connected components of a symmetric CSR graph.
+int64_t count_components(int64_t n, const int64_t* rp, const int32_t* idx, hipStream_t s);
+
+// Spectral coordinates (point-major, n x d) of a symmetric weighted graph;
+// false (out untouched) for several components, n <= d + 1, or no
+// convergence.
+template <typename T>
+bool spectral_embedding(int64_t n, const int64_t* rp, const int32_t* idx, const T* val, int max_degree, int d,
+                        const SpectralParams& prm, T* out, SpectralReport* rep, hipStream_t s);
+
+// Symmetric tridiagonal eigen-decomposition (host_api.cpp): diagonal d[k],
+// off-diagonal e[k - 1]; eigenvalues into d (unsorted), eigenvectors into
+// the columns of z (row-major k x k).  False if the iteration does not
+// converge.
+bool tridiagonal_eigen(int k, std::vector<double>& d, const std::vector<double>& e, std::vector<double>& z);
+
+// ---- exact kNN (knn.hip) ----
+
+// k nearest neighbours (self excluded, ties to the smaller index) of every
+// observation of x (n x dim, point-major, device); k <= 64 and k < n.
+template <typename T>
+void knn_bruteforce(int64_t n, int dim, const T* x, int k, int32_t* idx, T* dist, hipStream_t s);
 
 // ---- layout kernels (layout_kernels.hip) ----
 
