@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <optional>
 #include <stdexcept>
+#include <vector>
 
 #include "NeighborList.hpp"
 #include "Options.hpp"
@@ -79,6 +80,31 @@ Status<Index_, Float_> initialize(NeighborList<Index_, Float_> x, const std::siz
     umx_layout* L = nullptr;
     detail::check(umx_layout_create(g, *options.num_epochs, options.negative_sample_rate, &L));
     return Status<Index_, Float_>(L, std::move(options), num_dim);
+}
+
+// Data overload (reference initialize.hpp:165-177, which takes a
+// knncolle::Builder): the exact options.num_neighbors nearest neighbours of
+// every observation are found on the GPU (umx_knn_bruteforce), then as
+// above.  data: num_obs observations of data_dim values each
+// ("data_dim x num_obs" column-major, as in the reference).
+template <typename Index_, typename Float_>
+Status<Index_, Float_> initialize(const std::size_t data_dim, const Index_ num_obs, const Float_* const data,
+                                  const std::size_t num_dim, Float_* const embedding, Options options) {
+    const int k = options.num_neighbors;
+    std::vector<int32_t> idx(static_cast<std::size_t>(num_obs) * static_cast<std::size_t>(k));
+    std::vector<Float_> dist(idx.size());
+    detail::check(umx_knn_bruteforce(detail::dtype_of<Float_>(), static_cast<int64_t>(num_obs),
+                                     static_cast<int>(data_dim), data, k, idx.data(), dist.data(), 0));
+    NeighborList<Index_, Float_> nl(static_cast<std::size_t>(num_obs));
+    for (Index_ i = 0; i < num_obs; ++i) {
+        auto& row = nl[static_cast<std::size_t>(i)];
+        row.reserve(static_cast<std::size_t>(k));
+        for (int q = 0; q < k; ++q) {
+            const std::size_t at = static_cast<std::size_t>(i) * static_cast<std::size_t>(k) + static_cast<std::size_t>(q);
+            row.emplace_back(static_cast<Index_>(idx[at]), dist[at]);
+        }
+    }
+    return initialize<Index_, Float_>(std::move(nl), num_dim, embedding, std::move(options));
 }
 
 }  // namespace umappp

@@ -109,7 +109,10 @@ def test_cpp_dropin_headers_compile(tmp_path):
         "  std::vector<float> emb(6);\n"
         "  umappp::Options opt; opt.initialize_method = umappp::InitializeMethod::NONE;\n"
         "  auto st = umappp::initialize(x, 2, emb.data(), opt);\n"
-        "  st.run(emb.data(), 10); return st.epoch();\n"
+        "  st.run(emb.data(), 10);\n"
+        "  std::vector<float> data(30);\n"
+        "  auto st2 = umappp::initialize(10, 3, data.data(), 2, emb.data(), opt);\n"
+        "  return st.epoch() + st2.epoch();\n"
         "}\n")
     subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror", "-pedantic",
                     f"-I{ROOT / 'include'}", str(src)], check=True)

@@ -191,7 +191,7 @@ def _c5(torch, dev, umx):
         import bench
 
         x, _ = mixture(5, 200_000, 50, 30, dtype=np.float32)
-        kidx, kdist = bench.knn_torch(x, 30, dev)
+        kidx, kdist = bench.knn_gpu(x, 30, dev)
         n = x.shape[0]
         rp = np.arange(n + 1, dtype=np.int64) * 30
         g = umx.Graph(umx.NeighborList(rp, kidx.cpu().numpy().reshape(-1), kdist.cpu().numpy().reshape(-1)))
