@@ -306,7 +306,7 @@ def test_status_run_and_restart_schedule(umx):
 @pytest.mark.parametrize("fires,cap", [(32.0, 2**31 - 1), (2.0, 2**31 - 1), (4.0, 5), (0.5, 1)])
 def test_work_items_cover_every_edge_once(umx, fires, cap):
     """Work items partition each row into contiguous runs, respect the split
-    threshold and edge cap, mark split rows, and run heaviest first."""
+    threshold and edge cap, mark split rows, and run longest first."""
     _, (rp, idx, dist) = c1_knn()
     wd, _, _ = oracle.neighbor_similarities(rp, dist)
     crp, cidx, cw = oracle.combine_neighbor_sets(rp, idx, wd, 1.0)
@@ -325,14 +325,12 @@ def test_work_items_cover_every_edge_once(umx, fires, cap):
     assert np.array_equal(split, nitems[row] > 1)
     assert np.array_equal(nitems > 0, np.diff(rpe) > 0)
     inv = 1.0 / ed.epochs_per_sample.astype(np.float64)
-    work = np.array([inv[s_:s_ + l_].sum() for s_, l_ in zip(b, ln)]) * 16
-    lg = np.log2(np.maximum(work, 1e-30))
-    lvl = np.where(work >= 1, 1 + np.floor(lg), 0).clip(0, 31)
-    # heaviest first, by power-of-two level (levels can differ by one where
-    # the work sits on a level boundary to float rounding)
-    drop = np.nonzero(np.diff(lvl) > 0)[0]
-    near = np.abs(lg - np.round(lg)) < 1e-4
-    assert np.all(near[drop] | near[drop + 1]), "items run heaviest first"
+    # longest first: lengths up to 32 in exact order; a longer item may be
+    # followed by one at most 1/8 longer (same length class), never more
+    pm = np.minimum.accumulate(ln)[:-1]
+    nxt = ln[1:]
+    assert np.all(np.where(pm <= 32, nxt <= pm, nxt * 8 < pm * 9)), "items run longest first"
+    assert ln[0] >= np.max(ln) * 8 / 9
     rowf = np.add.reduceat(inv, rpe[:-1])[nitems > 1]
     assert np.all(rowf > fires) or cap < 2**31 - 1
 

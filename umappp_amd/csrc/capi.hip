@@ -511,6 +511,7 @@ struct umx_layout {
     DevBuf<float> ef, nf, gf, mf;  // eps, next, next_neg, embedding (float)
     DevBuf<double> ed, nd, gd, md;  // same (double)
     DevBuf<unsigned long long> tally;
+    DevBuf<unsigned long long> tally_part;  // partial tallies of one launch (zero between launches)
     DevBuf<unsigned long long> cursor;  // serial replay cursor + overrun flag
     int dim = 0;
     int mode = UMX_LAYOUT_PULL;
@@ -612,6 +613,7 @@ umx::LayoutView<T> view_of(umx_layout* L, DevBuf<T>& eps, DevBuf<T>& nxt, DevBuf
     v.gamma = static_cast<T>(L->gamma);
     v.nsr = static_cast<T>(L->nsr);
     v.tally = L->tally.p;
+    v.tally_part = L->tally_part.p;
     v.chunk_width = nullptr;
     v.chunk_base = nullptr;
     v.n_chunks = 0;
@@ -832,6 +834,8 @@ void layout_from_graph(umx_layout* L, const umx_graph* g, const std::vector<int3
     UMX_HIP(hipMemcpyAsync(&L->nnz, L->rp.p + L->n, sizeof(int64_t), hipMemcpyDeviceToHost, L->s));
     L->tally.alloc(static_cast<size_t>(3) * static_cast<size_t>(L->total_epochs));
     UMX_HIP(hipMemsetAsync(L->tally.p, 0, sizeof(unsigned long long) * L->tally.n, L->s));
+    L->tally_part.alloc(static_cast<size_t>(umx::kTallySlots) * umx::kTallyStride);
+    UMX_HIP(hipMemsetAsync(L->tally_part.p, 0, sizeof(unsigned long long) * L->tally_part.n, L->s));
     L->cursor.alloc(2);
     UMX_HIP(hipMemsetAsync(L->cursor.p, 0, sizeof(unsigned long long) * 2, L->s));
     L->epoch_ctr.alloc(1);
@@ -868,8 +872,10 @@ void collect_events(umx_layout* L, int count) {
 }
 
 // One epoch of the sliced-ELL kernel: 'epoch' >= 0, or -1 to take it from
-// the device counter and advance that (captured blocks).
-void launch_sell_once(umx_layout* L, int epoch) {
+// the device counter and advance that (captured blocks).  Optional events
+// bracket the epoch kernel alone; the tally fold follows them.
+void launch_sell_once(umx_layout* L, int epoch, hipEvent_t start = nullptr, hipEvent_t stop = nullptr) {
+    if (start) UMX_HIP(hipEventRecord(start, L->s));
     if (L->dtype == UMX_F32) {
         auto v = view_of<float>(L, L->ef, L->nf, L->gf, L->mf);
         attach_sell(v, L->sell_f);
@@ -879,6 +885,8 @@ void launch_sell_once(umx_layout* L, int epoch) {
         attach_sell(v, L->sell_d);
         umx::launch_layout_sell<double>(v, epoch, L->tile_cols, L->waves, L->s);
     }
+    if (stop) UMX_HIP(hipEventRecord(stop, L->s));
+    umx::finish_layout_epoch(L->tally_part.p, L->tally.p, epoch, L->epoch_ctr.p, L->s);
 }
 
 // Identity of what a captured block would launch: the kernel arguments,
@@ -937,11 +945,7 @@ void run_sell(umx_layout* L, int first, int count) {
     ensure_tables(L);
     if (L->timing) {
         prepare_events(L, count);
-        for (int q = 0; q < count; ++q) {
-            UMX_HIP(hipEventRecord(L->events[2 * q], L->s));
-            launch_sell_once(L, first + q);
-            UMX_HIP(hipEventRecord(L->events[2 * q + 1], L->s));
-        }
+        for (int q = 0; q < count; ++q) launch_sell_once(L, first + q, L->events[2 * q], L->events[2 * q + 1]);
     } else if (L->graph_block > 0 && L->trace_cap == 0 && count >= L->graph_block) {
         umx::set_epoch_counter(L->epoch_ctr.p, first, L->s);
         int done = 0;
@@ -1115,6 +1119,7 @@ int umx_layout_run(umx_layout* L, int epoch_limit) {
                     umx::launch_layout_epoch<double>(view_of<double>(L, L->ed, L->nd, L->gd, L->md), ea, L->s);
                 }
                 if (L->timing) UMX_HIP(hipEventRecord(L->events[2 * q + 1], L->s));
+                umx::finish_layout_epoch(L->tally_part.p, L->tally.p, ep, L->epoch_ctr.p, L->s);
             }
             UMX_HIP(hipStreamSynchronize(L->s));
             collect_events(L, count);

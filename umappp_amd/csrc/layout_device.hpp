@@ -100,9 +100,11 @@ __device__ inline Vec<float, 4> load_point<float, 4>(const float* emb, int64_t i
     return Vec<float, 4>{{q.x, q.y, q.z, q.w}};
 }
 
-// Wave-aggregated tally update: one atomic per wavefront per counter.
-__device__ inline void tally_add(unsigned long long* t, unsigned long long fires, unsigned long long draws,
-                                 unsigned long long self_hits) {
+// Wave-aggregated tally update: one atomic per wavefront per counter, into
+// partial-tally slot 'slot' % kTallySlots.
+__device__ inline void tally_add(unsigned long long* part, int64_t slot, unsigned long long fires,
+                                 unsigned long long draws, unsigned long long self_hits) {
+    unsigned long long* t = part + (slot % kTallySlots) * kTallyStride;
 #pragma unroll
     for (int m = kWave / 2; m >= 1; m >>= 1) {
         fires += __shfl_xor(fires, m, kWave);

@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(kBlock) k_layout_rows(LayoutView<T> v, EpochCo
 #pragma unroll
         for (int d = 0; d < D; ++d) add_to(v.emb + i * D + d, h.c[d] - h0.c[d]);
     }
-    tally_add(v.tally + 3 * static_cast<int64_t>(ec.epoch), fires, draws, self_hits);
+    tally_add(v.tally_part, m / kWave, fires, draws, self_hits);
 }
 
 // Any embedding dimension: the head is updated in place with atomics after
@@ -169,7 +169,7 @@ __global__ void __launch_bounds__(kBlock) k_layout_rows_generic(LayoutView<T> v,
             v.nxt_neg[j] = nn + static_cast<T>(nneg) * epns;
         }
     }
-    tally_add(v.tally + 3 * static_cast<int64_t>(ec.epoch), fires, draws, self_hits);
+    tally_add(v.tally_part, m / kWave, fires, draws, self_hits);
 }
 
 // Serial replay in the reference's order and precision (optimize_layout.hpp:
@@ -283,7 +283,7 @@ constexpr int kNegPrefetch = 8;   // negative samples whose coordinates are fetc
 // mask (schedule values fetched kMaskLoads at a time), then walks only the
 // firing edges with count-trailing-zeros, so on every trip of the loop each
 // lane that still has work applies a whole firing edge; items are sorted by
-// expected firings, so lanes of a wavefront finish together.  The tail and
+// length, so lanes of a wavefront finish together.  The tail and
 // the first kNegPrefetch negative samples of an edge are fetched as one
 // batch of independent loads.  Steps are applied in the reference's order.
 template <typename T, int D, bool TRACE>
@@ -395,7 +395,7 @@ __global__ void __launch_bounds__(kBlock) k_layout_pull(LayoutView<T> v, EpochCo
             }
         }
     }
-    tally_add(v.tally + 3 * static_cast<int64_t>(ec.epoch), fires, draws, self_hits);
+    tally_add(v.tally_part, m / kWave, fires, draws, self_hits);
 }
 
 // Pull mode for any embedding dimension: the owner updates its point in place.
@@ -450,7 +450,7 @@ __global__ void __launch_bounds__(kBlock) k_layout_pull_generic(LayoutView<T> v,
             v.nxt_neg[j] = nn + static_cast<T>(nneg) * epns;
         }
     }
-    tally_add(v.tally + 3 * static_cast<int64_t>(ec.epoch), fires, draws, self_hits);
+    tally_add(v.tally_part, m / kWave, fires, draws, self_hits);
 }
 
 // Work items (see build_work_items).  One thread per row: the expected
@@ -462,8 +462,7 @@ __global__ void __launch_bounds__(kBlock) k_layout_pull_generic(LayoutView<T> v,
 template <typename T, bool WRITE>
 __device__ inline int64_t split_row(const int64_t* __restrict__ rp, const T* __restrict__ eps, int64_t i,
                                     double fires_split, int edges_cap, int64_t m, int64_t* __restrict__ ibeg,
-                                    int32_t* __restrict__ ilen, int32_t* __restrict__ irow, float* __restrict__ iwork,
-                                    bool split_tag) {
+                                    int32_t* __restrict__ ilen, int32_t* __restrict__ irow, bool split_tag) {
     const int64_t r0 = rp[i], r1 = rp[i + 1];
     if (r1 == r0) return 0;
     double f = 0;
@@ -471,7 +470,7 @@ __device__ inline int64_t split_row(const int64_t* __restrict__ rp, const T* __r
     const double c = f > fires_split ? ceil(f / fires_split) : 1.0;
     const int32_t tag = static_cast<int32_t>(static_cast<uint32_t>(i) | (split_tag ? 0x80000000u : 0u));
     int64_t items = 0, start = r0;
-    double acc = 0, item_f = 0, q = 1;
+    double acc = 0, q = 1;
     for (int64_t j = r0; j < r1; ++j) {
         const double w = 1.0 / static_cast<double>(eps[j]);
         const bool cut = j > start && ((q < c && acc + w > q * f / c) || j - start >= edges_cap);
@@ -480,21 +479,17 @@ __device__ inline int64_t split_row(const int64_t* __restrict__ rp, const T* __r
                 ibeg[m + items] = start;
                 ilen[m + items] = static_cast<int32_t>(j - start);
                 irow[m + items] = tag;
-                iwork[m + items] = static_cast<float>(item_f);
             }
             ++items;
             if (acc + w > q * f / c) q += 1;
             start = j;
-            item_f = 0;
         }
         acc += w;
-        item_f += w;
     }
     if (WRITE) {
         ibeg[m + items] = start;
         ilen[m + items] = static_cast<int32_t>(r1 - start);
         irow[m + items] = tag;
-        iwork[m + items] = static_cast<float>(item_f);
     }
     return items + 1;
 }
@@ -505,50 +500,57 @@ __global__ void __launch_bounds__(kBlock) k_count_items(int64_t n, const int64_t
                                                       int64_t* __restrict__ count) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;
-    count[i] = split_row<T, false>(rp, eps, i, fires_split, edges_cap, 0, nullptr, nullptr, nullptr, nullptr, false);
+    count[i] = split_row<T, false>(rp, eps, i, fires_split, edges_cap, 0, nullptr, nullptr, nullptr, false);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_fill_items(int64_t n, const int64_t* __restrict__ rp,
                                                      const T* __restrict__ eps, double fires_split, int edges_cap,
                                                      const int64_t* __restrict__ offs, int64_t* __restrict__ ibeg,
-                                                     int32_t* __restrict__ ilen, int32_t* __restrict__ irow,
-                                                     float* __restrict__ iwork) {
+                                                     int32_t* __restrict__ ilen, int32_t* __restrict__ irow) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;
     const int64_t c = offs[i + 1] - offs[i];
     if (c == 0) return;
-    split_row<T, true>(rp, eps, i, fires_split, edges_cap, offs[i], ibeg, ilen, irow, iwork, c > 1);
+    split_row<T, true>(rp, eps, i, fires_split, edges_cap, offs[i], ibeg, ilen, irow, c > 1);
 }
 
-// Heaviest items first: a stable counting sort of the items on a 32-level
-// key (descending log2 of expected firings), so that long items start at
-// once and each wavefront gets items of similar length.  With placement
-// groups the key is group * 32 + level.  Pass A counts keys per tile of
-// kBlock items; pass B scatters each item to offset[key][tile] + its rank
-// among the tile's items with that key, then places it (see place_item).
-constexpr int kLevels = 32;
+// Longest items first: a stable counting sort of the items on a 128-level
+// length key, so that long items start at once and the 64 items of a
+// wavefront have nearly equal lengths (the sliced-ELL chunk of a wavefront
+// is as wide as its longest item, so mixed lengths would be padding).  With
+// placement groups the key is group * 128 + level.  Pass A counts keys per
+// tile of kBlock items; pass B scatters each item to offset[key][tile] + its
+// rank among the tile's items with that key, then places it (see
+// place_item).
+constexpr int kLevels = 128;
 
-__device__ inline int work_key(float f) {
-    // 31 for the heaviest items down to 0: key = 31 - clamp(floor(log2(16 f)) + 1, 0, 31)
-    int lvl = 0;
-    const float x = f * 16.0f;
-    if (x >= 1.0f) lvl = 1 + static_cast<int>(floorf(log2f(x)));
-    lvl = lvl > 31 ? 31 : lvl;
-    return 31 - lvl;
+// 127 for the longest items down to 0 for length 1 (length 0 never occurs):
+// lengths up to 32 each have a level of their own, longer ones eight levels
+// per octave (within 1/8 of each other); lengths from 122880 share key 0.
+__device__ inline int length_key(int32_t len) {
+    int lvl;
+    if (len <= 32) {
+        lvl = len;
+    } else {
+        const int o = 31 - __clz(len);  // floor(log2 len) >= 5
+        lvl = 33 + (o - 5) * 8 + ((len >> (o - 3)) & 7);
+    }
+    lvl = lvl > kLevels - 1 ? kLevels - 1 : lvl;
+    return kLevels - 1 - lvl;
 }
 
-__device__ inline int item_key(const float* __restrict__ iwork, const int32_t* __restrict__ irow, int64_t m,
+__device__ inline int item_key(const int32_t* __restrict__ ilen, const int32_t* __restrict__ irow, int64_t m,
                                const int32_t* __restrict__ of_row, const int32_t* __restrict__ group_of_inst) {
     int g = 0;
     if (group_of_inst) {
         const int64_t row = static_cast<int64_t>(static_cast<uint32_t>(irow[m]) & 0x7fffffffu);
         g = group_of_inst[of_row[row]];
     }
-    return g * kLevels + work_key(iwork[m]);
+    return g * kLevels + length_key(ilen[m]);
 }
 
-__global__ void __launch_bounds__(kBlock) k_item_key_hist(const float* __restrict__ iwork,
+__global__ void __launch_bounds__(kBlock) k_item_key_hist(const int32_t* __restrict__ ilen,
                                                         const int32_t* __restrict__ irow, int64_t n_items,
                                                         const int32_t* __restrict__ of_row,
                                                         const int32_t* __restrict__ group_of_inst, int n_keys,
@@ -557,7 +559,7 @@ __global__ void __launch_bounds__(kBlock) k_item_key_hist(const float* __restric
     for (int k = threadIdx.x; k < n_keys; k += kBlock) cnt[k] = 0;
     __syncthreads();
     const int64_t m = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    if (m < n_items) atomicAdd(&cnt[item_key(iwork, irow, m, of_row, group_of_inst)], 1);
+    if (m < n_items) atomicAdd(&cnt[item_key(ilen, irow, m, of_row, group_of_inst)], 1);
     __syncthreads();
     for (int k = threadIdx.x; k < n_keys; k += kBlock) hist[static_cast<int64_t>(k) * ntiles + blockIdx.x] = cnt[k];
 }
@@ -574,7 +576,7 @@ __device__ inline int64_t place_item(int64_t at, int64_t g0, int g, int waves, i
 }
 
 __global__ void __launch_bounds__(kBlock) k_item_scatter(const int64_t* __restrict__ ibeg, const int32_t* __restrict__ ilen,
-                                                       const int32_t* __restrict__ irow, const float* __restrict__ iwork,
+                                                       const int32_t* __restrict__ irow,
                                                        int64_t n_items, const int32_t* __restrict__ of_row,
                                                        const int32_t* __restrict__ group_of_inst, int n_keys,
                                                        int64_t ntiles, const int64_t* __restrict__ offs, int waves,
@@ -588,7 +590,7 @@ __global__ void __launch_bounds__(kBlock) k_item_scatter(const int64_t* __restri
     __syncthreads();
     const int64_t m = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     const bool live = m < n_items;
-    const int key = live ? item_key(iwork, irow, m, of_row, group_of_inst) : -1;
+    const int key = live ? item_key(ilen, irow, m, of_row, group_of_inst) : -1;
     // rank among earlier lanes of this wave with the same key: one round per
     // distinct key present in the wave
     int rank = 0;
@@ -731,13 +733,11 @@ void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_s
     const size_t M = static_cast<size_t>(real > 0 ? real : 1);
     DevBuf<int64_t> ib;
     DevBuf<int32_t> il, ir;
-    DevBuf<float> iw;
     ib.alloc(M);
     il.alloc(M);
     ir.alloc(M);
-    iw.alloc(M);
     if (n > 0)
-        k_fill_items<T><<<grid, kBlock, 0, s>>>(n, rp, eps, fires_split, edges_cap, offs.p, ib.p, il.p, ir.p, iw.p);
+        k_fill_items<T><<<grid, kBlock, 0, s>>>(n, rp, eps, fires_split, edges_cap, offs.p, ib.p, il.p, ir.p);
     UMX_HIP(hipGetLastError());
 
     const int G = place.n_groups;
@@ -753,7 +753,7 @@ void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_s
     int64_t padded = real;
     if (real > 0) {
         k_item_key_hist<<<static_cast<unsigned>(ntiles), kBlock, sizeof(int) * n_keys, s>>>(
-            iw.p, ir.p, real, of_row, goi, n_keys, ntiles, hist.p);
+            il.p, ir.p, real, of_row, goi, n_keys, ntiles, hist.p);
         UMX_HIP(hipGetLastError());
         exclusive_scan_i64(hist.p, hoff.p, n_keys * ntiles, unused, s);
         if (G > 1 || place.waves > 1) {
@@ -782,7 +782,7 @@ void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_s
     if (real > 0) {
         const size_t lds = sizeof(int) * static_cast<size_t>((kBlock / kWave) * n_keys);
         k_item_scatter<<<static_cast<unsigned>(ntiles), kBlock, lds, s>>>(
-            ib.p, il.p, ir.p, iw.p, real, of_row, goi, n_keys, ntiles, hoff.p, place.waves, G, item_begin.p,
+            ib.p, il.p, ir.p, real, of_row, goi, n_keys, ntiles, hoff.p, place.waves, G, item_begin.p,
             item_len.p, item_row.p, item_inst.p);
         UMX_HIP(hipGetLastError());
     }

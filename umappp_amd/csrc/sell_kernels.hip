@@ -1,7 +1,7 @@
 // Production layout epoch for gfx950 over a sliced-ELL copy of the schedule.
 //
 // Layout in HBM.  Work items (rows, heavy rows cut into pieces, sorted
-// heaviest first) are grouped 64 at a time into chunks, one chunk per
+// longest first) are grouped 64 at a time into chunks, one chunk per
 // wavefront.  A chunk of width W (its longest item) holds 64 * W slots stored
 // column-major: slot(k, lane) = chunk_base + k * 64 + lane, slot k of a lane
 // being edge k of its item (padding slots never fire).  So a wavefront's load
@@ -17,7 +17,9 @@
 // step, reverse-edge tail step, negative samples, exactly as in
 // k_layout_pull), fetching the tail and up to kNegPrefetch negative samples
 // in one batch of independent loads.  Updated schedule values go back to HBM
-// with coalesced stores.  The CSR arrays remain the canonical schedule state;
+// with coalesced stores.  Schedule loads and stores are non-temporal, so the
+// stream of schedule bytes does not evict from the L2 the embedding that the
+// gathers read.  The CSR arrays remain the canonical schedule state;
 // the host syncs the two copies when it switches between this kernel and the
 // CSR kernels (serial replay, push mode, export).
 //
@@ -26,8 +28,9 @@
 // and count, seed, epochs, learning rates), draws negative samples inside its
 // instance, and keys the sampler by the instance's seed and its own entry
 // numbers, so an instance sees the samples it would see when run alone.  The
-// epoch number lives in device memory and the last workgroup advances it, so
-// a block of launches captured once as a HIP graph replays for any epochs.
+// epoch number can live in device memory, advanced by a one-wavefront kernel
+// after each launch (the one that also folds the partial tallies), so a block
+// of launches captured once as a HIP graph replays for any epochs.
 #include "layout_device.hpp"
 #include "umx_internal.hpp"
 
@@ -134,7 +137,7 @@ struct SellTile {
 // instance constants (observation offset and count, sampler seed, epochs,
 // learning rate); a plain layout is one instance.  The epoch is epoch_arg,
 // or v.epoch_ctr[0] when epoch_arg < 0 (captured blocks of epochs, where a
-// one-thread kernel advances the counter after each launch).
+// one-wavefront kernel advances the counter after each launch).
 template <typename T, int D, int TW, bool TRACE>
 __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_arg) {
     extern __shared__ __align__(16) unsigned char sell_lds[];
@@ -186,11 +189,11 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
             for (int k = 0; k < TW; ++k) {
                 if (k < tw) {
                     const int64_t slot = base + static_cast<int64_t>(t0 + k) * kWave + lane;
-                    const T x = v.sell_nx[slot];
+                    const T x = ld_stream(v.sell_nx + slot);
                     tile.nx[k][lane] = x;
-                    tile.nn[k][lane] = v.sell_nn[slot];
-                    tile.eps[k][lane] = v.sell_eps[slot];
-                    tile.tg[k][lane] = v.sell_tg[slot];
+                    tile.nn[k][lane] = ld_stream(v.sell_nn + slot);
+                    tile.eps[k][lane] = ld_stream(v.sell_eps + slot);
+                    tile.tg[k][lane] = ld_stream(v.sell_tg + slot);
                     if (!(x > ep)) mask |= 1u << k;
                 }
             }
@@ -277,8 +280,8 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
             for (int k = 0; k < TW; ++k) {
                 if (k < tw && (fired >> k) & 1u) {
                     const int64_t slot = base + static_cast<int64_t>(t0 + k) * kWave + lane;
-                    v.sell_nx[slot] = tile.nx[k][lane];
-                    v.sell_nn[slot] = tile.nn[k][lane];
+                    st_stream(v.sell_nx + slot, tile.nx[k][lane]);
+                    st_stream(v.sell_nn + slot, tile.nn[k][lane]);
                 }
             }
         }
@@ -291,12 +294,35 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
             }
         }
     }
-    tally_add(v.tally + 3 * static_cast<int64_t>(epn), fires, draws, self_hits);
+    tally_add(v.tally_part, c, fires, draws, self_hits);
 }
 
 __global__ void k_set_epoch(int* ctr, int epoch) { ctr[0] = epoch; }
 
-__global__ void k_advance_epoch(int* ctr) { ctr[0] += 1; }
+// Folds the partial tallies of one epoch launch into the epoch's tally and
+// clears them (one lane per slot); with epoch < 0 the epoch is epoch_ctr[0],
+// advanced by one afterwards.
+__global__ void __launch_bounds__(kTallySlots) k_finish_epoch(unsigned long long* __restrict__ part,
+                                                             unsigned long long* __restrict__ tally, int epoch,
+                                                             int* __restrict__ ctr) {
+    static_assert(kTallySlots == kWave, "one lane per partial-tally slot");
+    unsigned long long* p = part + static_cast<int64_t>(threadIdx.x) * kTallyStride;
+    unsigned long long c[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) c[q] = atomicExch(p + q, 0ULL);
+#pragma unroll
+    for (int m = kWave / 2; m >= 1; m >>= 1) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) c[q] += __shfl_xor(c[q], m, kWave);
+    }
+    if (threadIdx.x == 0) {
+        const int e = epoch >= 0 ? epoch : ctr[0];
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            if (c[q]) atomicAdd(tally + 3 * static_cast<int64_t>(e) + q, c[q]);
+        if (epoch < 0) ctr[0] = e + 1;
+    }
+}
 
 template <typename T, int TW>
 void launch_sell_tw(const LayoutView<T>& v, int epoch, int waves, hipStream_t s) {
@@ -385,10 +411,12 @@ void launch_layout_sell(const LayoutView<T>& v, int epoch, int tile_cols, int wa
         }
         UMX_HIP(hipGetLastError());
     }
-    if (epoch < 0) {
-        k_advance_epoch<<<1, 1, 0, s>>>(v.epoch_ctr);
-        UMX_HIP(hipGetLastError());
-    }
+}
+
+void finish_layout_epoch(unsigned long long* tally_part, unsigned long long* tally, int epoch, int* epoch_ctr,
+                         hipStream_t s) {
+    k_finish_epoch<<<1, kTallySlots, 0, s>>>(tally_part, tally, epoch, epoch_ctr);
+    UMX_HIP(hipGetLastError());
 }
 
 void set_epoch_counter(int* epoch_ctr, int epoch, hipStream_t s) {

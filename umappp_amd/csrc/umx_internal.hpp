@@ -212,10 +212,14 @@ struct LayoutView {
     int mode;
     T a, b, gamma, nsr;
     unsigned long long* tally;  // 3 counters per epoch
+    // Per-epoch partial tallies of the parallel kernels: kTallySlots slots of
+    // kTallyStride counters (the first 3 used), folded into tally[] and
+    // cleared by finish_layout_epoch after every launch.
+    unsigned long long* tally_part;
     // Work items of the production kernels: item m covers edges
     // [item_begin[m], item_begin[m] + item_len[m]) of row
     // item_row[m] & 0x7fffffff; the top bit marks rows split over several
-    // items.  Items are ordered heaviest first.
+    // items.  Items are ordered longest first.
     const int64_t* item_begin;
     const int32_t* item_len;
     const int32_t* item_row;
@@ -249,6 +253,19 @@ struct LayoutView {
 template <typename T>
 void launch_layout_epoch(const LayoutView<T>& v, const EpochArgs& ea, hipStream_t s);
 
+// Partial tallies: a wavefront adds its counts to slot (its index modulo
+// kTallySlots); slots are kTallyStride counters (256 bytes) apart, so the
+// atomics of an epoch spread over many memory channels instead of queueing
+// on one address.
+constexpr int kTallySlots = 64;
+constexpr int kTallyStride = 32;
+
+// After each parallel epoch launch: adds the partial tallies into the tally
+// of the epoch (epoch >= 0, else the epoch in epoch_ctr[0], which is then
+// advanced by one) and clears them.  One wavefront; safe in graph capture.
+void finish_layout_epoch(unsigned long long* tally_part, unsigned long long* tally, int epoch, int* epoch_ctr,
+                         hipStream_t s);
+
 template <typename T>
 void launch_layout_serial(const LayoutView<T>& v, const EpochArgs& ea, const int32_t* samples, int64_t count,
                           unsigned long long* cursor, hipStream_t s);
@@ -256,7 +273,7 @@ void launch_layout_serial(const LayoutView<T>& v, const EpochArgs& ea, const int
 // Work items of the production layout kernels: one per row, except that rows
 // expected to fire more than 'fires_split' edges per epoch (sum of 1 / eps)
 // or longer than 'edges_cap' are cut into items of about equal expected
-// firings.  Items are sorted heaviest first.
+// firings.  Items are sorted longest first.
 //
 // Placement for the sliced-ELL kernel: a workgroup of 'waves' wavefronts
 // takes waves * 64 consecutive items.  With n_groups > 1, each instance
@@ -307,9 +324,9 @@ inline size_t sell_lds_bytes(size_t value_bytes, int tile_cols, int waves) {
 }
 
 // One epoch of the sliced-ELL kernel.  epoch >= 0: that epoch.  epoch < 0:
-// the epoch in v.epoch_ctr[0] on the device, which a one-thread kernel
-// launched right after advances (so a captured block of launches replays for
-// any epochs).
+// the epoch in v.epoch_ctr[0] on the device, which finish_layout_epoch,
+// launched right after, advances (so a captured block of launches replays
+// for any epochs).
 template <typename T>
 void launch_layout_sell(const LayoutView<T>& v, int epoch, int tile_cols, int waves, hipStream_t s);
 
