@@ -385,17 +385,36 @@ __global__ void k_scan_total(const int64_t* __restrict__ in, int64_t* __restrict
     out[count] = count > 0 ? out[count - 1] + in[count - 1] : 0;
 }
 
-__global__ void __launch_bounds__(kBlock) k_max_row(const int64_t* __restrict__ rp, int64_t n,
-                                                  unsigned long long* __restrict__ out) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    long long len = 0;
-    if (i < n) len = rp[i + 1] - rp[i];
+// Largest value of the workgroup (an integer type), for one atomic per workgroup:
+// single-address atomics serialise at one memory channel, so reductions
+// below keep their grids to at most kReduceBlocks workgroups.
+constexpr int64_t kReduceBlocks = 1024;
+
+template <typename U>
+__device__ inline U block_max(U v) {
+    __shared__ U part[kBlock / kWave];
 #pragma unroll
     for (int m = kWave / 2; m >= 1; m >>= 1) {
-        const long long o = __shfl_xor(len, m, kWave);
-        len = o > len ? o : len;
+        const U o = __shfl_xor(v, m, kWave);
+        v = o > v ? o : v;
     }
-    if ((threadIdx.x & (kWave - 1)) == 0 && len > 0) atomicMax(out, static_cast<unsigned long long>(len));
+    if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < kBlock / kWave; ++w) v = part[w] > v ? part[w] : v;
+    return v;  // meaningful in thread 0
+}
+
+__global__ void __launch_bounds__(kBlock) k_max_row(const int64_t* __restrict__ rp, int64_t n,
+                                                  unsigned long long* __restrict__ out) {
+    unsigned long long len = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kBlock) {
+        const unsigned long long l = static_cast<unsigned long long>(rp[i + 1] - rp[i]);
+        len = l > len ? l : len;
+    }
+    len = block_max(len);
+    if (threadIdx.x == 0 && len > 0) atomicMax(out, len);
 }
 
 // ---------------------------------------------------------------------------
@@ -708,12 +727,8 @@ __global__ void __launch_bounds__(kBlock) k_max_weight(const T* __restrict__ val
             best = bits > best ? bits : best;
         }
     }
-#pragma unroll
-    for (int m = kWave / 2; m >= 1; m >>= 1) {
-        const S o = __shfl_xor(best, m, kWave);
-        best = o > best ? o : best;
-    }
-    if ((threadIdx.x & (kWave - 1)) == 0 && best > 0) atomicMax(out + q, best);
+    best = block_max(best);
+    if (threadIdx.x == 0 && best > 0) atomicMax(out + q, best);
 }
 
 template <typename T>
@@ -876,7 +891,7 @@ int64_t max_row_length(const int64_t* rp, int64_t n, DevBuf<int64_t>& tmp, hipSt
     if (n == 0) return 0;
     Tmp<unsigned long long> m(1, s);
     UMX_HIP(hipMemsetAsync(m.p, 0, sizeof(unsigned long long), s));
-    k_max_row<<<grid_for(n), kBlock, 0, s>>>(rp, n, m.p);
+    k_max_row<<<static_cast<unsigned>(std::min<int64_t>((n + kBlock - 1) / kBlock, kReduceBlocks)), kBlock, 0, s>>>(rp, n, m.p);
     UMX_HIP(hipGetLastError());
     unsigned long long h = 0;
     UMX_HIP(hipMemcpyAsync(&h, m.p, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -1036,7 +1051,7 @@ void run_to_epochs(int64_t n, const int64_t* rp, const int32_t* idx, const T* va
     UMX_HIP(hipMemsetAsync(maxbits.p, 0, sizeof(S) * static_cast<size_t>(n_inst), s));
     if (nnz > 0) {
         const int64_t want = (nnz / n_inst + kBlock - 1) / kBlock + 1;
-        const unsigned gx = static_cast<unsigned>(std::min<int64_t>(want, n_inst > 1 ? 512 : 8192));
+        const unsigned gx = static_cast<unsigned>(std::min<int64_t>(want, n_inst > 1 ? 512 : kReduceBlocks));
         k_max_weight<T><<<dim3(gx, static_cast<unsigned>(n_inst)), kBlock, 0, s>>>(
             val, nnz, rp, n_inst > 1 ? inst.pt_off : nullptr, maxbits.p);
         UMX_HIP(hipGetLastError());

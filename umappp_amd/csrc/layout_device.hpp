@@ -37,11 +37,13 @@ __host__ __device__ inline uint32_t lowbias32(uint32_t x) {
 // the draw hashes the low key word with lowbias32 and maps the 32-bit result
 // by a high multiply (relative bias below n / 2^32); beyond, a full 64-bit
 // mix64 draw is mapped by a 64-bit high multiply.
+constexpr uint32_t kSmallDraw = 1u << 24;
+__device__ inline int32_t draw_index_small(uint64_t ej, uint32_t p, uint32_t n) {  // n <= kSmallDraw
+    const uint32_t r = lowbias32(static_cast<uint32_t>(ej) + 0x9e3779b9U * (p + 1)) ^ static_cast<uint32_t>(ej >> 32);
+    return static_cast<int32_t>(__umulhi(r, n));
+}
 __device__ inline int32_t draw_index(uint64_t ej, uint32_t p, uint32_t n) {
-    if (n <= (1u << 24)) {
-        const uint32_t r = lowbias32(static_cast<uint32_t>(ej) + 0x9e3779b9U * (p + 1)) ^ static_cast<uint32_t>(ej >> 32);
-        return static_cast<int32_t>(__umulhi(r, n));
-    }
+    if (n <= kSmallDraw) return draw_index_small(ej, p, n);
     const uint64_t r = mix64(ej + kGold * (static_cast<uint64_t>(p) + 1));
     return static_cast<int32_t>(__umul64hi(r, static_cast<uint64_t>(n)));
 }
@@ -50,6 +52,12 @@ template <typename T>
 __device__ inline T clip4(T v) {  // std::min(std::max(v, -4), 4), optimize_layout.hpp:105-110
     const T lo = v < static_cast<T>(-4) ? static_cast<T>(-4) : v;
     return static_cast<T>(4) < lo ? static_cast<T>(4) : lo;
+}
+// float: the median of (v, -4, 4) in one instruction, equal to the clamp
+// above for every non-NaN v.
+template <>
+__device__ inline float clip4<float>(float v) {
+    return __builtin_amdgcn_fmed3f(v, -4.0f, 4.0f);
 }
 template <typename T>
 __device__ inline T eps_of();

@@ -316,9 +316,12 @@ __global__ void k_zero_f64(double* p, int64_t count) {
 }
 __global__ void k_zero_i32(int* p) { *p = 0; }
 
+// Grid of the grid-stride loss kernels: at most 2048 workgroups (8 per CU),
+// so that the one scalar atomic per workgroup (loss, d/da, d/db), which
+// serialises at a single address, stays a small cost.
 unsigned loss_grid(int64_t nnz) {
     const int64_t want = (nnz + kLossBlock - 1) / kLossBlock;
-    return static_cast<unsigned>(want < 1 ? 1 : (want > 16384 ? 16384 : want));
+    return static_cast<unsigned>(want < 1 ? 1 : (want > 2048 ? 2048 : want));
 }
 
 template <typename F>

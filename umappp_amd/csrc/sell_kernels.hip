@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
     const int epn = epoch_arg >= 0 ? epoch_arg : v.epoch_ctr[0];
     const T ep = static_cast<T>(epn);
 
-    unsigned long long fires = 0, draws = 0, self_hits = 0;
+    uint32_t fires = 0, draws = 0, self_hits = 0;  // one chunk's counts fit in 32 bits
     if (c < v.n_chunks) {
         const int64_t m = c * kWave + lane;
         bool live = m < v.n_items;
@@ -156,6 +156,7 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
         const int64_t base = v.chunk_base[c];
 
         int64_t i = 0, e0 = 0, edge_off = 0, pt_off = 0;
+        int32_t i32 = 0;
         bool split = false;
         uint32_t n32 = 0;
         uint64_t ekey = 0;
@@ -172,6 +173,7 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
             alpha = live ? v.alpha_tbl[static_cast<int64_t>(q) * v.alpha_stride + epn] : static_cast<T>(0);
             const uint32_t packed = static_cast<uint32_t>(v.item_row[m]);
             i = static_cast<int64_t>(packed & 0x7fffffffu);
+            i32 = static_cast<int32_t>(i);
             split = (packed >> 31) != 0;
             e0 = v.item_begin[m];
             h0 = load_point<T, D>(v.emb, i);
@@ -180,6 +182,26 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
         const T two_ab = -2 * v.a * v.b;
         const T two_gb = 2 * v.gamma * v.b;
         bool moved = false;
+        // The sampler's arithmetic, chosen once per wavefront: the 32-bit
+        // path when every lane's instance has at most 2^24 observations (the
+        // draw is the same function of the key, draw number and n either way;
+        // a per-lane choice would run both paths under divergence).
+        const bool small_n = __ballot(n32 > kSmallDraw) == 0;
+        // Draws p0 .. p0 + kNegFetch - 1 of an edge and their points, as one
+        // batch of unconditional loads (slots past nneg read the head's own
+        // point and are marked -1); conditional loads would each be followed
+        // by a wait, leaving one or two gathers in flight instead of all.
+        auto fetch = [&](uint64_t ej, int p0, int nneg, int32_t* kk, Vec<T, D>* q) {
+#pragma unroll
+            for (int u = 0; u < kNegFetch; ++u) {
+                const uint32_t p = static_cast<uint32_t>(p0 + u);
+                const int32_t d =
+                    static_cast<int32_t>(pt_off + (small_n ? draw_index_small(ej, p, n32) : draw_index(ej, p, n32)));
+                const bool use = p0 + u < nneg;
+                kk[u] = use ? d : -1;
+                q[u] = load_point<T, D>(v.emb, use ? static_cast<int64_t>(d) : i);
+            }
+        };
 
         for (int t0 = 0; t0 < width; t0 += TW) {
             const int tw = width - t0 < TW ? width - t0 : TW;
@@ -212,15 +234,12 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
                 const T nnj = tile.nn[k][lane];
                 const T epns = epsj / v.nsr;
                 const int nneg = static_cast<int>((ep - nnj) / epns);
+                draws += static_cast<uint32_t>(nneg > 0 ? nneg : 0);
                 const uint64_t ej = edge_key(ekey, static_cast<uint64_t>(j));
                 const Vec<T, D> r = load_point<T, D>(v.emb, t);
                 int32_t kk[kNegFetch];
                 Vec<T, D> q[kNegFetch];
-#pragma unroll
-                for (int u = 0; u < kNegFetch; ++u) {
-                    kk[u] = u < nneg ? static_cast<int32_t>(pt_off + draw_index(ej, static_cast<uint32_t>(u), n32)) : -1;
-                    if (kk[u] >= 0) q[u] = load_point<T, D>(v.emb, kk[u]);
-                }
+                fetch(ej, 0, nneg, kk, q);
 #pragma unroll
                 for (int half = 0; half < 2; ++half) {
                     const T d2 = sqdist<T, D>(h, r);
@@ -239,23 +258,14 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
                     }
                 }
                 for (int p0 = 0; p0 < nneg; p0 += kNegFetch) {
-                    if (p0 > 0) {
-#pragma unroll
-                        for (int u = 0; u < kNegFetch; ++u) {
-                            kk[u] = p0 + u < nneg
-                                        ? static_cast<int32_t>(pt_off + draw_index(ej, static_cast<uint32_t>(p0 + u), n32))
-                                        : -1;
-                            if (kk[u] >= 0) q[u] = load_point<T, D>(v.emb, kk[u]);
-                        }
-                    }
+                    if (p0 > 0) fetch(ej, p0, nneg, kk, q);
+                    // every slot is visited (no early exit), so that every
+                    // load of the batch is waited for within the trip; unused
+                    // slots hold -1, never equal to the row
 #pragma unroll
                     for (int u = 0; u < kNegFetch; ++u) {
-                        if (kk[u] < 0) break;
-                        ++draws;
-                        if (kk[u] == i) {
-                            ++self_hits;
-                            continue;
-                        }
+                        self_hits += kk[u] == i32 ? 1u : 0u;
+                        if (kk[u] < 0 || kk[u] == i32) continue;
                         const T e2 = sqdist<T, D>(h, q[u]);
                         const T rc = prod_div(two_gb, (static_cast<T>(0.001) + e2) * (v.a * fast_pow(e2, v.b) + 1));
                         double* rec = TRACE ? trace_slot(v, 1, i, kk[u]) : nullptr;
