@@ -443,7 +443,12 @@ def main_autograd(args, umx, torch, dev):
     steps = max(1, args.steps)
     t0 = time.perf_counter()
     for _ in range(steps):
+        # the same work as the warm-up pass: gradients replaced, not
+        # accumulated (a first accumulation would load torch's add kernel
+        # inside the timed region)
         y.grad = None
+        at.grad = None
+        bt.grad = None
         ev[0].record()
         loss = umap_loss(y, at, bt, heads, tails, coef=w, m=m, seed=2024, step=0)
         ev[1].record()
@@ -485,6 +490,13 @@ def main_autograd(args, umx, torch, dev):
     # gradient updates (read + write)
     bytes_bwd = E * (12.0 + (2 + m) * D * 4.0 + (2 + m) * D * 16.0) + n * D * (8.0 + 8.0 + 4.0)
     achieved = bytes_bwd / (bwd_ms / 1000.0) / 1e9
+    # the backward is bound by its device-scope f64 atomics (one per
+    # coordinate of each tail and negative sample, about one per coordinate
+    # of each head run): their rate beside the device's rate for random f64
+    # atomics into an array of the same size (torch index_add_)
+    atomics = E * (1 + m) * D + n * D
+    atom_peak = atomic_rate(torch, dev, n * D, atomics)
+    atom_rate = atomics / (bwd_ms / 1000.0)
 
     cpu = None
     if not args.no_cpu:
@@ -520,11 +532,36 @@ def main_autograd(args, umx, torch, dev):
                      "avg_launch_us": round(bwd_ms * 1000.0, 2), "algorithmic_bytes_per_launch": round(bytes_bwd)},
         "cpu_baseline": cpu,
         "loss_pass_ms": {"forward": round(fwd_ms, 3), "backward": round(bwd_ms, 3), "wall": round(pass_ms, 3)},
+        "backward_atomics": {"count": int(atomics), "rate_per_s": round(atom_rate, 1),
+                             "device_rate_per_s": round(atom_peak, 1), "frac": round(atom_rate / atom_peak, 4),
+                             "device_rate_how": "torch index_add_ of as many random float64 adds into an array "
+                                                "of n * num_dim doubles, best of 3"},
         "loss_edges_per_s": round(E / ((fwd_ms + bwd_ms) / 1000.0), 1),
         "layout_200_epochs_ms": {"plain": round(plain_s * 1000, 3), "autograd": round(auto_s * 1000, 3)},
         "knn15_preservation": {"plain": round(pres_plain, 4), "autograd": round(pres_auto, 4)},
     }
     print(json.dumps(out), flush=True)
+
+
+def atomic_rate(torch, dev, targets, count):
+    """Random float64 atomic adds per second of the device (torch index_add_
+    of 'count' ones at uniform random indices into 'targets' doubles)."""
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(7)
+    idx = torch.randint(0, targets, (count,), device=dev, generator=gen)
+    src = torch.ones(count, dtype=torch.float64, device=dev)
+    acc = torch.zeros(targets, dtype=torch.float64, device=dev)
+    best = None
+    for _ in range(4):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        acc.index_add_(0, idx, src)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    del idx, src, acc
+    return count / (best / 1000.0)
 
 
 def _cpu_model():
