@@ -48,6 +48,12 @@ const char* umx_last_error(void);
 /* ABI version (major * 10000 + minor * 100 + patch). */
 int umx_version(void);
 
+/* Returns to the driver the device buffers that the library keeps for reuse
+ * on the current device (freed graph, layout and kNN buffers, up to 8 GiB per
+ * device, reused by later allocations of about the same size).  No
+ * counterpart in the reference (host memory there is the C++ allocator's). */
+int umx_trim_memory(void);
+
 /* ------------------------------------------------------------------ */
 /* Fuzzy similarities                                                  */
 /* ------------------------------------------------------------------ */

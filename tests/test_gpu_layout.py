@@ -64,6 +64,41 @@ def test_schedule_and_tallies_bit_exact(umx, nobs, k, dtype, dim):
     assert np.all(np.isfinite(emb)) and not np.array_equal(emb, emb0)
 
 
+def test_reused_buffers_and_trim(umx):
+    """Freed library buffers are reused by later graphs and layouts (their old
+    contents must not leak into new results) and can be handed back: the
+    serial replay, which is deterministic, gives the same embedding, schedule
+    and tallies on fresh, reused and trimmed buffers."""
+    x, rp, idx, w = _graph(200, 15, np.float32)
+    T, nsr, seed = 60, 5.0, 9
+
+    def run_once():
+        g = umx.Graph(umx.NeighborList(rp, idx, w.copy()))
+        g.combine(1.0)
+        lay = umx.Layout(graph=g, num_epochs=T, negative_sample_rate=nsr)
+        lay.set_params(2, A, B, 1.0, 1.0, seed)
+        lay.set_embedding(np.ascontiguousarray(x[:, :2], dtype=np.float32))
+        lay.run_serial(20)
+        lay.run(T)
+        ed = lay.epoch_data()
+        return lay.get_embedding(), ed.epoch_of_next_sample, ed.epoch_of_next_negative_sample, lay.tallies()
+
+    first = run_once()
+    # a differently sized problem in between dirties the cached blocks
+    xo, rpo, idxo, wo = _graph(100, 10, np.float32)
+    lo = umx.Layout(csr=umx.NeighborList(rpo, idxo, wo), num_epochs=30, negative_sample_rate=nsr)
+    lo.set_params(2, A, B, 1.0, 1.0, 3)
+    lo.set_embedding(np.ascontiguousarray(xo[:, :2], dtype=np.float32))
+    lo.run(30)
+    del lo
+    second = run_once()
+    umx.trim_memory()
+    third = run_once()
+    for a_, b_, c_ in zip(first[1:], second[1:], third[1:]):
+        assert np.array_equal(a_, b_) and np.array_equal(a_, c_)
+    assert np.all(np.isfinite(first[0])) and np.all(np.isfinite(third[0]))
+
+
 def _check_steps(trace, counts, dim, dtype, a, b, gamma, T, first_epoch, rtol, atol):
     """Every traced step's head move equals the oracle's step function applied
     to the traced coordinates before the step; returns the worst error."""

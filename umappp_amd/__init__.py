@@ -63,6 +63,7 @@ def _declare(L):
     sig = {
         "umx_last_error": (C.c_char_p, []),
         "umx_version": (C.c_int, []),
+        "umx_trim_memory": (C.c_int, []),
         "umx_neighbor_similarities": (C.c_int, [C.c_int, C.c_int64, _vp, _vp, C.c_double, C.c_double, C.c_double,
                                                 C.c_int, _vp, _vp]),
         "umx_combine_neighbor_sets": (C.c_int, [C.c_int, C.c_int64, _vp, _vp, _vp, C.c_double, C.c_int64, _vp, _vp,
@@ -575,6 +576,12 @@ def similarities_to_epochs(x: NeighborList, num_epochs: int, negative_sample_rat
     """optimize_layout.hpp:47."""
     lay = Layout(csr=x, num_epochs=num_epochs, negative_sample_rate=negative_sample_rate)
     return lay.epoch_data()
+
+
+def trim_memory() -> None:
+    """Returns the device buffers the library keeps for reuse on the current
+    device to the driver (umx_trim_memory)."""
+    _check(lib().umx_trim_memory())
 
 
 def find_ab(spread: float, min_dist: float):

@@ -35,10 +35,38 @@ void require(bool ok, int code, const char* msg) {
     if (!ok) throw Error(code, msg);
 }
 
+// Streams of graphs, layouts and kNN calls: one non-blocking stream of
+// default priority per device, created on first use and kept for the
+// process (creating a stream costs milliseconds, and a caller may create a
+// graph and a layout per run).  Every entry point waits for its own work
+// before returning, so objects that share the stream never see each other's
+// work in flight; calls from several threads serialise on it.
+struct StreamPool {
+    std::mutex mu;
+    std::map<int, hipStream_t> of_device;
+};
+StreamPool& stream_pool() {
+    static StreamPool* p = new StreamPool();  // never destroyed: kept for the process
+    return *p;
+}
+
 hipStream_t make_stream() {
+    int dev = 0;
+    UMX_HIP(hipGetDevice(&dev));
+    StreamPool& pool = stream_pool();
+    std::lock_guard<std::mutex> lock(pool.mu);
+    auto it = pool.of_device.find(dev);
+    if (it != pool.of_device.end()) return it->second;
     hipStream_t s = nullptr;
     UMX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    pool.of_device[dev] = s;
     return s;
+}
+
+// Waits for the stream's queued work (its owner's buffers are freed next,
+// and device_free needs no queued work to use them); the stream is kept.
+void release_stream(hipStream_t s) {
+    if (s) (void)hipStreamSynchronize(s);
 }
 
 __global__ void k_check_rowptr(const int64_t* rp, int64_t n, int64_t nnz, int* bad) {
@@ -88,16 +116,118 @@ void keep_pool_memory() {
 
 void umx::prepare_pool() { keep_pool_memory(); }
 
+namespace {
+
+// Freed library buffers kept per device for reuse (up to kCacheMax bytes):
+// hipFreeAsync of a large block takes milliseconds, and a caller that builds
+// a graph and a layout per run frees and allocates the same sizes each time.
+// Blocks are used and freed on the device's library stream only, so a reused
+// block's earlier work is ordered before its new work.
+constexpr size_t kCacheMax = size_t{8} << 30;
+
+struct BlockCache {
+    std::multimap<size_t, void*> free_blocks;  // size -> block
+    std::map<void*, size_t> owned;  // block -> size, cached or in use
+    size_t cached = 0;  // bytes in free_blocks
+};
+std::mutex& cache_mutex() {
+    static std::mutex* m = new std::mutex();
+    return *m;
+}
+std::map<int, BlockCache>& block_caches() {
+    static auto* c = new std::map<int, BlockCache>();  // never destroyed: blocks live for the process
+    return *c;
+}
+
+size_t round_block(size_t bytes) {
+    const size_t mib = size_t{1} << 20;
+    return bytes <= mib ? (bytes + 255) & ~size_t{255} : (bytes + mib - 1) & ~(mib - 1);
+}
+
+hipStream_t stream_or_null() {
+    try {
+        return make_stream();
+    } catch (...) {
+        return nullptr;
+    }
+}
+
+}  // namespace
+
 void* umx::device_alloc(size_t bytes) {
     keep_pool_memory();
+    int dev = 0;
+    UMX_HIP(hipGetDevice(&dev));
+    const hipStream_t s = make_stream();
+    const size_t want = round_block(bytes == 0 ? 1 : bytes);
     void* p = nullptr;
-    UMX_HIP(hipMallocAsync(&p, bytes, nullptr));
-    UMX_HIP(hipStreamSynchronize(nullptr));
+    {
+        std::lock_guard<std::mutex> lock(cache_mutex());
+        BlockCache& bc = block_caches()[dev];
+        auto it = bc.free_blocks.lower_bound(want);
+        if (it != bc.free_blocks.end() && it->first <= want + want / 4) {
+            p = it->second;
+            bc.cached -= it->first;
+            bc.free_blocks.erase(it);
+        }
+    }
+    if (!p) {
+        UMX_HIP(hipMallocAsync(&p, want, s));
+        std::lock_guard<std::mutex> lock(cache_mutex());
+        block_caches()[dev].owned[p] = want;
+    }
+    // ready on every stream: the block's earlier work (if reused) is done
+    UMX_HIP(hipStreamSynchronize(s));
     return p;
 }
 
+// Keeps the block for reuse while the device's cache holds less than
+// kCacheMax bytes, else frees it on the library stream (not the legacy
+// default stream, whose frees wait for the whole device).
 void umx::device_free(void* p) {
-    if (p) (void)hipFreeAsync(p, nullptr);
+    if (!p) return;
+    {
+        std::lock_guard<std::mutex> lock(cache_mutex());
+        for (auto& kv : block_caches()) {
+            BlockCache& bc = kv.second;
+            auto it = bc.owned.find(p);
+            if (it == bc.owned.end()) continue;
+            const size_t sz = it->second;
+            if (bc.cached + sz <= kCacheMax) {
+                bc.free_blocks.emplace(sz, p);
+                bc.cached += sz;
+                return;
+            }
+            bc.owned.erase(it);
+            break;
+        }
+    }
+    (void)hipFreeAsync(p, stream_or_null());
+}
+
+// Returns the current device's cached blocks to the driver.
+int umx_trim_memory(void) {
+    return guard([&] {
+        int dev = 0;
+        UMX_HIP(hipGetDevice(&dev));
+        const hipStream_t s = make_stream();
+        std::vector<void*> blocks;
+        {
+            std::lock_guard<std::mutex> lock(cache_mutex());
+            BlockCache& bc = block_caches()[dev];
+            for (auto& kv : bc.free_blocks) {
+                blocks.push_back(kv.second);
+                bc.owned.erase(kv.second);
+            }
+            bc.free_blocks.clear();
+            bc.cached = 0;
+        }
+        for (void* b : blocks) UMX_HIP(hipFreeAsync(b, s));
+        UMX_HIP(hipStreamSynchronize(s));
+        hipMemPool_t pool = nullptr;
+        UMX_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
+        UMX_HIP(hipMemPoolTrimTo(pool, 0));
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -129,7 +259,7 @@ struct umx_graph {
     void* val() { return dtype == UMX_F32 ? static_cast<void*>(vf.p) : static_cast<void*>(vd.p); }
     const void* val() const { return dtype == UMX_F32 ? static_cast<const void*>(vf.p) : static_cast<const void*>(vd.p); }
     ~umx_graph() {
-        if (s) (void)hipStreamDestroy(s);
+        release_stream(s);
     }
 };
 
@@ -242,7 +372,7 @@ int umx_knn_bruteforce(int dtype, int64_t n, int dim, const void* data, int k, i
         hipStream_t s = make_stream();
         struct StreamGuard {
             hipStream_t s;
-            ~StreamGuard() { (void)hipStreamDestroy(s); }
+            ~StreamGuard() { release_stream(s); }
         } sg{s};
         const size_t es = dsize(dtype);
         const size_t xbytes = es * static_cast<size_t>(n) * dim, nk = static_cast<size_t>(n) * k;
@@ -565,11 +695,9 @@ struct umx_layout {
     int gexec_block = 0;
     std::vector<unsigned char> gexec_key;
 
-    ~umx_layout() {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        for (auto e : events) (void)hipEventDestroy(e);
-        if (s) (void)hipStreamDestroy(s);
-    }
+    int events_device = -1;  // device the events were created on
+
+    ~umx_layout();
     int n_inst() const { return static_cast<int>(inst_off.size()) - 1; }
     umx::Instances instances() const {
         umx::Instances q;
@@ -852,9 +980,37 @@ void layout_from_graph(umx_layout* L, const umx_graph* g, const std::vector<int3
     UMX_HIP(hipStreamSynchronize(L->s));
 }
 
+// Timing events are kept in a process-wide pool per device and handed back
+// when a layout is destroyed: creating them costs several microseconds each,
+// and a timed run of 200 epochs needs 400 (a bench step creates a fresh
+// layout).
+struct EventPool {
+    std::mutex mu;
+    std::map<int, std::vector<hipEvent_t>> free;
+};
+EventPool& event_pool() {
+    static EventPool* p = new EventPool();  // never destroyed: events outlive static teardown order
+    return *p;
+}
+
 void prepare_events(umx_layout* L, int count) {
     if (!L->timing) return;
-    while (static_cast<int>(L->events.size()) < 2 * count) {
+    const size_t want = 2 * static_cast<size_t>(count);
+    if (L->events.size() >= want) return;
+    int dev = 0;
+    UMX_HIP(hipGetDevice(&dev));
+    if (L->events_device < 0) L->events_device = dev;
+    require(L->events_device == dev, UMX_ERR_ARGUMENT, "timed runs of one layout must use one device");
+    {
+        EventPool& pool = event_pool();
+        std::lock_guard<std::mutex> lock(pool.mu);
+        std::vector<hipEvent_t>& fr = pool.free[dev];
+        while (L->events.size() < want && !fr.empty()) {
+            L->events.push_back(fr.back());
+            fr.pop_back();
+        }
+    }
+    while (L->events.size() < want) {
         hipEvent_t e;
         UMX_HIP(hipEventCreate(&e));
         L->events.push_back(e);
@@ -971,6 +1127,18 @@ void check_epoch_args(int num_epochs, double nsr, int dtype) {
 }
 
 }  // namespace
+
+umx_layout::~umx_layout() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (!events.empty()) {
+        EventPool& pool = event_pool();
+        std::lock_guard<std::mutex> lock(pool.mu);
+        std::vector<hipEvent_t>& fr = pool.free[events_device];
+        fr.insert(fr.end(), events.begin(), events.end());
+    }
+    release_stream(s);
+}
+
 
 extern "C" {
 

@@ -15,8 +15,8 @@
 // fire mask per lane, and walks the firing slots with count-trailing-zeros:
 // on every trip each lane applies one whole firing edge (pull mode: head
 // step, reverse-edge tail step, negative samples, exactly as in
-// k_layout_pull), fetching the tail and up to kNegPrefetch negative samples
-// in one batch of independent loads.  Updated schedule values go back to HBM
+// k_layout_pull), fetching the tail and kNegFetch negative samples in one
+// batch of unconditional independent loads.  Updated schedule values go back to HBM
 // with coalesced stores.  Schedule loads and stores are non-temporal, so the
 // stream of schedule bytes does not evict from the L2 the embedding that the
 // gathers read.  The CSR arrays remain the canonical schedule state;
@@ -44,7 +44,9 @@ namespace {
 
 using namespace umx::dev;
 
-constexpr int kNegFetch = 8;
+// Negative samples fetched per batch: the reference's default rate (5) is
+// one batch for most firing edges; a few draw a sixth in a second batch.
+constexpr int kNegFetch = 5;
 
 // ---------------------------------------------------------------------------
 // Build: chunk widths, chunk offsets, constant slot data and the slot map.
