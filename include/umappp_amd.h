@@ -304,7 +304,9 @@ void umx_layout_destroy(umx_layout* L);
  * is skipped.  ab = device double[2] {a, b}.  All pointers are GPU pointers;
  * work is queued on 'stream' (a hipStream_t; NULL = the default stream) and
  * the call returns once it is done.  An edge index outside [0, n) is
- * UMX_ERR_GRAPH.  Arithmetic is in double.
+ * UMX_ERR_GRAPH.  Arithmetic is in double.  An edge with c_e = 0 contributes
+ * nothing and its terms are not evaluated (a caller can pass a whole edge
+ * list with a 0/1 mask as coef instead of compacting it).
  *
  * forward: *loss (device double) = L.
  * backward: grad_embedding (float[n * num_dim]) = grad_out * dL/dy and

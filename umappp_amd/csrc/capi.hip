@@ -144,6 +144,26 @@ size_t round_block(size_t bytes) {
     return bytes <= mib ? (bytes + 255) & ~size_t{255} : (bytes + mib - 1) & ~(mib - 1);
 }
 
+// Frees the device's cached blocks on stream s and trims the driver's pool.
+void trim_cache(int dev, hipStream_t s) {
+    std::vector<void*> blocks;
+    {
+        std::lock_guard<std::mutex> lock(cache_mutex());
+        BlockCache& bc = block_caches()[dev];
+        for (auto& kv : bc.free_blocks) {
+            blocks.push_back(kv.second);
+            bc.owned.erase(kv.second);
+        }
+        bc.free_blocks.clear();
+        bc.cached = 0;
+    }
+    for (void* b : blocks) UMX_HIP(hipFreeAsync(b, s));
+    UMX_HIP(hipStreamSynchronize(s));
+    hipMemPool_t pool = nullptr;
+    UMX_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
+    UMX_HIP(hipMemPoolTrimTo(pool, 0));
+}
+
 hipStream_t stream_or_null() {
     try {
         return make_stream();
@@ -172,7 +192,14 @@ void* umx::device_alloc(size_t bytes) {
         }
     }
     if (!p) {
-        UMX_HIP(hipMallocAsync(&p, want, s));
+        hipError_t e = hipMallocAsync(&p, want, s);
+        if (e == hipErrorOutOfMemory) {
+            // the cache may hold what the driver now lacks: hand it back, retry once
+            (void)hipGetLastError();
+            trim_cache(dev, s);
+            e = hipMallocAsync(&p, want, s);
+        }
+        UMX_HIP(e);
         std::lock_guard<std::mutex> lock(cache_mutex());
         block_caches()[dev].owned[p] = want;
     }
@@ -210,23 +237,7 @@ int umx_trim_memory(void) {
     return guard([&] {
         int dev = 0;
         UMX_HIP(hipGetDevice(&dev));
-        const hipStream_t s = make_stream();
-        std::vector<void*> blocks;
-        {
-            std::lock_guard<std::mutex> lock(cache_mutex());
-            BlockCache& bc = block_caches()[dev];
-            for (auto& kv : bc.free_blocks) {
-                blocks.push_back(kv.second);
-                bc.owned.erase(kv.second);
-            }
-            bc.free_blocks.clear();
-            bc.cached = 0;
-        }
-        for (void* b : blocks) UMX_HIP(hipFreeAsync(b, s));
-        UMX_HIP(hipStreamSynchronize(s));
-        hipMemPool_t pool = nullptr;
-        UMX_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
-        UMX_HIP(hipMemPoolTrimTo(pool, 0));
+        trim_cache(dev, make_stream());
     });
 }
 

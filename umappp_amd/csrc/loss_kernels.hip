@@ -145,6 +145,8 @@ __global__ void __launch_bounds__(kLossBlock) k_loss_forward(LossView v, double*
          e += static_cast<int64_t>(gridDim.x) * kLossBlock) {
         int64_t i, j;
         if (!edge_ok(v, e, i, j)) continue;
+        const double c = v.coef ? static_cast<double>(v.coef[e]) : 1.0;
+        if (c == 0) continue;  // every term of the edge is scaled by c
         const int m = v.nneg ? v.nneg[e] : v.m;
         const uint64_t ej = edge_key(v.ekey, v.keys ? static_cast<uint64_t>(v.keys[e]) : static_cast<uint64_t>(e));
         const DVec<D> yi = load_d<D>(v.emb, i, v.dim);
@@ -167,7 +169,7 @@ __global__ void __launch_bounds__(kLossBlock) k_loss_forward(LossView v, double*
                 t += v.gamma * (log1p(a * exp(b * ls)) - log_a - b * ls);
             }
         }
-        acc += (v.coef ? static_cast<double>(v.coef[e]) : 1.0) * t;
+        acc += c * t;
     }
     const double tot = block_sum(acc, lds);
     if (threadIdx.x == 0 && tot != 0) unsafeAtomicAdd(loss, tot);
@@ -238,6 +240,10 @@ __global__ void __launch_bounds__(kLossBlock) k_loss_backward(LossView v, double
             continue;
         }
         const double c = v.coef ? static_cast<double>(v.coef[e]) : 1.0;
+        if (c == 0) {  // every term of the edge, and its gradient, is scaled by c
+            add_head_segmented<D>(g64, -1, dim, DVec<D>{});
+            continue;
+        }
         const int m = v.nneg ? v.nneg[e] : v.m;
         const uint64_t ej = edge_key(v.ekey, v.keys ? static_cast<uint64_t>(v.keys[e]) : static_cast<uint64_t>(e));
         const DVec<D> yi = load_d<D>(v.emb, i, dim);
