@@ -105,6 +105,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU layout sample time per mode")
     ap.add_argument("--skip-sequential", action="store_true",
                     help="c4 only: time the batched run alone (for profiling the batched kernel)")
+    ap.add_argument("--quality", action="store_true",
+                    help="c3/c2 only: also run the oracle's serial reference layout over all epochs on the host "
+                         "(minutes at 1M points) and compare final 15-NN preservation and trustworthiness")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device-to-device copy three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -240,7 +243,80 @@ def main():
     }
     if cpu and cpu.get("value"):
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+    if args.quality:
+        out["quality"] = quality_vs_reference(x, kidx, kdist, emb0, emb, n, k, T, nsr, a, b, torch, dev)
     print(json.dumps(out), flush=True)
+
+
+def quality_vs_reference(x, kidx, kdist, emb0, emb_gpu, n, k, T, nsr, a, b, torch, dev):
+    """Final-embedding quality of the GPU step against the oracle's serial
+    reference layout (the reference's loop and mt19937_64 sampler, float
+    path) from the same kNN lists, initial embedding and seed: 15-NN
+    preservation on 20,000 and trustworthiness on 2,000 seeded query points,
+    both computed on the GPU."""
+    import oracle
+
+    cores = min(os.cpu_count() or 1, 16)
+    rp = np.arange(n + 1, dtype=np.int64) * k
+    w, _, _ = oracle.neighbor_similarities(rp, kdist.cpu().numpy().reshape(-1).copy(), nthreads=cores)
+    crp, cidx, cw = oracle.combine_neighbor_sets(rp, kidx.cpu().numpy().reshape(-1).copy(), w, 1.0)
+    ep = oracle.similarities_to_epochs(crp, cidx, cw, T, nsr)
+    lay = oracle.Layout(ep, T, nsr, rng_kind=oracle.RNG_MT, seed=1234567890, dtype=np.float32)
+    eo = emb0.cpu().numpy().copy()
+    t0 = time.perf_counter()
+    for e in range(10, T + 10, 10):
+        lay.run(eo, a, b, epoch_limit=min(e, T))
+        log(f"reference layout on the host: epoch {min(e, T)}/{T} ({time.perf_counter() - t0:.0f}s)")
+    ref_s = time.perf_counter() - t0
+    xh = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).to(dev)
+    y_ref = torch.from_numpy(eo).to(dev)
+    rng = np.random.default_rng(11)
+    qp = torch.from_numpy(rng.choice(n, 20000, replace=False)).to(dev)
+    qt = torch.from_numpy(rng.choice(n, 2000, replace=False)).to(dev)
+    res = {"reference_layout_s": round(ref_s, 1), "preservation_queries": 20000, "trustworthiness_queries": 2000}
+    for tag, y in (("gpu", emb_gpu), ("reference", y_ref)):
+        res[f"knn15_preservation_{tag}"] = round(preservation_subset(kidx, y, 15, qp, torch), 4)
+        res[f"trustworthiness_{tag}"] = round(trustworthiness_gpu(xh, y, 15, qt, torch), 4)
+    log(f"quality: {res}")
+    return res
+
+
+def knn_low_subset(y, k, q, torch):
+    """k nearest neighbours (self excluded) of the query rows q of a 2-D embedding."""
+    sq = (y * y).sum(1)
+    d = sq[q, None] + sq[None, :] - 2.0 * (y[q] @ y.T)
+    d[torch.arange(q.numel(), device=y.device), q] = float("inf")
+    return torch.topk(d, k, dim=1, largest=False).indices
+
+
+def preservation_subset(hi_idx, y, k, q, torch):
+    hits = torch.zeros((), dtype=torch.float64, device=y.device)
+    for s in range(0, q.numel(), 2048):
+        qq = q[s:s + 2048]
+        lo = knn_low_subset(y, k, qq, torch)
+        hits += (hi_idx[qq, :k].long()[:, :, None] == lo[:, None, :]).any(2).sum()
+    return float(hits) / (q.numel() * k)
+
+
+def trustworthiness_gpu(xh, y, k, q, torch):
+    """Trustworthiness (Venna & Kaski) at k over the query rows q: ranks of
+    each query's k embedding neighbours among its high-dimensional
+    distances (self excluded)."""
+    n = xh.shape[0]
+    sqh = (xh * xh).sum(1)
+    penalty = 0.0
+    for s in range(0, q.numel(), 128):
+        qq = q[s:s + 128]
+        lo = knn_low_subset(y, k, qq, torch)
+        d = sqh[qq, None] + sqh[None, :] - 2.0 * (xh[qq] @ xh.T)
+        d[torch.arange(qq.numel(), device=xh.device), qq] = -float("inf")  # self ranks 0
+        order = torch.argsort(d, dim=1, stable=True)
+        ranks = torch.empty_like(order)
+        ranks.scatter_(1, order, torch.arange(n, device=xh.device).expand(qq.numel(), n))
+        r = ranks.gather(1, lo)
+        penalty += float(torch.clamp(r - k, min=0).sum())
+    m = q.numel()
+    return 1.0 - 2.0 / (m * k * (2.0 * n - 3.0 * k - 1.0)) * penalty
 
 
 def knn_low(y, k, torch):
