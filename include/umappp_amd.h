@@ -225,7 +225,7 @@ int umx_layout_set_mode(umx_layout* L, int mode);
  * to fire more than fires_per_item edges per epoch (sum of
  * 1 / epochs_per_sample), or longer than edges_per_item, are cut into items
  * of about equal expected firings whose moves of the head are merged with
- * atomics; items are run heaviest first.  Defaults 32.0 and 2^31 - 1. */
+ * atomics; items are run longest first.  Defaults 32.0 and 2^31 - 1. */
 int umx_layout_set_work_split(umx_layout* L, double fires_per_item, int edges_per_item);
 /* LDS tile of the pull kernel: schedule columns staged per wavefront at a
  * time (4, 8, 16 or 32; default 8). */
@@ -240,7 +240,8 @@ int umx_layout_set_tile(umx_layout* L, int tile_cols);
 int umx_layout_set_placement(umx_layout* L, int waves, int groups);
 /* Epochs per captured HIP graph: umx_layout_run records 'epochs' launches
  * of the production kernel once (each launch reads the epoch from a counter
- * in device memory and a one-thread kernel advances it, so one graph serves
+ * in device memory, and the one-wavefront kernel that folds the epoch's
+ * tallies advances it, so one graph serves
  * every block of epochs) and replays it; 0 launches every epoch directly.
  * Default 0; timing and tracing always launch directly. */
 int umx_layout_set_graph_block(umx_layout* L, int epochs);
