@@ -1,0 +1,73 @@
+"""Edge cases of the layout through the C ABI, against the oracle.
+
+* Ragged graphs (empty rows, hub targets with many mirrors) after the fuzzy
+  union, with rows split over several work items (a low split threshold on
+  the hub graphs), chunks of the production kernel mixing lengths and
+  carrying padding.  Schedule and tallies stay bit-exact after T epochs, in
+  both update modes and several dimensions, and observations without edges
+  never move.
+* Degenerate sizes (no observations, one observation, no edges at all): the
+  whole path runs, moves nothing and counts nothing.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from _data import random_ragged_graph
+
+pytestmark = pytest.mark.gpu
+
+A, B = 1.8956, 0.8006
+
+
+@pytest.mark.parametrize("seed,n,maxk,hubs", [(1, 3000, 40, 4), (2, 800, 5, 2), (3, 2000, 90, 0)])
+@pytest.mark.parametrize("mode,dim", [("pull", 2), ("pull", 3), ("push", 2)])
+def test_ragged_layout_schedule_bit_exact(umx, seed, n, maxk, hubs, mode, dim):
+    rp, idx, d = random_ragged_graph(seed, n, maxk, np.float32, hubs)
+    w, _, _ = oracle.neighbor_similarities(rp, d)
+    crp, cidx, cw = oracle.combine_neighbor_sets(rp, idx, w, 1.0)
+    T, nsr, s = 60, 5.0, 11
+    lay = umx.Layout(csr=umx.NeighborList(crp, cidx, cw), num_epochs=T, negative_sample_rate=nsr)
+    ep = oracle.similarities_to_epochs(crp, cidx, cw, T, nsr)
+    orc = oracle.Layout(ep, T, nsr, rng_kind=oracle.RNG_COUNTER, seed=s, dtype=np.float32)
+    emb0 = np.random.default_rng(seed).uniform(-5, 5, size=(n, dim)).astype(np.float32)
+    lay.set_params(dim, A, B, 1.0, 1.0, s)
+    lay.set_mode(umx.Layout.PULL if mode == "pull" else umx.Layout.PUSH)
+    if hubs:
+        lay.set_work_split(4.0)  # rows expected to fire more than 4 edges per epoch become several items
+    lay.set_embedding(emb0)
+    lay.run(T)
+    eo = emb0.copy()
+    orc.run(eo, A, B)
+    ed = lay.epoch_data()
+    nx, nn, cur, tal = orc.state()
+    assert ed.current_epoch == T == cur
+    assert np.array_equal(ed.epoch_of_next_sample, nx)
+    assert np.array_equal(ed.epoch_of_next_negative_sample, nn)
+    assert np.array_equal(lay.tallies(), tal)
+    emb = lay.get_embedding()
+    assert np.all(np.isfinite(emb))
+    # an observation with no edges is never a head nor a tail
+    lonely = np.diff(crp) == 0
+    assert np.array_equal(emb[lonely], emb0[lonely])
+    if hubs:
+        _, _, _, split = lay.work_items()
+        assert split.any()
+
+
+@pytest.mark.parametrize("n", [0, 1, 5])
+def test_graph_without_edges(umx, n):
+    rp = np.zeros(n + 1, np.int64)
+    g = umx.Graph(umx.NeighborList(rp, np.zeros(0, np.int32), np.zeros(0, np.float32)))
+    g.neighbor_similarities()
+    g.combine(1.0)
+    nl = g.export()
+    assert nl.row_ptr.size == n + 1 and nl.idx.size == 0
+    lay = umx.Layout(graph=g, num_epochs=10, negative_sample_rate=5.0)
+    emb0 = np.arange(n * 2, dtype=np.float32).reshape(n, 2)
+    lay.set_params(2, A, B, 1.0, 1.0, 3)
+    lay.set_embedding(emb0)
+    lay.run(10)
+    assert np.array_equal(lay.get_embedding(), emb0)
+    assert int(lay.tallies().sum()) == 0
+    assert lay.epoch_data().current_epoch == 10
