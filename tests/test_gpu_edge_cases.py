@@ -71,3 +71,41 @@ def test_graph_without_edges(umx, n):
     assert np.array_equal(lay.get_embedding(), emb0)
     assert int(lay.tallies().sum()) == 0
     assert lay.epoch_data().current_epoch == 10
+
+
+def test_batch_with_degenerate_and_ragged_instances(umx):
+    """A batch mixing an ordinary kNN instance, a ragged one with hubs, and
+    instances with 0, 1 and 5 observations and no edges: every instance's
+    schedule equals the oracle's run of it alone, tallies add up, and
+    observations without edges never move."""
+    from test_gpu_batch import _check_schedule, _instances, _split
+
+    (_, rp0, idx0, d0), = _instances([300], 10, 900, np.float32)
+    rp1, idx1, d1 = random_ragged_graph(4, 400, 30, np.float32, 3)
+    empty = lambda m: umx.NeighborList(np.zeros(m + 1, np.int64), np.zeros(0, np.int32), np.zeros(0, np.float32))
+    lists = [umx.NeighborList(rp0, idx0, d0), empty(0), umx.NeighborList(rp1, idx1, d1), empty(1), empty(5)]
+    g = umx.Graph.batch(lists)
+    g.neighbor_similarities()
+    g.combine(1.0)
+    off = g.instances()
+    assert np.array_equal(off, [0, 300, 300, 700, 701, 706])
+    parts = _split(g.export(), off)
+    epochs = [40, 10, 35, 20, 40]
+    seeds = [5, 6, 7, 8, 9]
+    lay = umx.Layout(graph=g, num_epochs=np.asarray(epochs, np.int32), negative_sample_rate=5.0)
+    emb0 = np.random.default_rng(8).uniform(-10, 10, size=(int(off[-1]), 2)).astype(np.float32)
+    lay.set_params(2, A, B, 1.0, 1.0, 1)
+    lay.set_instance_seeds(seeds)
+    lay.set_embedding(emb0)
+    orcs = []
+    for q, (rp, idx, w) in enumerate(parts):
+        ep = oracle.similarities_to_epochs(rp, idx, w, epochs[q], 5.0)
+        orcs.append((ep, epochs[q], oracle.Layout(ep, epochs[q], 5.0, rng_kind=oracle.RNG_COUNTER, seed=seeds[q],
+                                                  dtype=np.float32)))
+    lay.run(40)
+    _check_schedule(lay, off, orcs, emb0, 40)
+    emb = lay.get_embedding()
+    ed = lay.epoch_data()
+    lonely = np.diff(ed.cumulative_num_edges) == 0
+    assert lonely[off[3]:].all()
+    assert np.array_equal(emb[lonely], emb0[lonely]) and np.all(np.isfinite(emb))
