@@ -538,13 +538,15 @@ __global__ void __launch_bounds__(kBlock) k_stage_mirrors(int64_t n, int G, cons
 // them in arbitrary order; keys are unique within a row, one per source row).
 // Rows with at most kSmallMirrors entries: a group of kSmallMirrors lanes per
 // row ranks its keys with shuffles.  Longer rows are listed for
-// k_sort_mirrors_big.
+// k_sort_mirrors_mid (up to 64 entries) or k_sort_mirrors_big (more).
 constexpr int kSmallMirrors = 32;
 constexpr int kBigSortCap = 8192;  // LDS bitonic sort capacity per row
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_sort_mirrors_small(int64_t n, const int64_t* __restrict__ inc_rp,
                                                              int32_t* __restrict__ src, T* __restrict__ val,
+                                                             int64_t* __restrict__ mid_rows,
+                                                             unsigned long long* __restrict__ n_mid,
                                                              int64_t* __restrict__ big_rows,
                                                              unsigned long long* __restrict__ n_big) {
     constexpr int G = kSmallMirrors;
@@ -556,7 +558,10 @@ __global__ void __launch_bounds__(kBlock) k_sort_mirrors_small(int64_t n, const 
     const int64_t ss = row < n ? inc_rp[row] : 0;
     const int64_t m = row < n ? inc_rp[row + 1] - ss : 0;
     const bool small = m <= G;
-    if (!small && lane == 0) big_rows[atomicAdd(n_big, 1ULL)] = row;
+    if (!small && lane == 0) {
+        if (m <= kWave) mid_rows[atomicAdd(n_mid, 1ULL)] = row;
+        else big_rows[atomicAdd(n_big, 1ULL)] = row;
+    }
     const bool mine = small && lane < m;
     const int32_t key = mine ? src[ss + lane] : 0x7fffffff;
     const T v = mine ? val[ss + lane] : static_cast<T>(0);
@@ -573,9 +578,35 @@ __global__ void __launch_bounds__(kBlock) k_sort_mirrors_small(int64_t n, const 
     }
 }
 
-// One workgroup per long row: bitonic sort of (key, value) in LDS.  Rows
-// beyond kBigSortCap entries are ranked in global memory instead
-// (quadratic, for extreme hubs only).
+// Rows with 33 to 64 entries: one wavefront per row, the same shuffle rank
+// sort as k_sort_mirrors_small over 64 lanes (every load of the row
+// completes before the first store, so the row is sorted in place).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_sort_mirrors_mid(const int64_t* __restrict__ mid_rows,
+                                                           unsigned long long count,
+                                                           const int64_t* __restrict__ inc_rp,
+                                                           int32_t* __restrict__ src, T* __restrict__ val) {
+    const int64_t w = static_cast<int64_t>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (w >= static_cast<int64_t>(count)) return;  // uniform per wavefront
+    const int64_t row = mid_rows[w];
+    const int64_t ss = inc_rp[row];
+    const int64_t m = inc_rp[row + 1] - ss;
+    const bool mine = lane < m;
+    const int32_t key = mine ? src[ss + lane] : 0x7fffffff;
+    const T v = mine ? val[ss + lane] : static_cast<T>(0);
+    int rank = 0;
+#pragma unroll
+    for (int o = 0; o < kWave; ++o) rank += __shfl(key, o, kWave) < key;
+    if (mine) {
+        src[ss + rank] = key;
+        val[ss + rank] = v;
+    }
+}
+
+// One workgroup per long row (more than 64 entries): bitonic sort of
+// (key, value) in LDS.  Rows beyond kBigSortCap entries are ranked in global
+// memory instead (quadratic, for extreme hubs only).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_sort_mirrors_big(const int64_t* __restrict__ big_rows,
                                                            const int64_t* __restrict__ inc_rp,
@@ -988,15 +1019,23 @@ void run_combine(int64_t n, const int64_t* rp, const int32_t* idx, const T* val,
         UMX_HIP(hipGetLastError());
     }
     if (n_inc > 0) {
+        Tmp<int64_t> mid_rows(static_cast<size_t>(n), s);
         Tmp<int64_t> big_rows(static_cast<size_t>(n), s);
-        Tmp<unsigned long long> n_big(1, s);
-        UMX_HIP(hipMemsetAsync(n_big.p, 0, sizeof(unsigned long long), s));
+        Tmp<unsigned long long> counts(2, s);  // mid, big
+        UMX_HIP(hipMemsetAsync(counts.p, 0, 2 * sizeof(unsigned long long), s));
         k_sort_mirrors_small<T><<<grid_for(n * kSmallMirrors), kBlock, 0, s>>>(n, inc_rp.p, stage_src.p, stage_val.p,
-                                                                               big_rows.p, n_big.p);
+                                                                               mid_rows.p, counts.p, big_rows.p,
+                                                                               counts.p + 1);
         UMX_HIP(hipGetLastError());
-        unsigned long long hbig = 0;
-        UMX_HIP(hipMemcpyAsync(&hbig, n_big.p, sizeof(hbig), hipMemcpyDeviceToHost, s));
+        unsigned long long hc[2] = {0, 0};
+        UMX_HIP(hipMemcpyAsync(hc, counts.p, sizeof(hc), hipMemcpyDeviceToHost, s));
         UMX_HIP(hipStreamSynchronize(s));
+        const unsigned long long hmid = hc[0], hbig = hc[1];
+        if (hmid > 0) {
+            const unsigned mid_grid = static_cast<unsigned>((hmid + kBlock / kWave - 1) / (kBlock / kWave));
+            k_sort_mirrors_mid<T><<<mid_grid, kBlock, 0, s>>>(mid_rows.p, hmid, inc_rp.p, stage_src.p, stage_val.p);
+            UMX_HIP(hipGetLastError());
+        }
         if (hbig > 0) {
             Tmp<int32_t> tsrc(static_cast<size_t>(n_inc), s);
             Tmp<T> tval(static_cast<size_t>(n_inc), s);
