@@ -98,6 +98,57 @@ def test_gradient_with_shuffled_edges(torch_dev):
     assert np.all(np.abs(g - g_ref) <= 1e-6 * np.abs(g_ref) + 1e-7 * scale)
 
 
+@pytest.mark.parametrize("n,deg", [(401, 7), (333, 5), (70, 3)])
+def test_gradient_partial_last_wavefront(torch_dev, n, deg):
+    """Edges in row order whose count is not a multiple of 64: the last
+    wavefront is partly past the end, and its last run of equal heads ends
+    next to those lanes.  Every coordinate of the gradient equals the
+    oracle's (a head reduction that read the idle lanes would not)."""
+    torch, dev = torch_dev
+    from umappp_amd.autograd import umap_loss
+
+    emb, heads, tails, coef, nneg, keys = _random_problem(5 * n + deg, n=n, deg=deg)
+    assert heads.size % 64 != 0
+    y, h, t, c, nn, k = _to(torch, dev, emb, heads, tails, coef, nneg, keys)
+    y.requires_grad_(True)
+    umap_loss(y, A, B, h, t, keys=k, coef=c, num_neg=nn, seed=21).backward()
+    P = oracle.LossProblem(emb.astype(np.float64), heads, tails, A, B, 1.0, seed=21, keys=keys,
+                           coef=coef.astype(np.float64), num_neg=nneg)
+    g_ref, _ = P.grad()
+    g = y.grad.cpu().numpy().astype(np.float64)
+    scale = np.max(np.abs(g_ref))
+    assert np.all(np.abs(g - g_ref) <= 1e-6 * np.abs(g_ref) + 1e-7 * scale)
+
+
+def test_gradient_only_pass(torch_dev):
+    """value=False skips the loss kernel: the result is 0 and the gradients
+    (embedding, a, b) are the full pass's, also with zero-coefficient edges
+    (which the kernels skip) in the list."""
+    torch, dev = torch_dev
+    from umappp_amd.autograd import umap_loss
+
+    emb, heads, tails, coef, nneg, keys = _random_problem(44, n=300, deg=6)
+    coef[::3] = 0
+    grads = []
+    for value in (True, False):
+        y, h, t, c, nn, k = _to(torch, dev, emb, heads, tails, coef, nneg, keys)
+        y.requires_grad_(True)
+        a = torch.tensor(A, dtype=torch.float64, device=dev, requires_grad=True)
+        b = torch.tensor(B, dtype=torch.float64, device=dev, requires_grad=True)
+        loss = umap_loss(y, a, b, h, t, keys=k, coef=c, num_neg=nn, seed=3, value=value)
+        assert value or float(loss) == 0.0
+        loss.backward()
+        grads.append((y.grad.cpu().numpy(), float(a.grad), float(b.grad)))
+    P = oracle.LossProblem(emb.astype(np.float64), heads, tails, A, B, 1.0, seed=3, keys=keys,
+                           coef=coef.astype(np.float64), num_neg=nneg)
+    g_ref, gab_ref = P.grad()
+    # atomics add in arbitrary order: both passes within float32 rounding of the oracle
+    for g, ga, gb in grads:
+        scale = np.max(np.abs(g_ref))
+        assert np.all(np.abs(g - g_ref) <= 1e-6 * np.abs(g_ref) + 1e-7 * scale)
+        assert abs(ga - gab_ref[0]) <= 1e-9 * abs(gab_ref[0]) and abs(gb - gab_ref[1]) <= 1e-9 * abs(gab_ref[1])
+
+
 def test_clipped_gradient_matches_oracle(torch_dev):
     """Per-term clipping (clip = 4) on a steep curve with close points, where
     the clip engages: equal to the oracle's clipped gradient."""

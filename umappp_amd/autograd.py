@@ -62,7 +62,7 @@ def _scalar(x, dev):
 
 
 def umap_loss(emb, a, b, heads, tails, *, keys=None, coef=None, num_neg=None, m=5, gamma=1.0, seed=0, step=0,
-              clip=0.0):
+              clip=0.0, value=True):
     """UMAP cross-entropy of ``emb`` over the edge list (see the module docstring).
 
     ``emb``: float32 (n, dim) GPU tensor; ``a``, ``b``: floats or float64
@@ -73,7 +73,9 @@ def umap_loss(emb, a, b, heads, tails, *, keys=None, coef=None, num_neg=None, m=
     makes the backward pass clip every term's gradient with respect to its
     head to ``[-clip, clip]`` per coordinate before summing (per-sample
     clipping; 4 is the reference's clip); ``clip = 0`` gives the exact
-    gradient.  Returns a float64 scalar tensor.
+    gradient.  Returns a float64 scalar tensor.  ``value=False`` skips the
+    loss kernel and returns 0 (the gradient is unchanged), for callers that
+    only differentiate.
     """
     import torch
 
@@ -85,7 +87,7 @@ def umap_loss(emb, a, b, heads, tails, *, keys=None, coef=None, num_neg=None, m=
         keys if keys is not None else empty(torch.int64),
         coef if coef is not None else empty(torch.float32),
         num_neg if num_neg is not None else empty(torch.int32),
-        int(m), float(gamma), int(seed) & (2**63 - 1), int(step), float(clip), int(stream))
+        int(m), float(gamma), int(seed) & (2**63 - 1), int(step), float(clip), int(stream), bool(value))
 
 
 def edge_heads(row_ptr):
@@ -172,7 +174,7 @@ class AutogradLayout:
                 y = emb.detach().requires_grad_(True)
                 loss = umap_loss(y, self.a, self.b, self.heads[idx].contiguous(), self.tails[idx].contiguous(),
                                  keys=self.keys[idx].contiguous(), num_neg=nneg.contiguous(),
-                                 gamma=self.gamma, seed=self.seed, step=n, clip=4.0)
+                                 gamma=self.gamma, seed=self.seed, step=n, clip=4.0, value=False)
                 (g,) = torch.autograd.grad(loss, [y])
                 with torch.no_grad():
                     emb.add_(g, alpha=-self._alpha(n))

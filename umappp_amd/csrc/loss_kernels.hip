@@ -227,74 +227,72 @@ __global__ void __launch_bounds__(kLossBlock) k_loss_backward(LossView v, double
     const uint32_t n32 = static_cast<uint32_t>(v.n);
     const int dim = v.dim;
     double da = 0, db = 0;
-    // every lane of a wavefront runs the same number of trips (the head
-    // reduction below is a wavefront collective); lanes past the end or with
-    // a bad edge take part with head -1 and a zero gradient
+    // every lane of a wavefront runs the same number of trips and reaches
+    // the head reduction (a wavefront collective: ballot and shuffles) at one
+    // call site; lanes past the end, with a bad edge or with a zero
+    // coefficient take part with head -1 and a zero gradient
     const int64_t stride = static_cast<int64_t>(gridDim.x) * kLossBlock;
     const int64_t first = static_cast<int64_t>(blockIdx.x) * kLossBlock + (threadIdx.x & ~(kWave - 1));
     for (int64_t e0 = first; e0 < v.nnz; e0 += stride) {
         const int64_t e = e0 + (threadIdx.x & (kWave - 1));
         int64_t i = -1, j = -1;
-        if (e >= v.nnz || !edge_ok(v, e, i, j)) {
-            add_head_segmented<D>(g64, -1, dim, DVec<D>{});
-            continue;
-        }
-        const double c = v.coef ? static_cast<double>(v.coef[e]) : 1.0;
-        if (c == 0) {  // every term of the edge, and its gradient, is scaled by c
-            add_head_segmented<D>(g64, -1, dim, DVec<D>{});
-            continue;
-        }
-        const int m = v.nneg ? v.nneg[e] : v.m;
-        const uint64_t ej = edge_key(v.ekey, v.keys ? static_cast<uint64_t>(v.keys[e]) : static_cast<uint64_t>(e));
-        const DVec<D> yi = load_d<D>(v.emb, i, dim);
-        const DVec<D> yj = load_d<D>(v.emb, j, dim);
-        int64_t kk[PF];
-        DVec<D> yk[PF];
-        gather_batch<D>(v, ej, n32, i, 0, m, kk, yk);
         DVec<D> gi{};
-        {
-            double s = sqd<D>(yi, yj);
-            const bool clamped = !(s > feps);
-            if (clamped) s = feps;
-            const double ls = log(s), sb = exp(b * ls), den = 1.0 + a * sb;
-            da += c * sb / den;
-            db += c * a * sb * ls / den;
-            if (!clamped) {
-                const double f = 2.0 * c * a * b * sb / (s * den);
-#pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    if (d >= dim) break;
-                    const double g = clip_term(f * (yi.c[d] - yj.c[d]), clip);
-                    gi.c[d] += g;
-                    unsafeAtomicAdd(g64 + j * dim + d, -g);
-                }
-            }
-        }
-        for (int p0 = 0; p0 < m; p0 += PF) {
-            if (p0 > 0) gather_batch<D>(v, ej, n32, i, p0, m, kk, yk);
-#pragma unroll
-            for (int u = 0; u < PF; ++u) {
-                if (kk[u] < 0) break;
-                if (kk[u] == i) continue;
-                double s = sqd<D>(yi, yk[u]);
+        bool active = e < v.nnz && edge_ok(v, e, i, j);
+        const double c = active && v.coef ? static_cast<double>(v.coef[e]) : 1.0;
+        if (c == 0) active = false;  // every term of the edge, and its gradient, is scaled by c
+        if (active) {
+            const int m = v.nneg ? v.nneg[e] : v.m;
+            const uint64_t ej =
+                edge_key(v.ekey, v.keys ? static_cast<uint64_t>(v.keys[e]) : static_cast<uint64_t>(e));
+            const DVec<D> yi = load_d<D>(v.emb, i, dim);
+            const DVec<D> yj = load_d<D>(v.emb, j, dim);
+            int64_t kk[PF];
+            DVec<D> yk[PF];
+            gather_batch<D>(v, ej, n32, i, 0, m, kk, yk);
+            {
+                double s = sqd<D>(yi, yj);
                 const bool clamped = !(s > feps);
                 if (clamped) s = feps;
                 const double ls = log(s), sb = exp(b * ls), den = 1.0 + a * sb;
-                da += c * v.gamma * (sb / den - 1.0 / a);
-                db += -c * v.gamma * ls / den;
+                da += c * sb / den;
+                db += c * a * sb * ls / den;
                 if (!clamped) {
-                    const double f = -2.0 * c * v.gamma * b / (s * den);
+                    const double f = 2.0 * c * a * b * sb / (s * den);
 #pragma unroll
                     for (int d = 0; d < D; ++d) {
                         if (d >= dim) break;
-                        const double g = clip_term(f * (yi.c[d] - yk[u].c[d]), clip);
+                        const double g = clip_term(f * (yi.c[d] - yj.c[d]), clip);
                         gi.c[d] += g;
-                        unsafeAtomicAdd(g64 + kk[u] * dim + d, -g);
+                        unsafeAtomicAdd(g64 + j * dim + d, -g);
+                    }
+                }
+            }
+            for (int p0 = 0; p0 < m; p0 += PF) {
+                if (p0 > 0) gather_batch<D>(v, ej, n32, i, p0, m, kk, yk);
+#pragma unroll
+                for (int u = 0; u < PF; ++u) {
+                    if (kk[u] < 0) break;
+                    if (kk[u] == i) continue;
+                    double s = sqd<D>(yi, yk[u]);
+                    const bool clamped = !(s > feps);
+                    if (clamped) s = feps;
+                    const double ls = log(s), sb = exp(b * ls), den = 1.0 + a * sb;
+                    da += c * v.gamma * (sb / den - 1.0 / a);
+                    db += -c * v.gamma * ls / den;
+                    if (!clamped) {
+                        const double f = -2.0 * c * v.gamma * b / (s * den);
+#pragma unroll
+                        for (int d = 0; d < D; ++d) {
+                            if (d >= dim) break;
+                            const double g = clip_term(f * (yi.c[d] - yk[u].c[d]), clip);
+                            gi.c[d] += g;
+                            unsafeAtomicAdd(g64 + kk[u] * dim + d, -g);
+                        }
                     }
                 }
             }
         }
-        add_head_segmented<D>(g64, i, dim, gi);
+        add_head_segmented<D>(g64, active ? i : -1, dim, gi);
     }
     const double ta = block_sum(da, lds);
     const double tb = block_sum(db, lds);

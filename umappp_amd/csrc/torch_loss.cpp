@@ -41,10 +41,12 @@ class UmapLoss : public torch::autograd::Function<UmapLoss> {
 public:
     // emb: float32 (n, dim); a, b: float64 scalars; heads, tails: int32 (nnz);
     // keys: int64 (nnz) or empty; coef: float32 (nnz) or empty; num_neg:
-    // int32 (nnz) or empty.
+    // int32 (nnz) or empty.  value = false skips the loss kernel (the result
+    // is 0; the backward pass, which recomputes everything it needs, is
+    // unchanged) for callers that only want the gradient.
     static Tensor forward(AutogradContext* ctx, Tensor emb, Tensor a, Tensor b, Tensor heads, Tensor tails,
                           Tensor keys, Tensor coef, Tensor num_neg, int64_t m, double gamma, int64_t seed,
-                          int64_t step, double clip, int64_t stream) {
+                          int64_t step, double clip, int64_t stream, bool value) {
         check_input(emb, torch::kFloat32, "embedding");
         TORCH_CHECK(emb.dim() == 2, "embedding must be (n, dim)");
         check_input(heads, torch::kInt32, "heads");
@@ -54,8 +56,10 @@ public:
         if (coef.numel()) check_input(coef, torch::kFloat32, "coef");
         if (num_neg.numel()) check_input(num_neg, torch::kInt32, "num_neg");
         Tensor ab = torch::stack({a.reshape({}), b.reshape({})}).to(emb.device(), torch::kFloat64).contiguous();
-        Tensor loss = torch::empty({}, emb.options().dtype(torch::kFloat64));
-        check(umx_loss_forward(emb.size(0), static_cast<int>(emb.size(1)), emb.data_ptr<float>(), heads.numel(),
+        Tensor loss = value ? torch::empty({}, emb.options().dtype(torch::kFloat64))
+                            : torch::zeros({}, emb.options().dtype(torch::kFloat64));
+        if (value)
+            check(umx_loss_forward(emb.size(0), static_cast<int>(emb.size(1)), emb.data_ptr<float>(), heads.numel(),
                                heads.data_ptr<int32_t>(), tails.data_ptr<int32_t>(), opt_ptr<int64_t>(keys),
                                opt_ptr<float>(coef), opt_ptr<int32_t>(num_neg), static_cast<int>(m),
                                ab.data_ptr<double>(), gamma, static_cast<uint64_t>(seed), static_cast<int>(step),
@@ -89,13 +93,14 @@ public:
                                 reinterpret_cast<void*>(ctx->saved_data["stream"].toInt())));
         return {grad_emb, grad_ab[0].to(a.options()).reshape(a.sizes()), grad_ab[1].to(b.options()).reshape(b.sizes()),
                 Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor(),
-                Tensor()};
+                Tensor(), Tensor()};
     }
 };
 
 Tensor umap_loss(Tensor emb, Tensor a, Tensor b, Tensor heads, Tensor tails, Tensor keys, Tensor coef,
-                 Tensor num_neg, int64_t m, double gamma, int64_t seed, int64_t step, double clip, int64_t stream) {
-    return UmapLoss::apply(emb, a, b, heads, tails, keys, coef, num_neg, m, gamma, seed, step, clip, stream);
+                 Tensor num_neg, int64_t m, double gamma, int64_t seed, int64_t step, double clip, int64_t stream,
+                 bool value) {
+    return UmapLoss::apply(emb, a, b, heads, tails, keys, coef, num_neg, m, gamma, seed, step, clip, stream, value);
 }
 
 }  // namespace
