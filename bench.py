@@ -42,6 +42,13 @@ CONFIGS = {
                label="200k x 50 f32, 30-component mixture (seed 5), exact kNN k=30, fuzzy graph, D=3, "
                      "UMAP cross-entropy over all edges + 5 negatives per edge (counter sampler), "
                      "forward + backward through torch autograd, then 200 autograd-driven epochs"),
+    "knn": dict(n=1_000_000, dim=50, ncomp=100, seed=3, k=15,
+                label="exact kNN producer on the headline input: 1M x 50 f32, 100-component mixture (seed 3), "
+                      "k=15, FP64 distances, self excluded"),
+    "spectral": dict(n=1_000_000, dim=50, ncomp=1, seed=13, k=15, out_dim=2,
+                     label="spectral initialisation on a connected graph: 1M x 50 f32 from one Gaussian "
+                           "(seed 13), exact kNN k=15, fuzzy union, D=2 (eigenvectors of 2I - L by Lanczos, "
+                           "FP64)"),
     "c4": dict(count=64, lo=2_000, hi=200_000, dim=30, ncomp=10, seed=4, k=15, epochs=200,
                label="64 independent UMAPs, sizes log-uniform in [2k, 200k] (seed 4), each 30 dims f32 from "
                      "its own 10-component mixture (seed 4000+i), exact kNN k=15, D=2, 200 epochs, nsr 5"),
@@ -139,6 +146,12 @@ def main():
         return
     if args.config == "c5":
         main_autograd(args, umx, torch, dev)
+        return
+    if args.config == "knn":
+        main_knn(args, umx, torch, dev)
+        return
+    if args.config == "spectral":
+        main_spectral(args, umx, torch, dev)
         return
     cfg = CONFIGS[args.config]
     n, k, T = cfg["n"], cfg["k"], cfg["epochs"]
@@ -617,6 +630,200 @@ def main_autograd(args, umx, torch, dev):
         "layout_200_epochs_ms": {"plain": round(plain_s * 1000, 3), "autograd": round(auto_s * 1000, 3)},
         "knn15_preservation": {"plain": round(pres_plain, 4), "autograd": round(pres_auto, 4)},
     }
+    print(json.dumps(out), flush=True)
+
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (FMA counted as two operations)
+
+
+def main_knn(args, umx, torch, dev):
+    """The kNN producer (umx_knn_bruteforce) at the headline's input size.
+
+    One step = one exact search of all 1M queries against all 1M points.
+    value = queries / s.  Roofline: FP64 arithmetic, 3 operations per
+    (query, point, dimension) -- a subtract, a multiply and an add, kept
+    separate so that each squared distance is that of a sequential double
+    loop -- against the published FP64 vector peak; without FMA the issue
+    ceiling for this instruction mix is half of it.  The kernel is one launch
+    per search, timed by the wall clock around the blocking call.
+    cpu_baseline: the oracle's brute force (16 threads) on a bounded range of
+    queries; those rows must equal the GPU's bit for bit."""
+    cfg = CONFIGS["knn"]
+    n, dim, k = cfg["n"], cfg["dim"], cfg["k"]
+    log(f"generating {cfg['label']}")
+    x_np = make_mixture(cfg["seed"], n, dim, cfg["ncomp"])
+    x = torch.from_numpy(x_np).to(dev)
+    idx = torch.empty((n, k), dtype=torch.int32, device=dev)
+    dist = torch.empty((n, k), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+
+    def search():
+        rc = umx.lib().umx_knn_bruteforce(umx.UMX_F32, n, dim, x.data_ptr(), k, idx.data_ptr(), dist.data_ptr(), 1)
+        if rc != 0:
+            raise RuntimeError(umx.lib().umx_last_error().decode())
+
+    for w in range(args.warmup):
+        search()
+        log(f"warmup {w + 1}/{args.warmup}")
+    torch.cuda.synchronize()
+    times = []
+    for s in range(args.steps):
+        t0 = time.perf_counter()
+        search()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        log(f"step {s + 1}/{args.steps}: {times[-1]:.3f}s")
+    sec = float(np.mean(times))
+    tflops = 3.0 * n * n * dim / sec / 1e12
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+
+        cores = min(os.cpu_count() or 1, 16)
+        xd = x_np.astype(np.float64)
+        t0 = time.perf_counter()
+        oracle.knn_bruteforce_rows(xd, k, 0, cores, nthreads=cores)  # one query per thread
+        t_round = max(time.perf_counter() - t0, 1e-6)
+        count = int(min(n, cores * max(1.0, args.cpu_seconds / t_round)))
+        log(f"cpu baseline: {count} queries on {cores} threads")
+        t0 = time.perf_counter()
+        ri, rd = oracle.knn_bruteforce_rows(xd, k, 0, count, nthreads=cores)
+        cpu_s = time.perf_counter() - t0
+        gi = idx[:count].cpu().numpy()
+        gd = dist[:count].cpu().numpy()
+        exact = bool(np.array_equal(gi, ri) and np.array_equal(gd, rd.astype(np.float32)))
+        cpu = {"value": round(count / cpu_s, 2), "unit": "queries/s", "cores": cores, "kind": "port",
+               "sample": f"the oracle's exact brute force (double distances, sequential sums) for queries "
+                         f"0..{count - 1} against all {n} points in {cpu_s:.2f}s on {cores} threads",
+               "sample_rows_equal_gpu": exact, "host_cpu": _cpu_model()}
+        if not exact:
+            log("cpu baseline: GPU rows differ from the oracle's")
+    value = n / sec
+    out = {
+        "metric": "exact kNN queries/s, 1M x 50 f32, k=15 (FP64 distances)",
+        "value": round(value, 1),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec * 1000.0, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (f32 input)",
+        "data": "synthetic",
+        "config": {"workload": cfg["label"], "n": n, "dim": dim, "k": k},
+        "roofline": {"bound": "fp64-valu", "achieved": round(tflops, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(tflops / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "kernel": "k_knn<float,16,32,16>", "avg_launch_us": round(sec * 1e6, 1),
+                     "ops_per_pair_dim": 3, "no_fma_issue_ceiling_tflops": FP64_PEAK_TFLOPS / 2},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+def main_spectral(args, umx, torch, dev):
+    """Spectral initialisation (umx_graph_spectral_init) on a connected graph.
+
+    Setup (untimed): one-Gaussian data, exact kNN, fuzzy similarities and
+    union on the GPU.  One step = one spectral initialisation from the
+    resident graph into a device embedding: components, normalized
+    adjacency, Lanczos (continued until the Ritz pairs converge), Ritz
+    vectors, scaling.  value = observations / s.  Roofline: the
+    algorithmic HBM bytes the library counts for the launches of one
+    initialisation (Gram-Schmidt streams of the Lanczos basis dominate) over
+    its wall time.  cpu_baseline: the reference's eigensolver (irlba, C++
+    with Eigen) is not in this image; scipy's ARPACK eigsh (implicitly
+    restarted Lanczos) on the same matrix 2I - L stands in, and its
+    eigenvalues are compared with the GPU's."""
+    import ctypes as C
+
+    cfg = CONFIGS["spectral"]
+    n, k, D = cfg["n"], cfg["k"], cfg["out_dim"]
+    log(f"generating {cfg['label']}")
+    x = make_mixture(cfg["seed"], n, cfg["dim"], cfg["ncomp"])
+    kidx, kdist = knn_gpu(x, k, dev)
+    del x
+    rp = torch.arange(n + 1, dtype=torch.int64, device=dev) * k
+    g = umx.Graph((rp.data_ptr(), kidx.data_ptr(), kdist.data_ptr()), device_ptrs=True, n=n, dtype=np.float32)
+    g.neighbor_similarities(1.0, 1.0, 1e-3, True)
+    g.combine(1.0)
+    emb = torch.empty((n, D), dtype=torch.float32, device=dev)
+    ok = C.c_int()
+    ev = np.full(D + 1, np.nan)
+    torch.cuda.synchronize()
+
+    def init():
+        rc = umx.lib().umx_graph_spectral_init(g._h, D, 10.0, 0, 1e-4, C.c_uint64(1234567890), emb.data_ptr(), 1,
+                                               C.byref(ok), ev.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise RuntimeError(umx.lib().umx_last_error().decode())
+        if not ok.value:
+            raise RuntimeError("spectral initialisation did not succeed on a connected graph")
+
+    for w in range(args.warmup):
+        init()
+        log(f"warmup {w + 1}/{args.warmup}")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        init()
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / args.steps
+    rep = g.spectral_report()
+    log(f"{args.steps} initialisations, {sec * 1000:.1f} ms each; {rep}")
+    achieved = rep["bytes"] / sec / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        import scipy.sparse as sp
+        import scipy.sparse.linalg as sla
+
+        nl = g.export()
+        W = sp.csr_matrix((nl.val.astype(np.float64), nl.idx, nl.row_ptr), shape=(n, n))
+        isd = 1.0 / np.sqrt(np.asarray(W.sum(axis=1)).ravel())
+        M = sp.identity(n, format="csr") + sp.diags(isd) @ W @ sp.diags(isd)
+        log("cpu baseline: scipy eigsh on 2I - L")
+        t0 = time.perf_counter()
+        wv, V = sla.eigsh(M, k=D + 1, which="LA", tol=1e-6)
+        cpu_s = time.perf_counter() - t0
+        wv = np.sort(wv)[::-1]
+        Y = emb.cpu().numpy().astype(np.float64)
+        resid = [float(np.linalg.norm(M @ Y[:, c] - ev[c + 1] * Y[:, c]) / np.linalg.norm(Y[:, c])) for c in range(D)]
+        cpu = {"value": round(n / cpu_s, 1), "unit": "observations/s", "cores": 1, "kind": "library",
+               "sample": f"scipy.sparse.linalg.eigsh (ARPACK, tol 1e-6) for the {D + 1} largest eigenpairs of "
+                         f"2I - L on the same {n}-point graph in {cpu_s:.2f}s; the reference's irlba is not in "
+                         f"the image",
+               "eigenvalues_max_rel_diff_vs_gpu": float(np.max(np.abs(wv - ev) / np.abs(wv))),
+               "gpu_eigenvector_residuals": resid, "host_cpu": _cpu_model()}
+    value = n / sec
+    out = {
+        "metric": "spectral initialisation observations/s, 1M-point connected kNN graph k=15, D=2",
+        "value": round(value, 1),
+        "unit": "observations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec * 1000.0, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": cfg["label"], "n": n, "k": k, "num_dim": D, "stored_edges": int(g.info()[1]),
+                   "krylov_dim": rep["krylov_dim"], "convergence_checks": rep["attempts"],
+                   "worst_ritz_residual": rep["worst_residual"], "eigenvalues_2I_minus_L": ev.tolist()},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "umx_graph_spectral_init (all launches; k_dots / k_subtract dominate)",
+                     "avg_launch_us": round(sec * 1e6, 1), "algorithmic_bytes_per_launch": round(rep["bytes"])},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
     print(json.dumps(out), flush=True)
 
 

@@ -71,6 +71,8 @@ def lib():
             "orc_gen_normal": (None, [C.c_uint64, C.c_int64, _vp]),
             "orc_gen_uniform01": (None, [C.c_uint64, C.c_int64, _vp]),
             "orc_knn_bruteforce": (None, [C.c_int64, C.c_int, _vp, C.c_int, C.c_int, _vp, _vp]),
+            "orc_knn_bruteforce_range": (None, [C.c_int64, C.c_int, _vp, C.c_int, C.c_int64, C.c_int64, C.c_int,
+                                                _vp, _vp]),
             "orc_loss": (C.c_double, _LOSS_ARGS),
             "orc_loss_grad": (None, _LOSS_ARGS + [C.c_double, _vp, _vp]),
             "orc_loss_fd": (None, _LOSS_ARGS + [C.c_int64, _vp, _vp, C.c_double, _vp]),
@@ -260,6 +262,20 @@ def knn_bruteforce(x: np.ndarray, k: int, nthreads=8):
     idx = np.empty((n, k), dtype=np.int32)
     dist = np.empty((n, k))
     lib().orc_knn_bruteforce(n, dim, _p(x), k, nthreads, _p(idx), _p(dist))
+    return idx, dist
+
+
+def knn_bruteforce_rows(x: np.ndarray, k: int, q0: int, q1: int, nthreads=8):
+    """knn_bruteforce for queries q0 .. q1 - 1 only (against all points):
+    (q1 - q0, k) indices and distances, equal to those rows of the full
+    search."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n, dim = x.shape
+    if not 0 <= q0 <= q1 <= n:
+        raise ValueError("query range outside the data")
+    idx = np.empty((q1 - q0, k), dtype=np.int32)
+    dist = np.empty((q1 - q0, k))
+    lib().orc_knn_bruteforce_range(n, dim, _p(x), k, q0, q1, nthreads, _p(idx), _p(dist))
     return idx, dist
 
 

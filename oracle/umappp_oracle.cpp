@@ -662,7 +662,9 @@ void gen_uniform01(uint64_t seed, int64_t count, double* out) {
 
 // Exact Euclidean kNN by brute force, self excluded; distances are the square
 // root of a sequential sum of squared differences; ties broken by index.
-void knn_bruteforce(int64_t n, int dim, const double* x, int k, int nthreads, int32_t* out_idx, double* out_dist) {
+// Queries q0 .. q1 - 1 against all n points; row i - q0 of the outputs.
+void knn_bruteforce(int64_t n, int dim, const double* x, int k, int64_t q0, int64_t q1, int nthreads,
+                    int32_t* out_idx, double* out_dist) {
     auto work = [&](int64_t b, int64_t e) {
         std::vector<std::pair<double, int32_t>> cand(static_cast<size_t>(n));
         for (int64_t i = b; i < e; ++i) {
@@ -679,16 +681,16 @@ void knn_bruteforce(int64_t n, int dim, const double* x, int k, int nthreads, in
             const size_t kk = std::min<size_t>(static_cast<size_t>(k), c);
             std::partial_sort(cand.begin(), cand.begin() + static_cast<long>(kk), cand.begin() + static_cast<long>(c));
             for (size_t q = 0; q < kk; ++q) {
-                out_idx[i * k + static_cast<int64_t>(q)] = cand[q].second;
-                out_dist[i * k + static_cast<int64_t>(q)] = std::sqrt(cand[q].first);
+                out_idx[(i - q0) * k + static_cast<int64_t>(q)] = cand[q].second;
+                out_dist[(i - q0) * k + static_cast<int64_t>(q)] = std::sqrt(cand[q].first);
             }
         }
     };
     if (nthreads < 1) nthreads = 1;
     std::vector<std::thread> pool;
-    const int64_t chunk = (n + nthreads - 1) / nthreads;
+    const int64_t chunk = (q1 - q0 + nthreads - 1) / nthreads;
     for (int t = 0; t < nthreads; ++t) {
-        const int64_t b = std::min<int64_t>(n, t * chunk), e = std::min<int64_t>(n, b + chunk);
+        const int64_t b = std::min<int64_t>(q1, q0 + t * chunk), e = std::min<int64_t>(q1, b + chunk);
         pool.emplace_back(work, b, e);
     }
     for (auto& th : pool) th.join();
@@ -972,7 +974,11 @@ int orc_discrete_uniform_stream(uint64_t seed, uint64_t n, int64_t count, int64_
 void orc_gen_normal(uint64_t seed, int64_t count, double* out) { gen_normal(seed, count, out); }
 void orc_gen_uniform01(uint64_t seed, int64_t count, double* out) { gen_uniform01(seed, count, out); }
 void orc_knn_bruteforce(int64_t n, int dim, const double* x, int k, int nthreads, int32_t* idx, double* dist) {
-    knn_bruteforce(n, dim, x, k, nthreads, idx, dist);
+    knn_bruteforce(n, dim, x, k, 0, n, nthreads, idx, dist);
+}
+void orc_knn_bruteforce_range(int64_t n, int dim, const double* x, int k, int64_t q0, int64_t q1, int nthreads,
+                              int32_t* idx, double* dist) {
+    knn_bruteforce(n, dim, x, k, q0, q1, nthreads, idx, dist);
 }
 
 // Loss value (double).

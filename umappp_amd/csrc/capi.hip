@@ -254,6 +254,7 @@ struct umx_graph {
     DevBuf<double> vd;
     DevBuf<double> rho, sigma;
     bool have_rho_sigma = false;
+    mutable umx::SpectralReport spectral;  // diagnostics of the last spectral initialisation
     // instances (a plain graph is one instance)
     std::vector<int64_t> inst_off;  // host, n_inst + 1
     DevBuf<int64_t> d_inst_off;
@@ -496,7 +497,19 @@ int umx_graph_spectral_init(const umx_graph* g, int num_dim, double scale, int j
         if (eigenvalues)
             for (size_t c = 0; c < rep.eigenvalues.size() && c < static_cast<size_t>(num_dim) + 1; ++c)
                 eigenvalues[c] = rep.eigenvalues[c];
+        g->spectral = rep;
         *success = ok ? 1 : 0;
+    });
+}
+
+int umx_graph_spectral_report(const umx_graph* g, int* krylov_dim, int* attempts, double* worst_residual,
+                              double* bytes) {
+    return guard([&] {
+        require(g != nullptr, UMX_ERR_ARGUMENT, "NULL argument");
+        if (krylov_dim) *krylov_dim = g->spectral.krylov_dim;
+        if (attempts) *attempts = g->spectral.attempts;
+        if (worst_residual) *worst_residual = g->spectral.worst_residual;
+        if (bytes) *bytes = g->spectral.bytes;
     });
 }
 

@@ -122,22 +122,38 @@ __global__ void k_spmv(int64_t n, int G, const int64_t* __restrict__ rp, const i
     if (i < n && lane == 0) y[i] = x[i] + s;
 }
 
-// out[c] = <V[c], w> for c < m (V column-major: vector c at V + c * n).
-__global__ void k_dots(int64_t n, int m, const double* __restrict__ V, const double* __restrict__ w,
-                       double* __restrict__ out) {
-    __shared__ double part[kB / kWave];
-    const int c = blockIdx.y;
-    const double* v = V + static_cast<int64_t>(c) * n;
-    double s = 0;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * kB)
-        s += v[i] * w[i];
-    for (int o = kWave / 2; o >= 1; o >>= 1) s += __shfl_xor(s, o, kWave);
-    if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = s;
+// out[c] += <V[c], w> for c < m (V column-major: vector c at V + c * n; out
+// zeroed by the caller).  A workgroup takes a slice of the rows for kDotCols
+// columns at once (blockIdx.y picks the group), so w is read once per group
+// of columns instead of once per column.
+constexpr int kDotCols = 8;
+__global__ void __launch_bounds__(kB) k_dots(int64_t n, int m, const double* __restrict__ V,
+                                             const double* __restrict__ w, double* __restrict__ out) {
+    __shared__ double part[kB / kWave][kDotCols];
+    const int c0 = static_cast<int>(blockIdx.y) * kDotCols;
+    const int cn = m - c0 < kDotCols ? m - c0 : kDotCols;
+    double s[kDotCols];
+#pragma unroll
+    for (int c = 0; c < kDotCols; ++c) s[c] = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kB) {
+        const double wi = w[i];
+#pragma unroll
+        for (int c = 0; c < kDotCols; ++c)
+            if (c < cn) s[c] += V[static_cast<int64_t>(c0 + c) * n + i] * wi;
+    }
+#pragma unroll
+    for (int c = 0; c < kDotCols; ++c)
+        for (int o = kWave / 2; o >= 1; o >>= 1) s[c] += __shfl_xor(s[c], o, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+#pragma unroll
+        for (int c = 0; c < kDotCols; ++c) part[threadIdx.x / kWave][c] = s[c];
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (static_cast<int>(threadIdx.x) < cn) {
         double t = 0;
-        for (int q = 0; q < kB / kWave; ++q) t += part[q];
-        unsafeAtomicAdd(out + c, t);
+        for (int q = 0; q < kB / kWave; ++q) t += part[q][threadIdx.x];
+        unsafeAtomicAdd(out + c0 + threadIdx.x, t);
     }
 }
 
@@ -151,9 +167,12 @@ __global__ void k_subtract(int64_t n, int m, const double* __restrict__ V, const
     w[i] = s;
 }
 
-__global__ void k_scale_into(int64_t n, const double* __restrict__ w, double f, double* __restrict__ v) {
+// v = w / sqrt(nrm2[0]): the norm stays on the device, so a Lanczos step
+// needs no host round trip.
+__global__ void k_normalize_into(int64_t n, const double* __restrict__ w, const double* __restrict__ nrm2,
+                                 double* __restrict__ v) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
-    if (i < n) v[i] = w[i] * f;
+    if (i < n) v[i] = w[i] * (1.0 / sqrt(nrm2[0]));
 }
 
 __global__ void k_fill_start(int64_t n, uint64_t seed, double* __restrict__ v) {
@@ -272,67 +291,96 @@ bool spectral_embedding(int64_t n, const int64_t* rp, const int32_t* idx, const 
     UMX_HIP(hipMemcpyAsync(&nnz, rp + n, sizeof(nnz), hipMemcpyDeviceToHost, s));
     UMX_HIP(hipStreamSynchronize(s));
 
+    // Algorithmic HBM bytes per launch (every array counted once per read,
+    // gathered vectors once): the report's roofline figure.
+    const double nb = 8.0 * static_cast<double>(n);
+    double bytes = 0;
+
     DevBuf<double> isd, a;
     isd.alloc(static_cast<size_t>(n));
     a.alloc(static_cast<size_t>(nnz));
     k_inv_sqrt_degree<T><<<blocks_for(n), kB, 0, s>>>(n, rp, val, isd.p);
     k_norm_values<T><<<blocks_for(n), kB, 0, s>>>(n, rp, idx, val, isd.p, a.p);
     UMX_HIP(hipGetLastError());
+    bytes += (8.0 * (n + 1) + sizeof(T) * nnz + nb) + (8.0 * (n + 1) + (4.0 + sizeof(T) + 8.0) * nnz + nb);
     (void)max_degree;
     const int G = lanes_for_degree(std::max<int64_t>(1, (nnz + n - 1) / n));  // from the average degree
+    const double spmv_bytes = 8.0 * (n + 1) + 12.0 * static_cast<double>(nnz) + 2.0 * nb;
 
-    int k = std::min<int64_t>(n, std::max(prm.start_dim, 2 * want + 8));
+    // One Lanczos run, extended when the Ritz pairs have not converged: the
+    // Krylov dimension k doubles (up to max_dim) and the run continues from
+    // where it stopped, so no step is computed twice.  alpha_j = <v_j, M v_j>
+    // and the squared norm of each orthogonalised direction stay on the
+    // device; the host reads them once per convergence check.
+    int k = static_cast<int>(std::min<int64_t>(n, std::max(prm.start_dim, 2 * want + 8)));
     const int kmax = static_cast<int>(std::min<int64_t>(n, prm.max_dim));
-    DevBuf<double> w, coef;
+    const int kcap = std::max(k, kmax);
+    DevBuf<double> w, coef, sc;
     w.alloc(static_cast<size_t>(n));
-    coef.alloc(static_cast<size_t>(kmax + 1));
-    std::vector<double> alpha, beta;
-    std::vector<double> hcoef;
+    coef.alloc(static_cast<size_t>(kcap) + 1);
+    sc.alloc(2 * (static_cast<size_t>(kcap) + 1));
+    double* const d_alpha = sc.p;              // alpha_j at [j]
+    double* const d_nrm2 = sc.p + kcap + 1;    // squared norm of direction j at [j]
+    UMX_HIP(hipMemsetAsync(sc.p, 0, sizeof(double) * sc.n, s));  // each slot is accumulated once
     DevBuf<double> V;
+    V.alloc(static_cast<size_t>(k) * static_cast<size_t>(n));
+    // v0: a fixed pseudo-random start (normalised as direction 0)
+    k_fill_start<<<blocks_for(n), kB, 0, s>>>(n, prm.start_seed, w.p);
+    bytes += nb;
+    const unsigned gx = static_cast<unsigned>(std::min<int64_t>((n + kB - 1) / kB, 256));
+    auto dots = [&](int m, const double* VV, const double* ww, double* to) {
+        k_dots<<<dim3(gx, static_cast<unsigned>((m + kDotCols - 1) / kDotCols)), kB, 0, s>>>(n, m, VV, ww, to);
+        bytes += (m + 1) * nb;
+    };
+    int j = 0;               // the step whose direction w holds
+    bool orthogonal = false; // w already orthogonalised against V[0..j) (after an extension)
+    std::vector<double> ha, hn, beta;
     while (true) {
-        V.alloc(static_cast<size_t>(k + 1) * static_cast<size_t>(n));
-        alpha.assign(static_cast<size_t>(k), 0.0);
-        beta.assign(static_cast<size_t>(k), 0.0);
-        // v0: a fixed pseudo-random start, normalised
-        k_fill_start<<<blocks_for(n), kB, 0, s>>>(n, prm.start_seed, w.p);
-        int m = 0;  // vectors in V
-        bool broke = false;
-        for (int j = 0; j <= k; ++j) {
-            // w holds the next direction; orthogonalise against V[0..m) twice
-            for (int pass = 0; pass < 2 && m > 0; ++pass) {
-                UMX_HIP(hipMemsetAsync(coef.p, 0, sizeof(double) * static_cast<size_t>(m), s));
-                const unsigned gx = static_cast<unsigned>(std::min<int64_t>((n + kB - 1) / kB, 256));
-                k_dots<<<dim3(gx, static_cast<unsigned>(m)), kB, 0, s>>>(n, m, V.p, w.p, coef.p);
-                k_subtract<<<blocks_for(n), kB, 0, s>>>(n, m, V.p, coef.p, w.p);
+        for (;; ++j) {
+            if (!orthogonal) {
+                // classical Gram-Schmidt twice against V[0..j), then |w|^2
+                for (int pass = 0; pass < 2 && j > 0; ++pass) {
+                    UMX_HIP(hipMemsetAsync(coef.p, 0, sizeof(double) * static_cast<size_t>(j), s));
+                    dots(j, V.p, w.p, coef.p);
+                    k_subtract<<<blocks_for(n), kB, 0, s>>>(n, j, V.p, coef.p, w.p);
+                    bytes += (j + 2) * nb;
+                }
+                dots(1, w.p, w.p, d_nrm2 + j);
                 UMX_HIP(hipGetLastError());
             }
-            UMX_HIP(hipMemsetAsync(coef.p, 0, sizeof(double), s));
-            const unsigned gx = static_cast<unsigned>(std::min<int64_t>((n + kB - 1) / kB, 256));
-            k_dots<<<dim3(gx, 1), kB, 0, s>>>(n, 1, w.p, w.p, coef.p);
-            double nrm2 = 0;
-            UMX_HIP(hipMemcpyAsync(&nrm2, coef.p, sizeof(double), hipMemcpyDeviceToHost, s));
-            UMX_HIP(hipStreamSynchronize(s));
-            const double nrm = std::sqrt(nrm2);
-            if (j > 0) beta[j - 1] = nrm;
+            orthogonal = false;
             if (j == k) break;
-            if (!(nrm > 1e-300) || (j > 0 && nrm < 1e-12 * std::fabs(alpha[j - 1]))) {
-                broke = true;  // invariant subspace: T is exact
-                break;
-            }
             double* vj = V.p + static_cast<int64_t>(j) * n;
-            k_scale_into<<<blocks_for(n), kB, 0, s>>>(n, w.p, 1.0 / nrm, vj);
-            m = j + 1;
-            // w = M v_j
+            k_normalize_into<<<blocks_for(n), kB, 0, s>>>(n, w.p, d_nrm2 + j, vj);
+            // w = M v_j, alpha_j = <v_j, w>
             k_spmv<<<blocks_for(n * G), kB, 0, s>>>(n, G, rp, idx, a.p, vj, w.p);
             UMX_HIP(hipGetLastError());
-            UMX_HIP(hipMemsetAsync(coef.p, 0, sizeof(double), s));
-            k_dots<<<dim3(gx, 1), kB, 0, s>>>(n, 1, vj, w.p, coef.p);
-            UMX_HIP(hipMemcpyAsync(&alpha[j], coef.p, sizeof(double), hipMemcpyDeviceToHost, s));
-            UMX_HIP(hipStreamSynchronize(s));
+            bytes += 2 * nb + spmv_bytes;
+            dots(1, vj, w.p, d_alpha + j);
         }
-        const int kk = m;
+        ha.resize(static_cast<size_t>(k));
+        hn.resize(static_cast<size_t>(k) + 1);
+        UMX_HIP(hipMemcpyAsync(ha.data(), d_alpha, sizeof(double) * ha.size(), hipMemcpyDeviceToHost, s));
+        UMX_HIP(hipMemcpyAsync(hn.data(), d_nrm2, sizeof(double) * hn.size(), hipMemcpyDeviceToHost, s));
+        UMX_HIP(hipStreamSynchronize(s));
+        // beta_{j-1} = |w_j|; a vanishing direction means an invariant
+        // subspace (T is exact): the steps after it are discarded
+        beta.assign(static_cast<size_t>(k), 0.0);
+        int kk = k;
+        bool broke = false;
+        for (int jj = 0; jj <= k; ++jj) {
+            const double nrm = std::sqrt(hn[jj]);
+            if (jj > 0) beta[jj - 1] = nrm;
+            if (jj == k) break;
+            if (!(nrm > 1e-300) || (jj > 0 && nrm < 1e-12 * std::fabs(ha[jj - 1]))) {
+                broke = true;
+                kk = jj;
+                break;
+            }
+        }
+        if (rep) ++rep->attempts;
         if (kk < want) return false;
-        std::vector<double> dd(alpha.begin(), alpha.begin() + kk), ee(beta.begin(), beta.begin() + (kk - 1)), z;
+        std::vector<double> dd(ha.begin(), ha.begin() + kk), ee(beta.begin(), beta.begin() + (kk - 1)), z;
         if (!tridiagonal_eigen(kk, dd, ee, z)) return false;
         // the 'want' largest Ritz values and their residuals beta_k * |z[k-1][c]|
         std::vector<int> order(static_cast<size_t>(kk));
@@ -350,18 +398,29 @@ bool spectral_embedding(int64_t n, const int64_t* rp, const int32_t* idx, const 
         if (rep) {
             rep->krylov_dim = kk;
             rep->worst_residual = worst;
+            rep->bytes = bytes;
             rep->eigenvalues.clear();
             for (int c = 0; c < want; ++c) rep->eigenvalues.push_back(dd[order[c]]);
         }
         if (!ok) {
             if (k >= kmax) return false;
-            k = std::min(kmax, 2 * k);
+            // extend: room for the next k vectors, then continue at step k,
+            // whose direction w is already orthogonalised (norm in d_nrm2[k])
+            const int k2 = std::min(kmax, 2 * k);
+            DevBuf<double> V2;
+            V2.alloc(static_cast<size_t>(k2) * static_cast<size_t>(n));
+            UMX_HIP(hipMemcpyAsync(V2.p, V.p, sizeof(double) * static_cast<size_t>(k) * static_cast<size_t>(n),
+                                   hipMemcpyDeviceToDevice, s));
+            bytes += 2.0 * k * nb;
+            V.swap(V2);
+            k = k2;
+            orthogonal = true;
             continue;
         }
         // Ritz vectors of eigenvalues 2 .. want (drop the largest)
         std::vector<double> S(static_cast<size_t>(kk) * d);
-        for (int j = 0; j < kk; ++j)
-            for (int c = 0; c < d; ++c) S[static_cast<size_t>(j) * d + c] = z[static_cast<size_t>(j) * kk + order[c + 1]];
+        for (int jj = 0; jj < kk; ++jj)
+            for (int c = 0; c < d; ++c) S[static_cast<size_t>(jj) * d + c] = z[static_cast<size_t>(jj) * kk + order[c + 1]];
         DevBuf<double> dS, U;
         dS.alloc(S.size());
         U.alloc(static_cast<size_t>(n) * d);
@@ -397,6 +456,8 @@ bool spectral_embedding(int64_t n, const int64_t* rp, const int32_t* idx, const 
         k_emit<T><<<blocks_for(n), kB, 0, s>>>(n, d, U.p, dmul.p, prm.jitter ? 1 : 0, prm.jitter_sd, prm.jitter_seed,
                                               out);
         UMX_HIP(hipGetLastError());
+        bytes += (kk + d) * nb + d * nb + d * nb + d * nb + sizeof(T) * d * static_cast<double>(n);
+        if (rep) rep->bytes = bytes;
         UMX_HIP(hipStreamSynchronize(s));
         return true;
     }
