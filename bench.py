@@ -815,6 +815,7 @@ def main_spectral(args, umx, torch, dev):
         "data": "synthetic",
         "config": {"workload": cfg["label"], "n": n, "k": k, "num_dim": D, "stored_edges": int(g.info()[1]),
                    "krylov_dim": rep["krylov_dim"], "convergence_checks": rep["attempts"],
+                   "second_reorth_passes": rep["second_passes"],
                    "worst_ritz_residual": rep["worst_residual"], "eigenvalues_2I_minus_L": ev.tolist()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

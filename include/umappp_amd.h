@@ -146,12 +146,13 @@ int umx_graph_spectral_init(const umx_graph* g, int num_dim, double scale, int j
                             uint64_t seed, void* embedding, int device_ptr, int* success, double* eigenvalues);
 /* Diagnostics of the last umx_graph_spectral_init on g (any pointer may be
  * NULL): the Lanczos vectors behind its Ritz pairs, the convergence checks
- * made (the Krylov dimension doubles between them), the worst relative Ritz
+ * made (the Krylov dimension doubles between them), the Lanczos steps that
+ * needed a second re-orthogonalisation pass, the worst relative Ritz
  * residual, and the algorithmic HBM bytes of the kernels it launched (for a
  * roofline).  No counterpart in the reference: spectral_init.hpp does not
  * surface irlba's own diagnostics. */
-int umx_graph_spectral_report(const umx_graph* g, int* krylov_dim, int* attempts, double* worst_residual,
-                              double* bytes);
+int umx_graph_spectral_report(const umx_graph* g, int* krylov_dim, int* attempts, int* second_passes,
+                              double* worst_residual, double* bytes);
 
 /* Batched graph (no counterpart in the reference, which runs one problem per
  * initialize() call): n_inst independent kNN graphs stored one after the

@@ -101,7 +101,8 @@ def _declare(L):
         "umx_graph_components": (C.c_int, [_vp, _i64p]),
         "umx_graph_spectral_init": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, C.c_double, C.c_uint64, _vp,
                                               C.c_int, C.POINTER(C.c_int), _vp]),
-        "umx_graph_spectral_report": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), _f64p, _f64p]),
+        "umx_graph_spectral_report": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                                _f64p, _f64p]),
         "umx_tridiagonal_eigen": (C.c_int, [C.c_int, _vp, _vp, _vp]),
         "umx_graph_create_batch": (C.c_int, [C.c_int, C.c_int32, _vp, _vp, _vp, _vp, C.c_int, C.POINTER(_vp)]),
         "umx_graph_instances": (C.c_int, [_vp, _i32p, _vp]),
@@ -336,11 +337,14 @@ class Graph:
 
     def spectral_report(self):
         """Diagnostics of the last spectral_init: dict with krylov_dim,
-        attempts, worst_residual and bytes (algorithmic HBM bytes)."""
-        kd, at = C.c_int(), C.c_int()
+        attempts, second_passes (steps re-orthogonalised twice),
+        worst_residual and bytes (algorithmic HBM bytes)."""
+        kd, at, sp = C.c_int(), C.c_int(), C.c_int()
         wr, by = C.c_double(), C.c_double()
-        _check(lib().umx_graph_spectral_report(self._h, C.byref(kd), C.byref(at), C.byref(wr), C.byref(by)))
-        return {"krylov_dim": kd.value, "attempts": at.value, "worst_residual": wr.value, "bytes": by.value}
+        _check(lib().umx_graph_spectral_report(self._h, C.byref(kd), C.byref(at), C.byref(sp), C.byref(wr),
+                                               C.byref(by)))
+        return {"krylov_dim": kd.value, "attempts": at.value, "second_passes": sp.value, "worst_residual": wr.value,
+                "bytes": by.value}
 
     def rho_sigma(self):
         n, _ = self.info()

@@ -502,12 +502,13 @@ int umx_graph_spectral_init(const umx_graph* g, int num_dim, double scale, int j
     });
 }
 
-int umx_graph_spectral_report(const umx_graph* g, int* krylov_dim, int* attempts, double* worst_residual,
-                              double* bytes) {
+int umx_graph_spectral_report(const umx_graph* g, int* krylov_dim, int* attempts, int* second_passes,
+                              double* worst_residual, double* bytes) {
     return guard([&] {
         require(g != nullptr, UMX_ERR_ARGUMENT, "NULL argument");
         if (krylov_dim) *krylov_dim = g->spectral.krylov_dim;
         if (attempts) *attempts = g->spectral.attempts;
+        if (second_passes) *second_passes = g->spectral.second_passes;
         if (worst_residual) *worst_residual = g->spectral.worst_residual;
         if (bytes) *bytes = g->spectral.bytes;
     });

@@ -8,8 +8,10 @@
 //
 //  * connected components first (spectral_init.hpp:142-168): a graph with
 //    several components fails, as in the reference;
-//  * Lanczos with full re-orthogonalisation (two classical Gram-Schmidt
-//    passes per step) builds V (k vectors of length n, resident in HBM) and
+//  * Lanczos with full re-orthogonalisation (the three-term recurrence, one
+//    classical Gram-Schmidt pass against all previous vectors, and a second
+//    pass where the first cancelled most of the direction) builds V (k
+//    vectors of length n, resident in HBM) and
 //    the tridiagonal T; T's eigen-decomposition (implicit-shift QL, host)
 //    gives Ritz pairs; the Krylov dimension doubles until the num_dim + 1
 //    largest Ritz pairs have residuals below tol * theta, else the
@@ -126,9 +128,12 @@ __global__ void k_spmv(int64_t n, int G, const int64_t* __restrict__ rp, const i
 // zeroed by the caller).  A workgroup takes a slice of the rows for kDotCols
 // columns at once (blockIdx.y picks the group), so w is read once per group
 // of columns instead of once per column.
+// With 'gate' given, the kernel does nothing unless gate[0] != 0.
 constexpr int kDotCols = 8;
 __global__ void __launch_bounds__(kB) k_dots(int64_t n, int m, const double* __restrict__ V,
-                                             const double* __restrict__ w, double* __restrict__ out) {
+                                             const double* __restrict__ w, double* __restrict__ out,
+                                             const int* __restrict__ gate) {
+    if (gate && gate[0] == 0) return;  // the same for every thread of the grid
     __shared__ double part[kB / kWave][kDotCols];
     const int c0 = static_cast<int>(blockIdx.y) * kDotCols;
     const int cn = m - c0 < kDotCols ? m - c0 : kDotCols;
@@ -157,14 +162,69 @@ __global__ void __launch_bounds__(kB) k_dots(int64_t n, int m, const double* __r
     }
 }
 
-// w -= sum_c coef[c] * V[c]
-__global__ void k_subtract(int64_t n, int m, const double* __restrict__ V, const double* __restrict__ coef,
-                           double* __restrict__ w) {
+// Adds the workgroup's sum of v to *out (one atomic per workgroup); every
+// thread of the workgroup must call it.
+__device__ inline void block_add(double v, double* out) {
+    __shared__ double part[kB / kWave];
+    for (int o = kWave / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int q = 0; q < kB / kWave; ++q) t += part[q];
+        unsafeAtomicAdd(out, t);
+    }
+}
+
+// w -= sum_c coef[c] * V[c], one row per thread (many wavefronts keep the m
+// column loads of each row in flight).  With 'gate' given, the kernel does
+// nothing unless gate[0] != 0.
+__global__ void __launch_bounds__(kB) k_subtract(int64_t n, int m, const double* __restrict__ V,
+                                                 const double* __restrict__ coef, double* __restrict__ w,
+                                                 const int* __restrict__ gate) {
+    if (gate && gate[0] == 0) return;  // the same for every thread of the grid
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
     if (i >= n) return;
     double s = w[i];
     for (int c = 0; c < m; ++c) s -= coef[c] * V[static_cast<int64_t>(c) * n + i];
     w[i] = s;
+}
+
+// At most this many workgroups reduce a sum into one double: their atomic
+// adds to that one address are serialised.
+constexpr int kNormBlocks = 512;
+
+// The Lanczos three-term recurrence on w = M v1: w -= alpha * v1, and
+// w -= beta * v2 when v2 is given (beta = sqrt(*beta2), the norm of the
+// direction that became v1); then |w|^2 added to *pre.  Strides over the
+// rows with at most kNormBlocks workgroups.
+__global__ void __launch_bounds__(kB) k_recurrence(int64_t n, const double* __restrict__ v1,
+                                                   const double* __restrict__ alpha, const double* __restrict__ v2,
+                                                   const double* __restrict__ beta2, double* __restrict__ w,
+                                                   double* __restrict__ pre) {
+    const double a = alpha[0];
+    const double b = v2 ? sqrt(beta2[0]) : 0.0;
+    double sq = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kB) {
+        double s = w[i] - a * v1[i];
+        if (v2) s -= b * v2[i];
+        w[i] = s;
+        sq += s * s;
+    }
+    block_add(sq, pre);
+}
+
+// Whether a second Gram-Schmidt pass is needed (the Daniel-Gragg-Kaufman-
+// Stewart test): the first pass removed more than 1 - 1/sqrt(2) of |w|.
+__global__ void k_reorth_gate(const double* __restrict__ pre, const double* __restrict__ post,
+                              int* __restrict__ gate) {
+    gate[0] = post[0] < 0.5 * pre[0] ? 1 : 0;
+}
+
+// The direction's squared norm after the second pass, where it ran.
+__global__ void k_pick_norm(const int* __restrict__ gate, const double* __restrict__ second, double* __restrict__ nrm2) {
+    if (gate[0]) nrm2[0] = second[0];
 }
 
 // v = w / sqrt(nrm2[0]): the norm stays on the device, so a Lanczos step
@@ -316,36 +376,61 @@ bool spectral_embedding(int64_t n, const int64_t* rp, const int32_t* idx, const 
     const int kmax = static_cast<int>(std::min<int64_t>(n, prm.max_dim));
     const int kcap = std::max(k, kmax);
     DevBuf<double> w, coef, sc;
+    DevBuf<int> gates;
     w.alloc(static_cast<size_t>(n));
     coef.alloc(static_cast<size_t>(kcap) + 1);
-    sc.alloc(2 * (static_cast<size_t>(kcap) + 1));
-    double* const d_alpha = sc.p;              // alpha_j at [j]
-    double* const d_nrm2 = sc.p + kcap + 1;    // squared norm of direction j at [j]
+    sc.alloc(4 * (static_cast<size_t>(kcap) + 1));
+    gates.alloc(static_cast<size_t>(kcap) + 1);
+    double* const d_alpha = sc.p;                  // alpha_j at [j]
+    double* const d_nrm2 = sc.p + (kcap + 1);      // squared norm of direction j at [j]
+    double* const d_pre = sc.p + 2 * (kcap + 1);   // |w|^2 after the recurrence, step j
+    double* const d_nrm2b = sc.p + 3 * (kcap + 1); // |w|^2 after a second pass, step j
     UMX_HIP(hipMemsetAsync(sc.p, 0, sizeof(double) * sc.n, s));  // each slot is accumulated once
+    UMX_HIP(hipMemsetAsync(gates.p, 0, sizeof(int) * gates.n, s));
     DevBuf<double> V;
     V.alloc(static_cast<size_t>(k) * static_cast<size_t>(n));
     // v0: a fixed pseudo-random start (normalised as direction 0)
     k_fill_start<<<blocks_for(n), kB, 0, s>>>(n, prm.start_seed, w.p);
     bytes += nb;
     const unsigned gx = static_cast<unsigned>(std::min<int64_t>((n + kB - 1) / kB, 256));
-    auto dots = [&](int m, const double* VV, const double* ww, double* to) {
-        k_dots<<<dim3(gx, static_cast<unsigned>((m + kDotCols - 1) / kDotCols)), kB, 0, s>>>(n, m, VV, ww, to);
-        bytes += (m + 1) * nb;
+    const unsigned gnorm = static_cast<unsigned>(std::min<int64_t>((n + kB - 1) / kB, kNormBlocks));
+    auto dots = [&](int m, const double* VV, const double* ww, double* to, const int* gate) {
+        k_dots<<<dim3(gx, static_cast<unsigned>((m + kDotCols - 1) / kDotCols)), kB, 0, s>>>(n, m, VV, ww, to,
+                                                                                             gate);
+        if (!gate) bytes += (m + 1) * nb;
     };
     int j = 0;               // the step whose direction w holds
     bool orthogonal = false; // w already orthogonalised against V[0..j) (after an extension)
+    int gates_read = 1;      // steps whose second-pass decision is counted in 'bytes'
     std::vector<double> ha, hn, beta;
+    std::vector<int> hg;
     while (true) {
         for (;; ++j) {
             if (!orthogonal) {
-                // classical Gram-Schmidt twice against V[0..j), then |w|^2
-                for (int pass = 0; pass < 2 && j > 0; ++pass) {
-                    UMX_HIP(hipMemsetAsync(coef.p, 0, sizeof(double) * static_cast<size_t>(j), s));
-                    dots(j, V.p, w.p, coef.p);
-                    k_subtract<<<blocks_for(n), kB, 0, s>>>(n, j, V.p, coef.p, w.p);
-                    bytes += (j + 2) * nb;
+                if (j == 0) {
+                    dots(1, w.p, w.p, d_nrm2, nullptr);
+                } else {
+                    // w = M v_{j-1}: the three-term recurrence, one classical
+                    // Gram-Schmidt pass against V[0..j), and a second pass
+                    // only where the first cancelled most of w
+                    const double* v1 = V.p + static_cast<int64_t>(j - 1) * n;
+                    const double* v2 = j >= 2 ? V.p + static_cast<int64_t>(j - 2) * n : nullptr;
+                    k_recurrence<<<gnorm, kB, 0, s>>>(n, v1, d_alpha + (j - 1), v2, d_nrm2 + (j - 1), w.p,
+                                                      d_pre + j);
+                    bytes += (v2 ? 4 : 3) * nb;
+                    for (int pass = 0; pass < 2; ++pass) {
+                        const int* gate = pass == 0 ? nullptr : gates.p + j;
+                        UMX_HIP(hipMemsetAsync(coef.p, 0, sizeof(double) * static_cast<size_t>(j), s));
+                        dots(j, V.p, w.p, coef.p, gate);
+                        k_subtract<<<blocks_for(n), kB, 0, s>>>(n, j, V.p, coef.p, w.p, gate);
+                        dots(1, w.p, w.p, pass == 0 ? d_nrm2 + j : d_nrm2b + j, gate);
+                        if (pass == 0) {
+                            bytes += (j + 2) * nb;
+                            k_reorth_gate<<<1, 1, 0, s>>>(d_pre + j, d_nrm2 + j, gates.p + j);
+                        }
+                    }
+                    k_pick_norm<<<1, 1, 0, s>>>(gates.p + j, d_nrm2b + j, d_nrm2 + j);
                 }
-                dots(1, w.p, w.p, d_nrm2 + j);
                 UMX_HIP(hipGetLastError());
             }
             orthogonal = false;
@@ -356,13 +441,21 @@ bool spectral_embedding(int64_t n, const int64_t* rp, const int32_t* idx, const 
             k_spmv<<<blocks_for(n * G), kB, 0, s>>>(n, G, rp, idx, a.p, vj, w.p);
             UMX_HIP(hipGetLastError());
             bytes += 2 * nb + spmv_bytes;
-            dots(1, vj, w.p, d_alpha + j);
+            dots(1, vj, w.p, d_alpha + j, nullptr);
         }
         ha.resize(static_cast<size_t>(k));
         hn.resize(static_cast<size_t>(k) + 1);
+        hg.resize(static_cast<size_t>(k) + 1);
         UMX_HIP(hipMemcpyAsync(ha.data(), d_alpha, sizeof(double) * ha.size(), hipMemcpyDeviceToHost, s));
         UMX_HIP(hipMemcpyAsync(hn.data(), d_nrm2, sizeof(double) * hn.size(), hipMemcpyDeviceToHost, s));
+        UMX_HIP(hipMemcpyAsync(hg.data(), gates.p, sizeof(int) * hg.size(), hipMemcpyDeviceToHost, s));
         UMX_HIP(hipStreamSynchronize(s));
+        for (; gates_read <= k; ++gates_read) {
+            if (hg[gates_read]) {
+                bytes += (2.0 * gates_read + 5) * nb;
+                if (rep) ++rep->second_passes;
+            }
+        }
         // beta_{j-1} = |w_j|; a vanishing direction means an invariant
         // subspace (T is exact): the steps after it are discarded
         beta.assign(static_cast<size_t>(k), 0.0);

@@ -150,6 +150,7 @@ struct SpectralReport {
     int64_t components = 0;
     int krylov_dim = 0;        // Lanczos vectors behind the final Ritz pairs
     int attempts = 0;          // convergence checks (the Krylov dimension doubles between them)
+    int second_passes = 0;     // steps that needed a second Gram-Schmidt pass
     double worst_residual = 0;
     double bytes = 0;          // algorithmic HBM bytes of the kernels launched (each array counted once per read)
     std::vector<double> eigenvalues;  // of 2I - L, descending, num_dim + 1 of them
