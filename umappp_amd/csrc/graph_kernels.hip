@@ -502,7 +502,9 @@ __global__ void __launch_bounds__(kBlock) k_reciprocal(int64_t n, int G, const i
         kept += keep;
         if (mirror) atomicAdd(reinterpret_cast<unsigned long long*>(inc_cnt + t), 1ULL);
     }
-    if (kept) atomicAdd(reinterpret_cast<unsigned long long*>(keep_cnt + row), static_cast<unsigned long long>(kept));
+    // the row's G lanes (aligned, all active) sum their counts; one store
+    for (int o = G / 2; o >= 1; o >>= 1) kept += __shfl_xor(kept, o, kWave);
+    if (lane == 0) keep_cnt[row] = kept;
 }
 
 __global__ void __launch_bounds__(kBlock) k_add_counts(const int64_t* __restrict__ a, const int64_t* __restrict__ b,
