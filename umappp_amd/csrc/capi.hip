@@ -1575,8 +1575,7 @@ int umx_layout_autotune(umx_layout* L, int trial_epochs, int* tile_cols, int* wa
         require(L->mode == UMX_LAYOUT_PULL && L->dim <= 4, UMX_ERR_STATE,
                 "the autotuner shapes the pull-mode kernel (num_dim <= 4)");
         const auto key = std::make_tuple(L->dtype, L->dim, L->n, L->nnz, L->n_inst(), L->total_epochs);
-        ensure_items(L);  // resolves automatic placement groups
-        TuneChoice best{L->tile_cols, L->waves, L->groups};
+        TuneChoice best{};
         bool cached = false;
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -1586,6 +1585,12 @@ int umx_layout_autotune(umx_layout* L, int trial_epochs, int* tile_cols, int* wa
                 cached = true;
             }
         }
+        if (!cached) {
+            ensure_items(L);  // resolves automatic placement groups
+            best = TuneChoice{L->tile_cols, L->waves, L->groups};
+        }
+        // (a cached shape is applied before any work items are built for
+        // another shape: the next run builds them once)
         const int warm = 4;
         if (!cached && L->current_epoch + warm + trial_epochs <= L->total_epochs) {
             const bool timing = L->timing;
