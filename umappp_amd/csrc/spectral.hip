@@ -57,12 +57,19 @@ __device__ inline int32_t cc_find(const int32_t* parent, int32_t x) {
     return x;
 }
 
-__global__ void k_cc_hook(int64_t n, const int64_t* __restrict__ rp, const int32_t* __restrict__ idx,
+// One round of hooking, G lanes per row: the row's root is found once, each
+// lane hooks the larger of it and an edge's root under the smaller.  Parents
+// only decrease (no cycles) and rounds repeat until no edge joins two roots,
+// so a root that another lane re-hooks meanwhile costs a round, not
+// correctness.
+__global__ void k_cc_hook(int64_t n, int G, const int64_t* __restrict__ rp, const int32_t* __restrict__ idx,
                           int32_t* parent, int* changed) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+    const int64_t i = gid / G;
+    const int lane = static_cast<int>(gid & (G - 1));
     if (i >= n) return;
-    for (int64_t e = rp[i]; e < rp[i + 1]; ++e) {
-        const int32_t a = cc_find(parent, static_cast<int32_t>(i));
+    const int32_t a = cc_find(parent, static_cast<int32_t>(i));
+    for (int64_t e = rp[i] + lane; e < rp[i + 1]; e += G) {
         const int32_t b = cc_find(parent, idx[e]);
         if (a != b) {
             const int32_t hi = a > b ? a : b, lo = a > b ? b : a;
@@ -103,12 +110,16 @@ __global__ void k_inv_sqrt_degree(int64_t n, const int64_t* __restrict__ rp, con
     isd[i] = 1.0 / sqrt(s);  // spectral_init.hpp:45-49 (degree > 0 in a connected graph with n > 1)
 }
 
+// a = D^-1/2 W D^-1/2 entry by entry, G lanes per row (coalesced).
 template <typename T>
-__global__ void k_norm_values(int64_t n, const int64_t* __restrict__ rp, const int32_t* __restrict__ idx,
+__global__ void k_norm_values(int64_t n, int G, const int64_t* __restrict__ rp, const int32_t* __restrict__ idx,
                               const T* __restrict__ val, const double* __restrict__ isd, double* __restrict__ a) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+    const int64_t i = gid / G;
+    const int lane = static_cast<int>(gid & (G - 1));
     if (i >= n) return;
-    for (int64_t e = rp[i]; e < rp[i + 1]; ++e) a[e] = static_cast<double>(val[e]) * isd[idx[e]] * isd[i];
+    const double si = isd[i];
+    for (int64_t e = rp[i] + lane; e < rp[i + 1]; e += G) a[e] = static_cast<double>(val[e]) * isd[idx[e]] * si;
 }
 
 // y = x + A x (M = 2I - L), one row per group of 'G' lanes.
@@ -257,16 +268,35 @@ __global__ void k_ritz(int64_t n, int k, int d, const double* __restrict__ V, co
     }
 }
 
-// Entry of largest magnitude per column (value with sign), by atomics on the
-// magnitude bits then a second pass that records a signed value.
-__global__ void k_col_absmax(int64_t n, int d, const double* __restrict__ U, unsigned long long* __restrict__ mag) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
-    if (i >= n) return;
+// Entry of largest magnitude per column (value with sign): the magnitude
+// bits' maximum (|v| >= 0 orders like its bits) over a grid of at most
+// kNormBlocks workgroups striding over the rows, one atomic per workgroup and
+// column (the atomics of many workgroups on one address serialise), then a
+// second pass that records a signed value.
+__global__ void __launch_bounds__(kB) k_col_absmax(int64_t n, int d, const double* __restrict__ U,
+                                                   unsigned long long* __restrict__ mag) {
+    __shared__ unsigned long long part[kB / kWave];
     for (int c = 0; c < d; ++c) {
-        const double v = fabs(U[i * d + c]);
-        unsigned long long b;
-        __builtin_memcpy(&b, &v, sizeof(b));
-        atomicMax(mag + c, b);
+        unsigned long long b = 0;
+        for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < n;
+             i += static_cast<int64_t>(gridDim.x) * kB) {
+            const double v = fabs(U[i * d + c]);
+            unsigned long long t;
+            __builtin_memcpy(&t, &v, sizeof(t));
+            b = t > b ? t : b;
+        }
+        for (int o = kWave / 2; o >= 1; o >>= 1) {
+            const unsigned long long t = __shfl_xor(b, o, kWave);
+            b = t > b ? t : b;
+        }
+        if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = b;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long m = 0;
+            for (int q = 0; q < kB / kWave; ++q) m = part[q] > m ? part[q] : m;
+            if (m) atomicMax(mag + c, m);
+        }
+        __syncthreads();  // part is reused for the next column
     }
 }
 
@@ -315,14 +345,17 @@ int64_t count_components(int64_t n, const int64_t* rp, const int32_t* idx, hipSt
     UMX_HIP(hipMemsetAsync(changed.p, 0, sizeof(int), s));
     k_check_indices<<<blocks_for(n), kB, 0, s>>>(n, rp, idx, changed.p);
     int bad = 0;
+    int64_t nnz = 0;
     UMX_HIP(hipMemcpyAsync(&bad, changed.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    UMX_HIP(hipMemcpyAsync(&nnz, rp + n, sizeof(nnz), hipMemcpyDeviceToHost, s));
     UMX_HIP(hipStreamSynchronize(s));
     if (bad) throw Error(-2, "neighbour index out of range");
+    const int G = lanes_for_degree(std::max<int64_t>(1, (nnz + n - 1) / n));  // from the average degree
     k_cc_init<<<blocks_for(n), kB, 0, s>>>(n, parent.p);
     for (int round = 0; round < 10000; ++round) {
         int h = 0;
         UMX_HIP(hipMemsetAsync(changed.p, 0, sizeof(int), s));
-        k_cc_hook<<<blocks_for(n), kB, 0, s>>>(n, rp, idx, parent.p, changed.p);
+        k_cc_hook<<<blocks_for(n * G), kB, 0, s>>>(n, G, rp, idx, parent.p, changed.p);
         k_cc_compress<<<blocks_for(n), kB, 0, s>>>(n, parent.p);
         UMX_HIP(hipGetLastError());
         UMX_HIP(hipMemcpyAsync(&h, changed.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -360,11 +393,11 @@ bool spectral_embedding(int64_t n, const int64_t* rp, const int32_t* idx, const 
     isd.alloc(static_cast<size_t>(n));
     a.alloc(static_cast<size_t>(nnz));
     k_inv_sqrt_degree<T><<<blocks_for(n), kB, 0, s>>>(n, rp, val, isd.p);
-    k_norm_values<T><<<blocks_for(n), kB, 0, s>>>(n, rp, idx, val, isd.p, a.p);
+    const int G = lanes_for_degree(std::max<int64_t>(1, (nnz + n - 1) / n));  // from the average degree
+    k_norm_values<T><<<blocks_for(n * G), kB, 0, s>>>(n, G, rp, idx, val, isd.p, a.p);
     UMX_HIP(hipGetLastError());
     bytes += (8.0 * (n + 1) + sizeof(T) * nnz + nb) + (8.0 * (n + 1) + (4.0 + sizeof(T) + 8.0) * nnz + nb);
     (void)max_degree;
-    const int G = lanes_for_degree(std::max<int64_t>(1, (nnz + n - 1) / n));  // from the average degree
     const double spmv_bytes = 8.0 * (n + 1) + 12.0 * static_cast<double>(nnz) + 2.0 * nb;
 
     // One Lanczos run, extended when the Ritz pairs have not converged: the
@@ -525,7 +558,7 @@ bool spectral_embedding(int64_t n, const int64_t* rp, const int32_t* idx, const 
         neg.alloc(static_cast<size_t>(d));
         UMX_HIP(hipMemsetAsync(mag.p, 0, sizeof(unsigned long long) * d, s));
         UMX_HIP(hipMemsetAsync(neg.p, 0, sizeof(int) * d, s));
-        k_col_absmax<<<blocks_for(n), kB, 0, s>>>(n, d, U.p, mag.p);
+        k_col_absmax<<<gnorm, kB, 0, s>>>(n, d, U.p, mag.p);
         k_col_sign<<<blocks_for(n), kB, 0, s>>>(n, d, U.p, mag.p, neg.p);
         UMX_HIP(hipGetLastError());
         std::vector<unsigned long long> hmag(static_cast<size_t>(d));
