@@ -210,11 +210,14 @@ def test_batched_autotune_restores_state(umx):
     assert np.array_equal(after.epoch_of_next_negative_sample, before.epoch_of_next_negative_sample)
     assert np.array_equal(lay.get_embedding(), emb0)
     assert not lay.tallies().any()
-    # a second handle of the same shape takes the cached choice
-    lay2, *_ = _batch_setup(umx, sizes, epochs, seeds)
+    # a second handle of the same shape takes the cached choice (applied
+    # before its work items are built) and runs bit-exact with it
+    lay2, off2, _, orcs2, emb02 = _batch_setup(umx, sizes, epochs, seeds)
     assert lay2.autotune(2) == (tile, waves, groups)
     lay.run(50)
     _check_schedule(lay, off, orcs, emb0, 50)
+    lay2.run(50)
+    _check_schedule(lay2, off2, orcs2, emb02, 50)
 
 
 def test_batched_layout_restrictions(umx):
