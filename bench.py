@@ -131,8 +131,8 @@ def main():
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: the library runs one GPU per process and has no "
+              "multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -229,7 +229,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(x if False else None, kidx, kdist, n, k, T, nsr, a, b, args.cpu_seconds, U)
+        cpu = cpu_baseline(kidx, kdist, n, k, T, nsr, a, b, args.cpu_seconds, U)
 
     out = {
         "metric": METRIC,
@@ -859,7 +859,7 @@ def _cpu_model():
     return "unknown"
 
 
-def cpu_baseline(_x, kidx, kdist, n, k, T, nsr, a, b, target_s, U_gpu):
+def cpu_baseline(kidx, kdist, n, k, T, nsr, a, b, target_s, U_gpu):
     """The reference CPU path (the oracle's restatement) on the host cores:
     graph build on the full input, then the conflict-scheduled parallel
     layout (optimize_layout_parallel) for as many epochs as fit ~target_s."""
