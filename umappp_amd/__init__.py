@@ -462,7 +462,9 @@ class Layout:
     def set_work_split(self, fires_per_item: float = 32.0, edges_per_item: int = 2**31 - 1):
         _check(lib().umx_layout_set_work_split(self._h, float(fires_per_item), int(edges_per_item)))
 
-    def set_tile(self, tile_cols: int = 8):
+    def set_tile(self, tile_cols: int = 0):
+        """LDS tile of the pull kernel: 4, 8, 16 or 32 columns, or 0 to choose
+        from the number of chunks (the default)."""
         _check(lib().umx_layout_set_tile(self._h, int(tile_cols)))
 
     def work_items(self):
