@@ -249,7 +249,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "stages_ms_per_step": {key: round(1000 * v / args.steps, 3) for key, v in stage.items()},
-        "knn_input": {"ms": round(knn_t["knn_ms"], 1), "kernel": "k_knn<float,16,32,16> (exact, FP64 distances)",
+        "knn_input": {"ms": round(knn_t["knn_ms"], 1), "kernel": "k_knn_qlds<16,32,16> (exact, FP64 distances)",
                       "fp64_tflops": round(3.0 * n * n * cfg["dim"] / (knn_t["knn_ms"] / 1000.0) / 1e12, 2),
                       "fp64_peak_tflops": 78.6},
         "layout_kernel_ms_per_step": round(kernel_ms / args.steps, 3),
@@ -716,7 +716,7 @@ def main_knn(args, umx, torch, dev):
         "config": {"workload": cfg["label"], "n": n, "dim": dim, "k": k},
         "roofline": {"bound": "fp64-valu", "achieved": round(tflops, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(tflops / FP64_PEAK_TFLOPS, 4), "traffic": None,
-                     "kernel": "k_knn<float,16,32,16>", "avg_launch_us": round(sec * 1e6, 1),
+                     "kernel": "k_knn_qlds<16,32,16>", "avg_launch_us": round(sec * 1e6, 1),
                      "ops_per_pair_dim": 3, "no_fma_issue_ceiling_tflops": FP64_PEAK_TFLOPS / 2},
         "cpu_baseline": cpu,
     }
