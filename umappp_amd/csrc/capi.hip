@@ -666,7 +666,7 @@ struct umx_layout {
     DevBuf<float> ef, nf, gf, mf;  // eps, next, next_neg, embedding (float)
     DevBuf<double> ed, nd, gd, md;  // same (double)
     DevBuf<unsigned long long> tally;
-    DevBuf<unsigned long long> tally_part;  // partial tallies of one launch (zero between launches)
+    DevBuf<unsigned long long> tally_part;  // ring of per-epoch partial tallies (zero between runs)
     DevBuf<unsigned long long> cursor;  // serial replay cursor + overrun flag
     int dim = 0;
     int mode = UMX_LAYOUT_PULL;
@@ -987,7 +987,7 @@ void layout_from_graph(umx_layout* L, const umx_graph* g, const std::vector<int3
     UMX_HIP(hipMemcpyAsync(&L->nnz, L->rp.p + L->n, sizeof(int64_t), hipMemcpyDeviceToHost, L->s));
     L->tally.alloc(static_cast<size_t>(3) * static_cast<size_t>(L->total_epochs));
     UMX_HIP(hipMemsetAsync(L->tally.p, 0, sizeof(unsigned long long) * L->tally.n, L->s));
-    L->tally_part.alloc(static_cast<size_t>(umx::kTallySlots) * umx::kTallyStride);
+    L->tally_part.alloc(static_cast<size_t>(umx::kTallyRing) * umx::kTallySlots * umx::kTallyStride);
     UMX_HIP(hipMemsetAsync(L->tally_part.p, 0, sizeof(unsigned long long) * L->tally_part.n, L->s));
     L->cursor.alloc(2);
     UMX_HIP(hipMemsetAsync(L->cursor.p, 0, sizeof(unsigned long long) * 2, L->s));
@@ -1071,8 +1071,11 @@ int effective_tile(const umx_layout* L) {
 
 // One epoch of the sliced-ELL kernel: 'epoch' >= 0, or -1 to take it from
 // the device counter and advance that (captured blocks).  Optional events
-// bracket the epoch kernel alone; the tally fold follows them.
-void launch_sell_once(umx_layout* L, int epoch, hipEvent_t start = nullptr, hipEvent_t stop = nullptr) {
+// bracket the epoch kernel alone; the tally fold follows them unless 'fold'
+// is false (the caller then folds the epoch with fold_layout_epochs before
+// its ring entry is reused).
+void launch_sell_once(umx_layout* L, int epoch, hipEvent_t start = nullptr, hipEvent_t stop = nullptr,
+                      bool fold = true) {
     if (start) UMX_HIP(hipEventRecord(start, L->s));
     if (L->dtype == UMX_F32) {
         auto v = view_of<float>(L, L->ef, L->nf, L->gf, L->mf);
@@ -1084,7 +1087,23 @@ void launch_sell_once(umx_layout* L, int epoch, hipEvent_t start = nullptr, hipE
         umx::launch_layout_sell<double>(v, epoch, effective_tile(L), L->waves, L->s);
     }
     if (stop) UMX_HIP(hipEventRecord(stop, L->s));
-    umx::finish_layout_epoch(L->tally_part.p, L->tally.p, epoch, L->epoch_ctr.p, L->s);
+    if (fold) umx::finish_layout_epoch(L->tally_part.p, L->tally.p, epoch, L->epoch_ctr.p, L->s);
+}
+
+// Epochs [first, first + count) launched one by one, their tallies folded by
+// one launch per kTallyRing epochs (a fold launch after every epoch cost a
+// few microseconds of kernel and launch gap each).  With 'timed', event pair
+// q brackets epoch kernel q.
+void launch_sell_run(umx_layout* L, int first, int count, bool timed) {
+    int pending = 0;
+    for (int q = 0; q < count; ++q) {
+        launch_sell_once(L, first + q, timed ? L->events[2 * q] : nullptr, timed ? L->events[2 * q + 1] : nullptr,
+                         false);
+        if (++pending == umx::kTallyRing || q == count - 1) {
+            umx::fold_layout_epochs(L->tally_part.p, L->tally.p, first + q + 1 - pending, pending, L->s);
+            pending = 0;
+        }
+    }
 }
 
 // Identity of what a captured block would launch: the kernel arguments,
@@ -1143,14 +1162,14 @@ void run_sell(umx_layout* L, int first, int count) {
     ensure_tables(L);
     if (L->timing) {
         prepare_events(L, count);
-        for (int q = 0; q < count; ++q) launch_sell_once(L, first + q, L->events[2 * q], L->events[2 * q + 1]);
+        launch_sell_run(L, first, count, true);
     } else if (L->graph_block > 0 && L->trace_cap == 0 && count >= L->graph_block) {
         umx::set_epoch_counter(L->epoch_ctr.p, first, L->s);
         int done = 0;
         for (; count - done >= L->graph_block; done += L->graph_block) replay_block(L, L->graph_block);
-        for (; done < count; ++done) launch_sell_once(L, first + done);
+        launch_sell_run(L, first + done, count - done, false);
     } else {
-        for (int q = 0; q < count; ++q) launch_sell_once(L, first + q);
+        launch_sell_run(L, first, count, false);
     }
     UMX_HIP(hipStreamSynchronize(L->s));
     collect_events(L, count);

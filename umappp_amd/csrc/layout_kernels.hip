@@ -656,7 +656,10 @@ __global__ void __launch_bounds__(kBlock) k_instance_fires(const int64_t* __rest
 uint64_t counter_epoch_key(uint64_t seed, int epoch) { return mix64(seed + kGold * (static_cast<uint64_t>(epoch) + 1)); }
 
 template <typename T>
-void launch_layout_epoch(const LayoutView<T>& v, const EpochArgs& ea, hipStream_t s) {
+void launch_layout_epoch(const LayoutView<T>& view, const EpochArgs& ea, hipStream_t s) {
+    // the epoch's partial tallies go to its ring entry
+    LayoutView<T> v = view;
+    v.tally_part += tally_ring_offset(ea.epoch);
     const EpochConst ec{ea.epoch, ea.ekey};
     const T ep = static_cast<T>(ea.epoch);
     const T alpha = static_cast<T>(ea.alpha);

@@ -306,19 +306,20 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
             }
         }
     }
-    tally_add(v.tally_part, c, fires, draws, self_hits);
+    tally_add(v.tally_part + tally_ring_offset(epn), c, fires, draws, self_hits);
 }
 
 __global__ void k_set_epoch(int* ctr, int epoch) { ctr[0] = epoch; }
 
-// Folds the partial tallies of one epoch launch into the epoch's tally and
-// clears them (one lane per slot); with epoch < 0 the epoch is epoch_ctr[0],
-// advanced by one afterwards.
+// Folds the partial tallies of one epoch launch (ring entry of the epoch)
+// into the epoch's tally and clears them (one lane per slot); with epoch < 0
+// the epoch is epoch_ctr[0], advanced by one afterwards.
 __global__ void __launch_bounds__(kTallySlots) k_finish_epoch(unsigned long long* __restrict__ part,
                                                              unsigned long long* __restrict__ tally, int epoch,
                                                              int* __restrict__ ctr) {
     static_assert(kTallySlots == kWave, "one lane per partial-tally slot");
-    unsigned long long* p = part + static_cast<int64_t>(threadIdx.x) * kTallyStride;
+    const int e = epoch >= 0 ? epoch : ctr[0];
+    unsigned long long* p = part + tally_ring_offset(e) + static_cast<int64_t>(threadIdx.x) * kTallyStride;
     unsigned long long c[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) c[q] = atomicExch(p + q, 0ULL);
@@ -328,11 +329,35 @@ __global__ void __launch_bounds__(kTallySlots) k_finish_epoch(unsigned long long
         for (int q = 0; q < 3; ++q) c[q] += __shfl_xor(c[q], m, kWave);
     }
     if (threadIdx.x == 0) {
-        const int e = epoch >= 0 ? epoch : ctr[0];
 #pragma unroll
         for (int q = 0; q < 3; ++q)
             if (c[q]) atomicAdd(tally + 3 * static_cast<int64_t>(e) + q, c[q]);
         if (epoch < 0) ctr[0] = e + 1;
+    }
+}
+
+// The same fold for epochs first .. first + gridDim.x - 1, one workgroup of
+// one wavefront per epoch (the epochs lie in distinct ring entries).
+__global__ void __launch_bounds__(kTallySlots) k_fold_epochs(unsigned long long* __restrict__ part,
+                                                            unsigned long long* __restrict__ tally, int first) {
+    static_assert(kTallySlots == kWave, "one lane per partial-tally slot");
+    const int e = first + static_cast<int>(blockIdx.x);
+    unsigned long long* p = part + tally_ring_offset(e) + static_cast<int64_t>(threadIdx.x) * kTallyStride;
+    unsigned long long c[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        c[q] = p[q];
+        p[q] = 0ULL;
+    }
+#pragma unroll
+    for (int m = kWave / 2; m >= 1; m >>= 1) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) c[q] += __shfl_xor(c[q], m, kWave);
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            if (c[q]) atomicAdd(tally + 3 * static_cast<int64_t>(e) + q, c[q]);
     }
 }
 
@@ -428,6 +453,14 @@ void launch_layout_sell(const LayoutView<T>& v, int epoch, int tile_cols, int wa
 void finish_layout_epoch(unsigned long long* tally_part, unsigned long long* tally, int epoch, int* epoch_ctr,
                          hipStream_t s) {
     k_finish_epoch<<<1, kTallySlots, 0, s>>>(tally_part, tally, epoch, epoch_ctr);
+    UMX_HIP(hipGetLastError());
+}
+
+void fold_layout_epochs(unsigned long long* tally_part, unsigned long long* tally, int first, int count,
+                        hipStream_t s) {
+    if (count <= 0) return;
+    if (count > kTallyRing || first < 0) throw Error(-1, "tally fold covers at most one ring of epochs");
+    k_fold_epochs<<<static_cast<unsigned>(count), kTallySlots, 0, s>>>(tally_part, tally, first);
     UMX_HIP(hipGetLastError());
 }
 

@@ -215,9 +215,10 @@ struct LayoutView {
     int mode;
     T a, b, gamma, nsr;
     unsigned long long* tally;  // 3 counters per epoch
-    // Per-epoch partial tallies of the parallel kernels: kTallySlots slots of
-    // kTallyStride counters (the first 3 used), folded into tally[] and
-    // cleared by finish_layout_epoch after every launch.
+    // Per-epoch partial tallies of the parallel kernels: a ring of
+    // kTallyRing entries of kTallySlots slots of kTallyStride counters (the
+    // first 3 used), folded into tally[] and cleared by finish_layout_epoch
+    // after a launch or by fold_layout_epochs after a run of launches.
     unsigned long long* tally_part;
     // Work items of the production kernels: item m covers edges
     // [item_begin[m], item_begin[m] + item_len[m]) of row
@@ -263,11 +264,26 @@ void launch_layout_epoch(const LayoutView<T>& v, const EpochArgs& ea, hipStream_
 constexpr int kTallySlots = 64;
 constexpr int kTallyStride = 32;
 
+// The partial tallies of kTallyRing consecutive epochs are kept apart: epoch
+// e adds into ring entry e % kTallyRing (kTallySlots * kTallyStride
+// counters), so a run of epochs can be folded by one launch per kTallyRing
+// epochs instead of one launch after every epoch.
+constexpr int kTallyRing = 64;
+constexpr int64_t tally_ring_offset(int epoch) {
+    return static_cast<int64_t>(epoch % kTallyRing) * kTallySlots * kTallyStride;
+}
+
 // After each parallel epoch launch: adds the partial tallies into the tally
 // of the epoch (epoch >= 0, else the epoch in epoch_ctr[0], which is then
 // advanced by one) and clears them.  One wavefront; safe in graph capture.
 void finish_layout_epoch(unsigned long long* tally_part, unsigned long long* tally, int epoch, int* epoch_ctr,
                          hipStream_t s);
+
+// Folds the partial tallies of epochs [first, first + count) into their
+// tallies and clears them: one launch, count <= kTallyRing.  Launched after
+// the epochs' kernels on the same stream.
+void fold_layout_epochs(unsigned long long* tally_part, unsigned long long* tally, int first, int count,
+                        hipStream_t s);
 
 template <typename T>
 void launch_layout_serial(const LayoutView<T>& v, const EpochArgs& ea, const int32_t* samples, int64_t count,
