@@ -539,18 +539,56 @@ __global__ void __launch_bounds__(kBlock) k_stage_mirrors(int64_t n, int G, cons
 // Sorts each row's staged mirrored entries by index (the staging atomics leave
 // them in arbitrary order; keys are unique within a row, one per source row).
 // Rows with at most kSmallMirrors entries: a group of kSmallMirrors lanes per
-// row ranks its keys with shuffles.  Longer rows are listed for
-// k_sort_mirrors_mid (up to 64 entries) or k_sort_mirrors_big (more).
+// row ranks its keys with shuffles.  Longer rows are listed by
+// k_list_long_rows for k_sort_mirrors_mid (up to 64 entries) or
+// k_sort_mirrors_big (more).
 constexpr int kSmallMirrors = 32;
 constexpr int kBigSortCap = 8192;  // LDS bitonic sort capacity per row
+constexpr int kListRowsPerThread = 4;
+
+// Appends the rows with more than kSmallMirrors staged entries to the mid
+// (at most 64) or big list, in no particular order.  A workgroup takes
+// kBlock * kListRowsPerThread consecutive rows, numbers its long rows with
+// LDS atomics and reserves their list slots with one global atomic per list
+// (one global atomic per long row queued on the two counters: about 1 ms for
+// the hub rows of a 1M-point kNN graph).
+__global__ void __launch_bounds__(kBlock) k_list_long_rows(int64_t n, const int64_t* __restrict__ inc_rp,
+                                                         int64_t* __restrict__ mid_rows,
+                                                         unsigned long long* __restrict__ n_mid,
+                                                         int64_t* __restrict__ big_rows,
+                                                         unsigned long long* __restrict__ n_big) {
+    __shared__ unsigned int cnt[2];
+    __shared__ unsigned long long base[2];
+    if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kBlock * kListRowsPerThread + threadIdx.x;
+    int cls[kListRowsPerThread];
+    unsigned int slot[kListRowsPerThread];
+#pragma unroll
+    for (int j = 0; j < kListRowsPerThread; ++j) {
+        const int64_t row = r0 + static_cast<int64_t>(j) * kBlock;
+        const int64_t m = row < n ? inc_rp[row + 1] - inc_rp[row] : 0;
+        cls[j] = m <= kSmallMirrors ? -1 : (m <= kWave ? 0 : 1);
+        slot[j] = cls[j] >= 0 ? atomicAdd(&cnt[cls[j]], 1u) : 0u;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const unsigned int c = cnt[threadIdx.x];
+        base[threadIdx.x] = c ? atomicAdd(threadIdx.x == 0 ? n_mid : n_big, static_cast<unsigned long long>(c)) : 0ULL;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kListRowsPerThread; ++j) {
+        if (cls[j] >= 0) {
+            const int64_t row = r0 + static_cast<int64_t>(j) * kBlock;
+            (cls[j] == 0 ? mid_rows : big_rows)[base[cls[j]] + slot[j]] = row;
+        }
+    }
+}
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_sort_mirrors_small(int64_t n, const int64_t* __restrict__ inc_rp,
-                                                             int32_t* __restrict__ src, T* __restrict__ val,
-                                                             int64_t* __restrict__ mid_rows,
-                                                             unsigned long long* __restrict__ n_mid,
-                                                             int64_t* __restrict__ big_rows,
-                                                             unsigned long long* __restrict__ n_big) {
+                                                             int32_t* __restrict__ src, T* __restrict__ val) {
     constexpr int G = kSmallMirrors;
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     const int64_t row = gid / G;
@@ -560,10 +598,6 @@ __global__ void __launch_bounds__(kBlock) k_sort_mirrors_small(int64_t n, const 
     const int64_t ss = row < n ? inc_rp[row] : 0;
     const int64_t m = row < n ? inc_rp[row + 1] - ss : 0;
     const bool small = m <= G;
-    if (!small && lane == 0) {
-        if (m <= kWave) mid_rows[atomicAdd(n_mid, 1ULL)] = row;
-        else big_rows[atomicAdd(n_big, 1ULL)] = row;
-    }
     const bool mine = small && lane < m;
     const int32_t key = mine ? src[ss + lane] : 0x7fffffff;
     const T v = mine ? val[ss + lane] : static_cast<T>(0);
@@ -1025,9 +1059,12 @@ void run_combine(int64_t n, const int64_t* rp, const int32_t* idx, const T* val,
         Tmp<int64_t> big_rows(static_cast<size_t>(n), s);
         Tmp<unsigned long long> counts(2, s);  // mid, big
         UMX_HIP(hipMemsetAsync(counts.p, 0, 2 * sizeof(unsigned long long), s));
-        k_sort_mirrors_small<T><<<grid_for(n * kSmallMirrors), kBlock, 0, s>>>(n, inc_rp.p, stage_src.p, stage_val.p,
-                                                                               mid_rows.p, counts.p, big_rows.p,
-                                                                               counts.p + 1);
+        const int64_t list_rows = static_cast<int64_t>(kBlock) * kListRowsPerThread;
+        k_list_long_rows<<<static_cast<unsigned>((n + list_rows - 1) / list_rows), kBlock, 0, s>>>(
+            n, inc_rp.p, mid_rows.p, counts.p, big_rows.p, counts.p + 1);
+        UMX_HIP(hipGetLastError());
+        k_sort_mirrors_small<T><<<grid_for(n * kSmallMirrors), kBlock, 0, s>>>(n, inc_rp.p, stage_src.p,
+                                                                               stage_val.p);
         UMX_HIP(hipGetLastError());
         unsigned long long hc[2] = {0, 0};
         UMX_HIP(hipMemcpyAsync(hc, counts.p, sizeof(hc), hipMemcpyDeviceToHost, s));
