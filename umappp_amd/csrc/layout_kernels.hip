@@ -459,6 +459,26 @@ __global__ void __launch_bounds__(kBlock) k_layout_pull_generic(LayoutView<T> v,
 // c = ceil(f / threshold) items, a cut falling where the running sum of 1/eps
 // would pass q * f / c (q = 1..c-1) or where the edge cap is reached.  The
 // same walk counts the items (WRITE = false) and then writes them.
+//
+// Both walks take kSplitBatch entries per trip: their loads (clamped to the
+// row's last entry, so unconditional) and reciprocals come first, then the
+// running sums in entry order, so the sums round exactly as one entry at a
+// time while a hub row of thousands of entries waits for one load round trip
+// per batch instead of one per entry.
+constexpr int kSplitBatch = 16;
+
+template <typename T>
+__device__ inline void split_weights(const T* __restrict__ eps, int64_t j0, int64_t r1, double (&w)[kSplitBatch]) {
+    T e[kSplitBatch];
+#pragma unroll
+    for (int u = 0; u < kSplitBatch; ++u) e[u] = eps[j0 + u < r1 ? j0 + u : r1 - 1];
+    // keeps every load ahead of the first conversion (otherwise the
+    // scheduler interleaves them and waits for each load in turn)
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < kSplitBatch; ++u) w[u] = 1.0 / static_cast<double>(e[u]);
+}
+
 template <typename T, bool WRITE>
 __device__ inline int64_t split_row(const int64_t* __restrict__ rp, const T* __restrict__ eps, int64_t i,
                                     double fires_split, int edges_cap, int64_t m, int64_t* __restrict__ ibeg,
@@ -466,25 +486,47 @@ __device__ inline int64_t split_row(const int64_t* __restrict__ rp, const T* __r
     const int64_t r0 = rp[i], r1 = rp[i + 1];
     if (r1 == r0) return 0;
     double f = 0;
-    for (int64_t j = r0; j < r1; ++j) f += 1.0 / static_cast<double>(eps[j]);
+    for (int64_t j0 = r0; j0 < r1; j0 += kSplitBatch) {
+        double w[kSplitBatch];
+        split_weights(eps, j0, r1, w);
+#pragma unroll
+        for (int u = 0; u < kSplitBatch; ++u)
+            if (j0 + u < r1) f += w[u];
+    }
     const double c = f > fires_split ? ceil(f / fires_split) : 1.0;
     const int32_t tag = static_cast<int32_t>(static_cast<uint32_t>(i) | (split_tag ? 0x80000000u : 0u));
+    if (c == 1.0 && r1 - r0 <= edges_cap) {
+        // no cut can occur below (q < c never holds; no piece reaches the cap)
+        if (WRITE) {
+            ibeg[m] = r0;
+            ilen[m] = static_cast<int32_t>(r1 - r0);
+            irow[m] = tag;
+        }
+        return 1;
+    }
     int64_t items = 0, start = r0;
     double acc = 0, q = 1;
-    for (int64_t j = r0; j < r1; ++j) {
-        const double w = 1.0 / static_cast<double>(eps[j]);
-        const bool cut = j > start && ((q < c && acc + w > q * f / c) || j - start >= edges_cap);
-        if (cut) {
-            if (WRITE) {
-                ibeg[m + items] = start;
-                ilen[m + items] = static_cast<int32_t>(j - start);
-                irow[m + items] = tag;
+    for (int64_t j0 = r0; j0 < r1; j0 += kSplitBatch) {
+        double wb[kSplitBatch];
+        split_weights(eps, j0, r1, wb);
+#pragma unroll
+        for (int u = 0; u < kSplitBatch; ++u) {
+            const int64_t j = j0 + u;
+            if (j >= r1) break;
+            const double w = wb[u];
+            const bool cut = j > start && ((q < c && acc + w > q * f / c) || j - start >= edges_cap);
+            if (cut) {
+                if (WRITE) {
+                    ibeg[m + items] = start;
+                    ilen[m + items] = static_cast<int32_t>(j - start);
+                    irow[m + items] = tag;
+                }
+                ++items;
+                if (acc + w > q * f / c) q += 1;
+                start = j;
             }
-            ++items;
-            if (acc + w > q * f / c) q += 1;
-            start = j;
+            acc += w;
         }
-        acc += w;
     }
     if (WRITE) {
         ibeg[m + items] = start;

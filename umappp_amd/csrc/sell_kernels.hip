@@ -85,7 +85,41 @@ __global__ void __launch_bounds__(kWave) k_fill_sell(const int64_t* __restrict__
     const int64_t e0 = live ? item_begin[m] : 0;
     const int len = live ? item_len[m] : 0;
     const int32_t self = live ? static_cast<int32_t>(static_cast<uint32_t>(item_row[m]) & 0x7fffffffu) : 0;
-    for (int k = 0; k < w; ++k) {
+    // kFillUnroll columns per trip: the loads of a trip are issued together
+    // (padding slots of a nonempty item read its first edge again; empty
+    // items, which are padding, and lanes past the last item load nothing)
+    // before any store, so a long chunk waits for one round trip per trip
+    // instead of one per column
+    constexpr int kFillUnroll = 8;
+    int k = 0;
+    for (; k + kFillUnroll <= w; k += kFillUnroll) {
+        int32_t t[kFillUnroll];
+        T ep[kFillUnroll];
+        if (len > 0) {
+#pragma unroll
+            for (int u = 0; u < kFillUnroll; ++u) {
+                const int64_t e = k + u < len ? e0 + k + u : e0;
+                t[u] = tg[e];
+                ep[u] = eps[e];
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);  // every load of the trip before its stores
+#pragma unroll
+        for (int u = 0; u < kFillUnroll; ++u) {
+            const int64_t slot = b0 + static_cast<int64_t>(k + u) * kWave + lane;
+            if (k + u < len) {
+                stg[slot] = t[u];
+                seps[slot] = ep[u];
+                slot_of_edge[e0 + k + u] = slot;
+            } else {
+                stg[slot] = self;
+                seps[slot] = static_cast<T>(1);
+                snx[slot] = static_cast<T>(INFINITY);  // padding never fires
+                snn[slot] = static_cast<T>(0);
+            }
+        }
+    }
+    for (; k < w; ++k) {
         const int64_t slot = b0 + static_cast<int64_t>(k) * kWave + lane;
         if (k < len) {
             const int64_t e = e0 + k;
