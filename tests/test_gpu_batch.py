@@ -158,6 +158,21 @@ def test_batched_graph_blocks(umx):
     _check_schedule(lay, off, orcs, emb0, 40)
 
 
+def test_batched_graph_blocks_longer_than_tally_ring(umx):
+    """A captured block of 70 epochs folds its tallies in two parts (64 + 6
+    epochs, the partial-tally ring holding 64); schedule and per-epoch
+    tallies equal the oracle's, across replays and the direct launches
+    after them."""
+    sizes = [300, 500, 150]
+    epochs = [160, 120, 160]
+    seeds = [21, 22, 23]
+    lay, off, parts, orcs, emb0 = _batch_setup(umx, sizes, epochs, seeds)
+    lay.set_graph_block(70)
+    lay.run(5)
+    lay.run(160)  # two replays of one graph, then 15 direct launches
+    _check_schedule(lay, off, orcs, emb0, 160)
+
+
 def test_batched_samples_match_plain_runs(umx):
     """Per epoch, each instance's traced (kind, head, other) steps equal those
     of the same problem run alone with its seed; negatives stay inside their

@@ -1140,7 +1140,14 @@ void replay_block(umx_layout* L, int block) {
         hipGraph_t graph = nullptr;
         UMX_HIP(hipStreamBeginCapture(L->s, hipStreamCaptureModeThreadLocal));
         try {
-            for (int q = 0; q < block; ++q) launch_sell_once(L, -1);
+            // launch q of a sub-block reads the device counter + q; one fold
+            // per sub-block of at most kTallyRing epochs folds their
+            // tallies and advances the counter
+            for (int q0 = 0; q0 < block; q0 += umx::kTallyRing) {
+                const int nb = std::min(block - q0, umx::kTallyRing);
+                for (int u = 0; u < nb; ++u) launch_sell_once(L, -1 - u, nullptr, nullptr, false);
+                umx::fold_layout_block(L->tally_part.p, L->tally.p, nb, L->epoch_ctr.p, L->s);
+            }
         } catch (...) {
             (void)hipStreamEndCapture(L->s, &graph);
             if (graph) (void)hipGraphDestroy(graph);

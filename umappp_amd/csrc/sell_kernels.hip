@@ -172,8 +172,9 @@ struct SellTile {
 // takes chunks blockIdx.x * W .. + W - 1.  Each lane reads its item's
 // instance constants (observation offset and count, sampler seed, epochs,
 // learning rate); a plain layout is one instance.  The epoch is epoch_arg,
-// or v.epoch_ctr[0] when epoch_arg < 0 (captured blocks of epochs, where a
-// one-wavefront kernel advances the counter after each launch).
+// or v.epoch_ctr[0] + (-1 - epoch_arg) when epoch_arg < 0 (captured blocks
+// of epochs: launch q of a block passes -1 - q, and a one-wavefront kernel
+// after the block's launches advances the counter by the block length).
 template <typename T, int D, int TW, bool TRACE>
 __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_arg) {
     extern __shared__ __align__(16) unsigned char sell_lds[];
@@ -181,7 +182,7 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
     const int wid = threadIdx.x / kWave;
     SellTile<T, TW>& tile = reinterpret_cast<SellTile<T, TW>*>(sell_lds)[wid];
     const int64_t c = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + wid;
-    const int epn = epoch_arg >= 0 ? epoch_arg : v.epoch_ctr[0];
+    const int epn = epoch_arg >= 0 ? epoch_arg : v.epoch_ctr[0] + (-1 - epoch_arg);
     const T ep = static_cast<T>(epn);
 
     uint32_t fires = 0, draws = 0, self_hits = 0;  // one chunk's counts fit in 32 bits
@@ -370,6 +371,39 @@ __global__ void __launch_bounds__(kTallySlots) k_finish_epoch(unsigned long long
     }
 }
 
+// The fold of a captured block: epochs epoch_ctr[0] .. epoch_ctr[0] + count - 1
+// in turn (count <= kTallyRing), then the counter advanced by count.  One
+// wavefront, launched after the block's epoch kernels.
+__global__ void __launch_bounds__(kTallySlots) k_fold_block(unsigned long long* __restrict__ part,
+                                                           unsigned long long* __restrict__ tally, int count,
+                                                           int* __restrict__ ctr) {
+    static_assert(kTallySlots == kWave, "one lane per partial-tally slot");
+    const int base = ctr[0];
+    for (int b = 0; b < count; ++b) {
+        const int e = base + b;
+        unsigned long long* p = part + tally_ring_offset(e) + static_cast<int64_t>(threadIdx.x) * kTallyStride;
+        unsigned long long c[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            c[q] = p[q];
+            p[q] = 0ULL;
+        }
+#pragma unroll
+        for (int m = kWave / 2; m >= 1; m >>= 1) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) c[q] += __shfl_xor(c[q], m, kWave);
+        }
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                if (c[q]) atomicAdd(tally + 3 * static_cast<int64_t>(e) + q, c[q]);
+        }
+    }
+    // every lane read the counter before the loop; the one wavefront runs in
+    // lockstep, so no lane reads it after this store
+    if (threadIdx.x == 0) ctr[0] = base + count;
+}
+
 // The same fold for epochs first .. first + gridDim.x - 1, one workgroup of
 // one wavefront per epoch (the epochs lie in distinct ring entries).
 __global__ void __launch_bounds__(kTallySlots) k_fold_epochs(unsigned long long* __restrict__ part,
@@ -495,6 +529,14 @@ void fold_layout_epochs(unsigned long long* tally_part, unsigned long long* tall
     if (count <= 0) return;
     if (count > kTallyRing || first < 0) throw Error(-1, "tally fold covers at most one ring of epochs");
     k_fold_epochs<<<static_cast<unsigned>(count), kTallySlots, 0, s>>>(tally_part, tally, first);
+    UMX_HIP(hipGetLastError());
+}
+
+void fold_layout_block(unsigned long long* tally_part, unsigned long long* tally, int count, int* epoch_ctr,
+                       hipStream_t s) {
+    if (count <= 0) return;
+    if (count > kTallyRing) throw Error(-1, "tally fold covers at most one ring of epochs");
+    k_fold_block<<<1, kTallySlots, 0, s>>>(tally_part, tally, count, epoch_ctr);
     UMX_HIP(hipGetLastError());
 }
 

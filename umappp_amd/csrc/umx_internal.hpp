@@ -279,6 +279,12 @@ constexpr int64_t tally_ring_offset(int epoch) {
 void finish_layout_epoch(unsigned long long* tally_part, unsigned long long* tally, int epoch, int* epoch_ctr,
                          hipStream_t s);
 
+// Folds the partial tallies of epochs [c, c + count), c = epoch_ctr[0] on
+// the device, and advances the counter by count (count <= kTallyRing).  One
+// launch after a captured block's epoch kernels; safe in graph capture.
+void fold_layout_block(unsigned long long* tally_part, unsigned long long* tally, int count, int* epoch_ctr,
+                       hipStream_t s);
+
 // Folds the partial tallies of epochs [first, first + count) into their
 // tallies and clears them: one launch, count <= kTallyRing.  Launched after
 // the epochs' kernels on the same stream.
@@ -343,9 +349,10 @@ inline size_t sell_lds_bytes(size_t value_bytes, int tile_cols, int waves) {
 }
 
 // One epoch of the sliced-ELL kernel.  epoch >= 0: that epoch.  epoch < 0:
-// the epoch in v.epoch_ctr[0] on the device, which finish_layout_epoch,
-// launched right after, advances (so a captured block of launches replays
-// for any epochs).
+// epoch v.epoch_ctr[0] + (-1 - epoch), the counter on the device being
+// advanced after the block of such launches by fold_layout_block (or after
+// one launch with epoch -1 by finish_layout_epoch), so a captured block of
+// launches replays for any epochs.
 template <typename T>
 void launch_layout_sell(const LayoutView<T>& v, int epoch, int tile_cols, int waves, hipStream_t s);
 
