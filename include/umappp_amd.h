@@ -250,11 +250,11 @@ int umx_layout_set_tile(umx_layout* L, int tile_cols);
  * picks the XCD count when the instances can be balanced across it, else 1. */
 int umx_layout_set_placement(umx_layout* L, int waves, int groups);
 /* Epochs per captured HIP graph: umx_layout_run records 'epochs' launches
- * of the production kernel once (each launch reads the epoch from a counter
- * in device memory, and the one-wavefront kernel that folds the epoch's
- * tallies advances it, so one graph serves
- * every block of epochs) and replays it; 0 launches every epoch directly.
- * Default 0; timing and tracing always launch directly. */
+ * of the production kernel once (launch q of the block runs epoch c + q,
+ * c being a counter in device memory, and the one-wavefront kernel that
+ * folds the block's tallies, once per 64 epochs, advances c, so one graph
+ * serves every block of epochs) and replays it; 0 launches every epoch
+ * directly.  Default 0; timing and tracing always launch directly. */
 int umx_layout_set_graph_block(umx_layout* L, int epochs);
 /* Autotuner for the production kernel's launch shape: from the current
  * state, runs a few warm-up epochs, then trial_epochs epochs for each
