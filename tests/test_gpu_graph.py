@@ -141,6 +141,42 @@ def test_combine_ragged_hubs_bit_exact(umx, dtype, mix, seed, n, maxk, hubs):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_combine_mirror_sort_classes_bit_exact(umx, dtype):
+    """Rows whose mirrored entries fall in every sorting class of the union:
+    up to 32 (32-lane shuffle sort), 33-64 (one wavefront), 65-512 and
+    513-8192 (LDS bitonic sorts of two capacities), above 8192 (rank sort in
+    global memory).  Hubs with chosen in-degrees are planted in a sparse
+    ragged graph."""
+    n = 20000
+    rp, idx, d = random_ragged_graph(11, n, 6, dtype=dtype, hubs=0)
+    rows = [list(zip(idx[rp[i]:rp[i + 1]].tolist(), d[rp[i]:rp[i + 1]].tolist())) for i in range(n)]
+    rng = np.random.default_rng(12)
+    hubs = {17: 48, 4242: 300, 9001: 2000, 15555: 9000}
+    for h, count in hubs.items():
+        for i in rng.choice(n, size=count, replace=False):
+            i = int(i)
+            if i != h and all(t != h for t, _ in rows[i]):
+                rows[i].append((h, float(rng.exponential(1.0))))
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    idx = np.array([t for r in rows for t, _ in r], np.int32)
+    val = np.exp(-np.array([v for r in rows for _, v in r])).astype(dtype)
+    # mirrors of row t: entries i -> t without t -> i
+    pairs = set(zip(np.repeat(np.arange(n), np.diff(rp)).tolist(), idx.tolist()))
+    mirrors = np.zeros(n, np.int64)
+    for i, t in pairs:
+        if (t, i) not in pairs:
+            mirrors[t] += 1
+    for lo, hi in [(1, 32), (33, 64), (65, 512), (513, 8192), (8193, n)]:
+        assert np.any((mirrors >= lo) & (mirrors <= hi)), (lo, hi)
+    for mix in (1.0, 0.3):
+        g = _gpu_combine(umx, rp, idx, val, mix)
+        e = oracle.combine_neighbor_sets(rp, idx, val, mix)
+        for a, b in zip(g, e):
+            assert np.array_equal(a, b)
+
+
 def test_combine_c1_after_similarities(umx):
     _, (rp, idx, dist) = c1_knn()
     ew, _, _ = oracle.neighbor_similarities(rp, dist)

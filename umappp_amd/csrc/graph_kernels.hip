@@ -541,25 +541,29 @@ __global__ void __launch_bounds__(kBlock) k_stage_mirrors(int64_t n, int G, cons
 // Rows with at most kSmallMirrors entries: a group of kSmallMirrors lanes per
 // row ranks its keys with shuffles.  Longer rows are listed by
 // k_list_long_rows for k_sort_mirrors_mid (up to 64 entries) or
-// k_sort_mirrors_big (more).
+// k_sort_mirrors_big (more; an LDS capacity of kBigSortCapSmall entries for
+// rows up to that length, kBigSortCap above: the large variant's 64 KB of
+// LDS fits two workgroups per CU, the small one's 4 KB eight).
 constexpr int kSmallMirrors = 32;
+constexpr int kBigSortCapSmall = 512;
 constexpr int kBigSortCap = 8192;  // LDS bitonic sort capacity per row
 constexpr int kListRowsPerThread = 4;
+constexpr int kLongLists = 3;       // mid, big (small LDS), big (large LDS)
 
-// Appends the rows with more than kSmallMirrors staged entries to the mid
-// (at most 64) or big list, in no particular order.  A workgroup takes
+// Appends the rows with more than kSmallMirrors staged entries to list 0
+// (at most 64 entries), 1 (at most kBigSortCapSmall) or 2 (more), in no
+// particular order; counts[l] is list l's length.  A workgroup takes
 // kBlock * kListRowsPerThread consecutive rows, numbers its long rows with
 // LDS atomics and reserves their list slots with one global atomic per list
-// (one global atomic per long row queued on the two counters: about 1 ms for
+// (one global atomic per long row queued on the counters: about 1 ms for
 // the hub rows of a 1M-point kNN graph).
 __global__ void __launch_bounds__(kBlock) k_list_long_rows(int64_t n, const int64_t* __restrict__ inc_rp,
-                                                         int64_t* __restrict__ mid_rows,
-                                                         unsigned long long* __restrict__ n_mid,
-                                                         int64_t* __restrict__ big_rows,
-                                                         unsigned long long* __restrict__ n_big) {
-    __shared__ unsigned int cnt[2];
-    __shared__ unsigned long long base[2];
-    if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
+                                                         int64_t* __restrict__ list0, int64_t* __restrict__ list1,
+                                                         int64_t* __restrict__ list2,
+                                                         unsigned long long* __restrict__ counts) {
+    __shared__ unsigned int cnt[kLongLists];
+    __shared__ unsigned long long base[kLongLists];
+    if (threadIdx.x < kLongLists) cnt[threadIdx.x] = 0;
     __syncthreads();
     const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kBlock * kListRowsPerThread + threadIdx.x;
     int cls[kListRowsPerThread];
@@ -568,20 +572,21 @@ __global__ void __launch_bounds__(kBlock) k_list_long_rows(int64_t n, const int6
     for (int j = 0; j < kListRowsPerThread; ++j) {
         const int64_t row = r0 + static_cast<int64_t>(j) * kBlock;
         const int64_t m = row < n ? inc_rp[row + 1] - inc_rp[row] : 0;
-        cls[j] = m <= kSmallMirrors ? -1 : (m <= kWave ? 0 : 1);
+        cls[j] = m <= kSmallMirrors ? -1 : (m <= kWave ? 0 : (m <= kBigSortCapSmall ? 1 : 2));
         slot[j] = cls[j] >= 0 ? atomicAdd(&cnt[cls[j]], 1u) : 0u;
     }
     __syncthreads();
-    if (threadIdx.x < 2) {
+    if (threadIdx.x < kLongLists) {
         const unsigned int c = cnt[threadIdx.x];
-        base[threadIdx.x] = c ? atomicAdd(threadIdx.x == 0 ? n_mid : n_big, static_cast<unsigned long long>(c)) : 0ULL;
+        base[threadIdx.x] = c ? atomicAdd(counts + threadIdx.x, static_cast<unsigned long long>(c)) : 0ULL;
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kListRowsPerThread; ++j) {
         if (cls[j] >= 0) {
             const int64_t row = r0 + static_cast<int64_t>(j) * kBlock;
-            (cls[j] == 0 ? mid_rows : big_rows)[base[cls[j]] + slot[j]] = row;
+            int64_t* list = cls[j] == 0 ? list0 : (cls[j] == 1 ? list1 : list2);
+            list[base[cls[j]] + slot[j]] = row;
         }
     }
 }
@@ -643,17 +648,17 @@ __global__ void __launch_bounds__(kBlock) k_sort_mirrors_mid(const int64_t* __re
 // One workgroup per long row (more than 64 entries): bitonic sort of
 // (key, value) in LDS.  Rows beyond kBigSortCap entries are ranked in global
 // memory instead (quadratic, for extreme hubs only).
-template <typename T>
+template <typename T, int CAP>
 __global__ void __launch_bounds__(kBlock) k_sort_mirrors_big(const int64_t* __restrict__ big_rows,
                                                            const int64_t* __restrict__ inc_rp,
                                                            int32_t* __restrict__ src, T* __restrict__ val,
                                                            int32_t* __restrict__ tmp_src, T* __restrict__ tmp_val) {
-    __shared__ int32_t keys[kBigSortCap];
-    __shared__ int32_t pos[kBigSortCap];
+    __shared__ int32_t keys[CAP];
+    __shared__ int32_t pos[CAP];
     const int64_t row = big_rows[blockIdx.x];
     const int64_t ss = inc_rp[row];
     const int64_t m = inc_rp[row + 1] - ss;
-    if (m <= kBigSortCap) {
+    if (m <= CAP) {
         int p2 = 1;
         while (p2 < m) p2 <<= 1;
         for (int q = threadIdx.x; q < p2; q += kBlock) {
@@ -1057,30 +1062,38 @@ void run_combine(int64_t n, const int64_t* rp, const int32_t* idx, const T* val,
     if (n_inc > 0) {
         Tmp<int64_t> mid_rows(static_cast<size_t>(n), s);
         Tmp<int64_t> big_rows(static_cast<size_t>(n), s);
-        Tmp<unsigned long long> counts(2, s);  // mid, big
-        UMX_HIP(hipMemsetAsync(counts.p, 0, 2 * sizeof(unsigned long long), s));
+        Tmp<int64_t> huge_rows(static_cast<size_t>(n), s);
+        Tmp<unsigned long long> counts(kLongLists, s);  // mid, big, huge
+        UMX_HIP(hipMemsetAsync(counts.p, 0, kLongLists * sizeof(unsigned long long), s));
         const int64_t list_rows = static_cast<int64_t>(kBlock) * kListRowsPerThread;
         k_list_long_rows<<<static_cast<unsigned>((n + list_rows - 1) / list_rows), kBlock, 0, s>>>(
-            n, inc_rp.p, mid_rows.p, counts.p, big_rows.p, counts.p + 1);
+            n, inc_rp.p, mid_rows.p, big_rows.p, huge_rows.p, counts.p);
         UMX_HIP(hipGetLastError());
         k_sort_mirrors_small<T><<<grid_for(n * kSmallMirrors), kBlock, 0, s>>>(n, inc_rp.p, stage_src.p,
                                                                                stage_val.p);
         UMX_HIP(hipGetLastError());
-        unsigned long long hc[2] = {0, 0};
+        unsigned long long hc[kLongLists] = {0, 0, 0};
         UMX_HIP(hipMemcpyAsync(hc, counts.p, sizeof(hc), hipMemcpyDeviceToHost, s));
         UMX_HIP(hipStreamSynchronize(s));
-        const unsigned long long hmid = hc[0], hbig = hc[1];
+        const unsigned long long hmid = hc[0], hbig = hc[1], hhuge = hc[2];
         if (hmid > 0) {
             const unsigned mid_grid = static_cast<unsigned>((hmid + kBlock / kWave - 1) / (kBlock / kWave));
             k_sort_mirrors_mid<T><<<mid_grid, kBlock, 0, s>>>(mid_rows.p, hmid, inc_rp.p, stage_src.p, stage_val.p);
             UMX_HIP(hipGetLastError());
         }
-        if (hbig > 0) {
+        if (hbig + hhuge > 0) {
             Tmp<int32_t> tsrc(static_cast<size_t>(n_inc), s);
             Tmp<T> tval(static_cast<size_t>(n_inc), s);
-            k_sort_mirrors_big<T><<<static_cast<unsigned>(hbig), kBlock, 0, s>>>(big_rows.p, inc_rp.p, stage_src.p,
-                                                                                stage_val.p, tsrc.p, tval.p);
-            UMX_HIP(hipGetLastError());
+            if (hbig > 0) {
+                k_sort_mirrors_big<T, kBigSortCapSmall><<<static_cast<unsigned>(hbig), kBlock, 0, s>>>(
+                    big_rows.p, inc_rp.p, stage_src.p, stage_val.p, tsrc.p, tval.p);
+                UMX_HIP(hipGetLastError());
+            }
+            if (hhuge > 0) {
+                k_sort_mirrors_big<T, kBigSortCap><<<static_cast<unsigned>(hhuge), kBlock, 0, s>>>(
+                    huge_rows.p, inc_rp.p, stage_src.p, stage_val.p, tsrc.p, tval.p);
+                UMX_HIP(hipGetLastError());
+            }
         }
     }
     out_idx.alloc(static_cast<size_t>(nnz_out));
