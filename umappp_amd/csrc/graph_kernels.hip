@@ -458,7 +458,8 @@ __global__ void __launch_bounds__(kBlock) k_reciprocal(int64_t n, int G, const i
                                                      const int32_t* __restrict__ sidx, const T* __restrict__ sval,
                                                      T mix, int mode, T* __restrict__ nval,
                                                      unsigned char* __restrict__ flags,
-                                                     int64_t* __restrict__ keep_cnt, int64_t* __restrict__ inc_cnt) {
+                                                     int64_t* __restrict__ keep_cnt, int64_t* __restrict__ inc_cnt,
+                                                     int32_t* __restrict__ mslot) {
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     const int64_t row = gid / G;
     const int lane = static_cast<int>(gid & (G - 1));
@@ -500,7 +501,9 @@ __global__ void __launch_bounds__(kBlock) k_reciprocal(int64_t n, int G, const i
         nval[q] = v;
         flags[q] = static_cast<unsigned char>(keep | (mirror << 1));
         kept += keep;
-        if (mirror) atomicAdd(reinterpret_cast<unsigned long long*>(inc_cnt + t), 1ULL);
+        // the count's old value numbers the mirrored entry within row t
+        if (mirror)
+            mslot[q] = static_cast<int32_t>(atomicAdd(reinterpret_cast<unsigned long long*>(inc_cnt + t), 1ULL));
     }
     // the row's G lanes (aligned, all active) sum their counts; one store
     for (int o = G / 2; o >= 1; o >>= 1) kept += __shfl_xor(kept, o, kWave);
@@ -513,12 +516,14 @@ __global__ void __launch_bounds__(kBlock) k_add_counts(const int64_t* __restrict
     if (i < n) out[i] = a[i] + b[i];
 }
 
+// Writes each mirrored entry i -> t as (i, value) into row t's staging
+// range, at the place k_reciprocal numbered it (no atomics here).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_stage_mirrors(int64_t n, int G, const int64_t* __restrict__ rp,
                                                         const int32_t* __restrict__ sidx, const T* __restrict__ nval,
                                                         const unsigned char* __restrict__ flags,
                                                         const int64_t* __restrict__ inc_rp,
-                                                        unsigned long long* __restrict__ fill,
+                                                        const int32_t* __restrict__ mslot,
                                                         int32_t* __restrict__ stage_src, T* __restrict__ stage_val) {
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     const int64_t row = gid / G;
@@ -528,8 +533,7 @@ __global__ void __launch_bounds__(kBlock) k_stage_mirrors(int64_t n, int G, cons
     for (int64_t q = s + lane; q < e; q += G) {
         if (flags[q] & 2) {
             const int32_t t = sidx[q];
-            const unsigned long long slot = atomicAdd(fill + t, 1ULL);
-            const int64_t at = inc_rp[t] + static_cast<int64_t>(slot);
+            const int64_t at = inc_rp[t] + mslot[q];
             stage_src[at] = static_cast<int32_t>(row);
             stage_val[at] = nval[q];
         }
@@ -1033,7 +1037,9 @@ void run_combine(int64_t n, const int64_t* rp, const int32_t* idx, const T* val,
     int64_t* keep_cnt = cnt.p;
     int64_t* inc_cnt = cnt.p + n;
     UMX_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int64_t) * static_cast<size_t>(2 * n), s));
-    k_reciprocal<T><<<grid, kBlock, 0, s>>>(n, G, rp, sidx.p, sval.p, mix, mode, nval.p, flags.p, keep_cnt, inc_cnt);
+    Tmp<int32_t> mslot(static_cast<size_t>(nnz_in), s);  // place of each mirrored entry in its target row
+    k_reciprocal<T><<<grid, kBlock, 0, s>>>(n, G, rp, sidx.p, sval.p, mix, mode, nval.p, flags.p, keep_cnt, inc_cnt,
+                                            mslot.p);
     UMX_HIP(hipGetLastError());
 
     DevBuf<int64_t> unused;
@@ -1053,10 +1059,8 @@ void run_combine(int64_t n, const int64_t* rp, const int32_t* idx, const T* val,
     Tmp<int32_t> stage_src(static_cast<size_t>(n_inc), s);
     Tmp<T> stage_val(static_cast<size_t>(n_inc), s);
     if (n_inc > 0) {
-        UMX_HIP(hipMemsetAsync(keep_cnt, 0, sizeof(int64_t) * static_cast<size_t>(n), s));  // reused as fill cursors
-        k_stage_mirrors<T><<<grid, kBlock, 0, s>>>(n, G, rp, sidx.p, nval.p, flags.p, inc_rp.p,
-                                                   reinterpret_cast<unsigned long long*>(keep_cnt), stage_src.p,
-                                                   stage_val.p);
+        k_stage_mirrors<T><<<grid, kBlock, 0, s>>>(n, G, rp, sidx.p, nval.p, flags.p, inc_rp.p, mslot.p,
+                                                   stage_src.p, stage_val.p);
         UMX_HIP(hipGetLastError());
     }
     if (n_inc > 0) {

@@ -512,20 +512,21 @@ __device__ inline int64_t split_row(const int64_t* __restrict__ rp, const T* __r
 #pragma unroll
         for (int u = 0; u < kSplitBatch; ++u) {
             const int64_t j = j0 + u;
-            if (j >= r1) break;
-            const double w = wb[u];
-            const bool cut = j > start && ((q < c && acc + w > q * f / c) || j - start >= edges_cap);
-            if (cut) {
-                if (WRITE) {
-                    ibeg[m + items] = start;
-                    ilen[m + items] = static_cast<int32_t>(j - start);
-                    irow[m + items] = tag;
+            if (j < r1) {  // (a guard, not a break: the loop unrolls fully)
+                const double w = wb[u];
+                const bool cut = j > start && ((q < c && acc + w > q * f / c) || j - start >= edges_cap);
+                if (cut) {
+                    if (WRITE) {
+                        ibeg[m + items] = start;
+                        ilen[m + items] = static_cast<int32_t>(j - start);
+                        irow[m + items] = tag;
+                    }
+                    ++items;
+                    if (acc + w > q * f / c) q += 1;
+                    start = j;
                 }
-                ++items;
-                if (acc + w > q * f / c) q += 1;
-                start = j;
+                acc += w;
             }
-            acc += w;
         }
     }
     if (WRITE) {
