@@ -28,9 +28,10 @@
 // and count, seed, epochs, learning rates), draws negative samples inside its
 // instance, and keys the sampler by the instance's seed and its own entry
 // numbers, so an instance sees the samples it would see when run alone.  The
-// epoch number can live in device memory, advanced by a one-wavefront kernel
-// after each launch (the one that also folds the partial tallies), so a block
-// of launches captured once as a HIP graph replays for any epochs.
+// epoch number can live in device memory (launch q of a block runs epoch
+// counter + q), advanced by a one-wavefront kernel after the block's
+// launches (the one that also folds their partial tallies), so a block of
+// launches captured once as a HIP graph replays for any epochs.
 #include "layout_device.hpp"
 #include "umx_internal.hpp"
 
