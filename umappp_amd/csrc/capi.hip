@@ -1069,11 +1069,12 @@ int effective_tile(const umx_layout* L) {
     return t;
 }
 
-// One epoch of the sliced-ELL kernel: 'epoch' >= 0, or -1 to take it from
-// the device counter and advance that (captured blocks).  Optional events
+// One epoch of the sliced-ELL kernel: 'epoch' >= 0, or -1 - q for launch q
+// of a captured block (the device counter + q; the block's fold advances
+// the counter, as does this call's own fold for epoch -1).  Optional events
 // bracket the epoch kernel alone; the tally fold follows them unless 'fold'
-// is false (the caller then folds the epoch with fold_layout_epochs before
-// its ring entry is reused).
+// is false (the caller then folds the epoch with fold_layout_epochs or
+// fold_layout_block before its ring entry is reused).
 void launch_sell_once(umx_layout* L, int epoch, hipEvent_t start = nullptr, hipEvent_t stop = nullptr,
                       bool fold = true) {
     if (start) UMX_HIP(hipEventRecord(start, L->s));

@@ -540,8 +540,9 @@ __global__ void __launch_bounds__(kBlock) k_stage_mirrors(int64_t n, int G, cons
     }
 }
 
-// Sorts each row's staged mirrored entries by index (the staging atomics leave
-// them in arbitrary order; keys are unique within a row, one per source row).
+// Sorts each row's staged mirrored entries by index (the numbering atomics of
+// k_reciprocal leave them in arbitrary order; keys are unique within a row,
+// one per source row).
 // Rows with at most kSmallMirrors entries: a group of kSmallMirrors lanes per
 // row ranks its keys with shuffles.  Longer rows are listed by
 // k_list_long_rows for k_sort_mirrors_mid (up to 64 entries) or
