@@ -15,12 +15,16 @@
 // so no FMA contraction can change a rounding).
 //
 // Parity status per function (details in the tests):
-//   orc_neighbor_similarities   pinned (reference invariants + closed-form root check)
+//   orc_neighbor_similarities   pinned (reference invariants + closed-form root check;
+//                               rows whose exact sigma follows from the reference's
+//                               steps; bit-equal to an independent Python restatement
+//                               of :89-202 on rows taking every branch of the solve)
 //   orc_combine_neighbor_sets   pinned (brute-force symmetrisation, reference test :47-112)
 //   orc_similarities_to_epochs  pinned (closed form eps = max/w, reference Epochs test)
 //   orc_layout_* (serial)       pinned (per-edge update vs finite differences of the
-//                               UMAP losses; schedule vs closed-form firing counts;
-//                               restart equality)
+//                               UMAP losses; schedule vs closed-form firing counts and
+//                               bit-equal to a numpy float32 replay of :131-179 at
+//                               non-dyadic negative-sample rates; restart equality)
 //   orc_layout_* (parallel)     pinned (bit-equal to serial, reference test :106-125)
 //   discrete_uniform (rng kind 0) parity unpinned: aarand::discrete_uniform is not in
 //                               the image; this is a documented rejection sampler.
