@@ -65,6 +65,8 @@ def lib():
             "orc_step_delta": (C.c_int, [C.c_int, C.c_int, C.c_int, _vp, _vp, C.c_double, C.c_double, C.c_double,
                                          C.c_double, _vp]),
             "orc_epoch_alpha": (C.c_double, [C.c_int, C.c_double, C.c_int, C.c_int]),
+            "orc_step_delta_many": (C.c_int, [C.c_int, C.c_int64, _vp, C.c_int, _vp, _vp, C.c_double, C.c_double,
+                                              C.c_double, C.c_double, _vp]),
             "orc_counter_draw": (C.c_uint32, [C.c_uint64, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32]),
             "orc_mt19937_64_nth": (C.c_uint64, [C.c_uint64, C.c_int64]),
             "orc_discrete_uniform_stream": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, _vp]),
@@ -223,6 +225,19 @@ def step_delta(kind, head, other, a, b, gamma, alpha, dtype):
     out = np.empty(h.size, dtype=dt)
     lib().orc_step_delta(_dt(dt), int(kind), h.size, _p(h), _p(o), float(a), float(b), float(gamma), float(alpha),
                          _p(out))
+    return out
+
+
+def step_delta_many(kinds, heads, others, a, b, gamma, alpha, dtype):
+    """step_delta for many steps at once: kinds (m,), heads and others (m, D)."""
+    dt = np.dtype(dtype)
+    k = np.ascontiguousarray(kinds, dtype=np.int32)
+    h = np.ascontiguousarray(heads, dtype=dt)
+    o = np.ascontiguousarray(others, dtype=dt)
+    out = np.empty_like(h)
+    D = h.shape[1] if h.ndim == 2 else 1
+    lib().orc_step_delta_many(_dt(dt), k.size, _p(k), D, _p(h), _p(o), float(a), float(b), float(gamma),
+                              float(alpha), _p(out))
     return out
 
 

@@ -954,6 +954,22 @@ int orc_step_delta(int dtype, int kind, int D, const void* head, const void* oth
                            static_cast<double*>(delta));
     return 0;
 }
+// step_delta over many steps: kinds[q], head row q and other row q of the
+// point-major arrays (D values each) into row q of delta.
+int orc_step_delta_many(int dtype, int64_t count, const int32_t* kinds, int D, const void* head, const void* other,
+                        double a, double b, double gamma, double alpha, void* delta) {
+    for (int64_t q = 0; q < count; ++q) {
+        if (dtype == 0)
+            step_delta<float>(kinds[q], D, static_cast<const float*>(head) + q * D,
+                              static_cast<const float*>(other) + q * D, static_cast<float>(a), static_cast<float>(b),
+                              static_cast<float>(gamma), static_cast<float>(alpha), static_cast<float*>(delta) + q * D);
+        else
+            step_delta<double>(kinds[q], D, static_cast<const double*>(head) + q * D,
+                               static_cast<const double*>(other) + q * D, a, b, gamma, alpha,
+                               static_cast<double*>(delta) + q * D);
+    }
+    return 0;
+}
 double orc_epoch_alpha(int dtype, double alpha0, int n, int total) {
     return dtype == 0 ? static_cast<double>(epoch_alpha<float>(static_cast<float>(alpha0), n, total))
                       : epoch_alpha<double>(alpha0, n, total);

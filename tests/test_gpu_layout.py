@@ -1,18 +1,18 @@
 """GPU parity of the layout epochs against the oracle, through the C ABI.
 
-* Schedule: after any number of epochs the Hogwild kernel's
+* Schedule: after any number of epochs the production kernel's
   epoch_of_next_sample / epoch_of_next_negative_sample equal the oracle's bit
   for bit, and its per-epoch tallies (firing edges, draws, self-hits) equal
   the oracle's exactly (same counter sampler).
 * Per-edge updates: both kernels record every step (head, other, coordinates
   before, move).  The step sequence (serial) or per-epoch step multiset
-  (Hogwild) equals the oracle's exactly, and every step equals the
+  (production) equals the oracle's exactly, and every step equals the
   reference's update computed from the same inputs: 1e-5 relative in f32,
   1e-12 in f64.  Whole-embedding comparisons over several epochs are not
   meaningful: the SGD is chaotic (a 1-ulp difference grows to O(1) within
   four epochs on the CPU alone).
-* Final embedding: the Hogwild result and the oracle's serial result at the
-  same seed and initial embedding agree on 15-NN preservation and
+* Final embedding: the production result and the oracle's serial result at
+  the same seed and initial embedding agree on 15-NN preservation and
   trustworthiness within 0.01.
 """
 import numpy as np
@@ -40,15 +40,24 @@ def _pair(umx, rp, idx, w, T, nsr, seed, dtype):
     return lay, orc
 
 
-@pytest.mark.parametrize("nobs,k", [(50, 5), (100, 10), (200, 15)])
-@pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("dim", [2, 3, 5])
-def test_schedule_and_tallies_bit_exact(umx, nobs, k, dtype, dim):
+# sliced-ELL launch shapes (tile columns, wavefronts per workgroup) within
+# the 64 KiB LDS budget of a float workgroup
+_SHAPES = [(t, w) for t in (4, 8, 16, 32) for w in (1, 2, 4) if t * w <= 64]
+
+_SCHED = ([(nobs, k, dt, dim, 0, 1) for nobs, k in ((50, 5), (100, 10), (200, 15))
+           for dt in (np.float32, np.float64) for dim in (2, 3, 5)]
+          + [(200, 15, np.float32, 2, t, w) for t, w in _SHAPES])
+
+
+@pytest.mark.parametrize("nobs,k,dtype,dim,tile,waves", _SCHED)
+def test_schedule_and_tallies_bit_exact(umx, nobs, k, dtype, dim, tile, waves):
     x, rp, idx, w = _graph(nobs, k, dtype)
     T, nsr, seed = 120, 5.0, 77
     lay, orc = _pair(umx, rp, idx, w, T, nsr, seed, dtype)
     emb0 = np.ascontiguousarray(x[:, :dim], dtype=dtype) if dim <= 5 else None
     lay.set_params(dim, A, B, 1.0, 1.0, seed)
+    lay.set_tile(tile)
+    lay.set_placement(waves, 1)
     lay.set_embedding(emb0)
     lay.run(37)
     lay.run(T)
@@ -105,19 +114,33 @@ def _check_steps(trace, counts, dim, dtype, a, b, gamma, T, first_epoch, rtol, a
     worst = 0.0
     pos = 0
     for e, m in enumerate(counts):
+        m = int(m)
         alpha = oracle.epoch_alpha(1.0, first_epoch + e, T, dtype)
-        for r in trace[pos:pos + m]:
-            kind = int(r[0])
-            h = r[3:3 + dim].astype(dtype)
-            o = r[3 + dim:3 + 2 * dim].astype(dtype)
-            ref = oracle.step_delta(kind, h, o, a, b, gamma, alpha, dtype).astype(np.float64)
-            got = r[3 + 2 * dim:3 + 3 * dim]
-            err = np.abs(got - ref)
-            assert np.all(err <= rtol * np.abs(ref) + atol), (e, r[:3], got, ref)
+        r = trace[pos:pos + m]
+        h = r[:, 3:3 + dim].astype(dtype)
+        o = r[:, 3 + dim:3 + 2 * dim].astype(dtype)
+        ref = oracle.step_delta_many(r[:, 0].astype(np.int32), h, o, a, b, gamma, alpha, dtype).astype(np.float64)
+        got = r[:, 3 + 2 * dim:3 + 3 * dim]
+        err = np.abs(got - ref)
+        bad = np.any(err > rtol * np.abs(ref) + atol, axis=1)
+        assert not bad.any(), (e, r[bad][:3, :3], got[bad][:3], ref[bad][:3])
+        if m:
             worst = max(worst, float(np.max(err / np.maximum(np.abs(ref), 1e-30))))
         pos += m
     assert pos == trace.shape[0]
     return worst
+
+
+def _check_reverse_steps(trace, dim, dtype, a, b, gamma, alpha, rtol, atol):
+    """Kind-2 records (pull mode): head i's move is the tail move of edge
+    t->i, minus the head move of t evaluated at (y_t, y_i); every record is
+    checked."""
+    h = trace[:, 3:3 + dim].astype(dtype)
+    o = trace[:, 3 + dim:3 + 2 * dim].astype(dtype)
+    ref = -oracle.step_delta_many(np.zeros(len(trace), np.int32), o, h, a, b, gamma, alpha, dtype).astype(np.float64)
+    got = trace[:, 3 + 2 * dim:]
+    bad = np.any(np.abs(got - ref) > rtol * np.abs(ref) + atol, axis=1)
+    assert not bad.any(), (trace[bad][:3, :3], got[bad][:3], ref[bad][:3])
 
 
 @pytest.mark.parametrize("dtype,rtol,atol", [(np.float64, 1e-12, 1e-15), (np.float32, 1e-5, 1e-7)])
@@ -196,16 +219,24 @@ def test_serial_replay_injected_mt_samples(umx, dtype, rtol, atol):
         lay2.run_serial(2, samples=samples[:10])
 
 
-@pytest.mark.parametrize("mode", ["pull", "push"])
-@pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("dim", [2, 3])
-def test_hogwild_per_step(umx, dtype, dim, mode):
-    """Production kernels on the config C1 graph.  Per epoch the steps are
-    exactly the oracle's (as a multiset), each traced step equals the
-    reference's update from its traced inputs within 1e-5 relative, and no
-    step is lost: the embedding moves by exactly the traced steps' sum.
-    Push mode records kind 0 (head and tail move) and kind 1; pull mode also
-    records kind 2, the tail move that the reverse edge gives each head."""
+_PER_STEP = ([("pull", np.float32, dim, t, w, 32.0) for dim in (2, 3) for t, w in _SHAPES]
+             + [("pull", np.float32, 2, t, 1, 2.0) for t in (8, 32)]
+             + [("pull", np.float64, dim, 0, 1, 32.0) for dim in (2, 3)]
+             + [("push", dt, dim, 0, 1, 32.0) for dt in (np.float32, np.float64) for dim in (2, 3)])
+
+
+@pytest.mark.parametrize("mode,dtype,dim,tile,waves,split", _PER_STEP)
+def test_production_per_step(umx, mode, dtype, dim, tile, waves, split):
+    """Production kernels on the config C1 graph, every tile width and
+    wavefronts-per-workgroup of the sliced-ELL kernel (pull mode), and a work
+    split that cuts most rows into pieces.  Per epoch the steps are exactly
+    the oracle's (as a multiset), every traced step equals the reference's
+    update from its traced inputs within 1e-5 relative, and no step is lost:
+    the embedding moves by exactly the traced steps' sum.  Push mode records
+    kind 0 (head and tail move) and kind 1; pull mode also records kind 2, the
+    tail move that the reverse edge gives each head.  Pull mode is
+    double-buffered: every other point is read as it was at the epoch's start,
+    and every head starts from its position at the epoch's start."""
     _, (rp, idx, dist) = c1_knn()
     wd, _, _ = oracle.neighbor_similarities(rp, dist)
     crp, cidx, cw = oracle.combine_neighbor_sets(rp, idx, wd, 1.0)
@@ -215,13 +246,17 @@ def test_hogwild_per_step(umx, dtype, dim, mode):
     emb = np.random.default_rng(1).uniform(-10, 10, size=(2000, dim)).astype(dtype)
     lay.set_params(dim, A, B, 1.0, 1.0, seed)
     lay.set_mode(umx.Layout.PULL if mode == "pull" else umx.Layout.PUSH)
+    if mode == "pull":
+        lay.set_tile(tile)
+        lay.set_placement(waves, 1)
+        lay.set_work_split(split)
     lay.set_embedding(emb)
     lay.set_trace(600_000)
     orc.trace(600_000)
     eo = emb.copy()
     key = lambda a: a[np.lexsort(tuple(a[:, c] for c in range(a.shape[1] - 1, -1, -1)))]
     for limit in range(1, 7):
-        before = lay.get_embedding().astype(np.float64)
+        before = lay.get_embedding()
         lay.run(limit)
         tr = lay.get_trace()
         orc.run(eo, A, B, epoch_limit=limit)
@@ -231,22 +266,20 @@ def test_hogwild_per_step(umx, dtype, dim, mode):
         trip = tr[:, :3].astype(np.int64)
         assert np.array_equal(key(trip[kinds != 2]), key(seq))
         alpha = oracle.epoch_alpha(1.0, limit - 1, T, dtype)
+        heads = tr[:, 1].astype(np.int64)
+        others = tr[:, 2].astype(np.int64)
         if mode == "pull":
             att = seq[seq[:, 0] == 0]
             rev = trip[kinds == 2][:, [1, 2]]
             assert np.array_equal(key(rev), key(att[:, [2, 1]]))
-            # kind 2: i's move is the tail move of edge t->i, i.e. minus the
-            # head move of t evaluated at (y_t, y_i)
-            for r in tr[kinds == 2][::5]:
-                h = r[3:3 + dim].astype(dtype)
-                o = r[3 + dim:3 + 2 * dim].astype(dtype)
-                ref = -oracle.step_delta(0, o, h, A, B, 1.0, alpha, dtype).astype(np.float64)
-                got = r[3 + 2 * dim:]
-                assert np.all(np.abs(got - ref) <= 1e-5 * np.abs(ref) + 1e-6)
+            _check_reverse_steps(tr[kinds == 2], dim, dtype, A, B, 1.0, alpha, 1e-5, 1e-6)
+            # double buffer: others as they were at the start of the epoch,
+            # and each head's first traced step from its starting position
+            assert np.array_equal(tr[:, 3 + dim:3 + 2 * dim].astype(dtype), before[others])
+            _, first = np.unique(heads, return_index=True)
+            assert np.array_equal(tr[first, 3:3 + dim].astype(dtype), before[heads[first]])
         _check_steps(tr[kinds != 2], [int((kinds != 2).sum())], dim, dtype, A, B, 1.0, T, limit - 1, 1e-5, 1e-6)
-        moved = np.zeros_like(before)
-        heads = tr[:, 1].astype(np.int64)
-        others = tr[:, 2].astype(np.int64)
+        moved = np.zeros(before.shape, np.float64)
         d = tr[:, 3 + 2 * dim:]
         np.add.at(moved, heads, d)
         if mode == "push":
@@ -254,7 +287,7 @@ def test_hogwild_per_step(umx, dtype, dim, mode):
             np.add.at(moved, others[att], -d[att])
         after = lay.get_embedding().astype(np.float64)
         tol = (1e-4 if dtype == np.float32 else 1e-10) * max(1.0, np.max(np.abs(before)))
-        assert np.max(np.abs(after - (before + moved))) <= tol
+        assert np.max(np.abs(after - (before.astype(np.float64) + moved))) <= tol
 
 
 _MIX = {}

@@ -663,8 +663,13 @@ struct umx_layout {
     hipStream_t s = nullptr;
     DevBuf<int64_t> rp;
     DevBuf<int32_t> tg;
-    DevBuf<float> ef, nf, gf, mf;  // eps, next, next_neg, embedding (float)
-    DevBuf<double> ed, nd, gd, md;  // same (double)
+    // eps, next, next_neg and the two embedding buffers (float, double).
+    // The embedding at epoch c is in buffer c & 1 (mf / md for even c,
+    // mf2 / md2 for odd c): a double-buffered epoch e reads buffer e & 1 and
+    // writes the other; kernels that update in place (serial replay, push
+    // mode) leave both buffers equal after their run.
+    DevBuf<float> ef, nf, gf, mf, mf2;
+    DevBuf<double> ed, nd, gd, md, md2;
     DevBuf<unsigned long long> tally;
     DevBuf<unsigned long long> tally_part;  // ring of per-epoch partial tallies (zero between runs)
     DevBuf<unsigned long long> cursor;  // serial replay cursor + overrun flag
@@ -673,7 +678,12 @@ struct umx_layout {
     DevBuf<int64_t> item_begin;  // work items of the production kernels
     DevBuf<int32_t> item_len;
     DevBuf<int32_t> item_row;
+    DevBuf<int32_t> item_aux;    // padding / whole row / split piece slot
+    umx::SplitPlan split;        // split rows and their piece slots
+    DevBuf<float> split_df;      // moves of the pieces of split rows (float, double)
+    DevBuf<double> split_dd;
     int64_t n_items = -1;        // -1: not built yet
+    int split_dim = -1;          // num_dim split_delta was sized for (-1: not sized)
     // Only heavy rows are split: a row's steps should see each other
     // sequentially (items of a split row start from one snapshot of the head
     // and their summed moves overshoot), but one very heavy row would
@@ -731,8 +741,14 @@ struct umx_layout {
         q.of_row = n_inst() > 1 ? of_row.p : nullptr;
         return q;
     }
-    void* emb() { return dtype == UMX_F32 ? static_cast<void*>(mf.p) : static_cast<void*>(md.p); }
-    const void* emb() const { return dtype == UMX_F32 ? static_cast<const void*>(mf.p) : static_cast<const void*>(md.p); }
+    // embedding buffer 0 or 1
+    void* emb_buf(int which) const {
+        if (dtype == UMX_F32) return which ? static_cast<void*>(mf2.p) : static_cast<void*>(mf.p);
+        return which ? static_cast<void*>(md2.p) : static_cast<void*>(md.p);
+    }
+    // the buffer holding the embedding at the current epoch
+    void* emb() const { return emb_buf(current_epoch & 1); }
+    size_t emb_bytes() const { return dsize(dtype) * static_cast<size_t>(n) * static_cast<size_t>(dim); }
 };
 
 namespace {
@@ -749,7 +765,22 @@ DevBuf<double>& alpha_buf<double>(umx_layout* L) {
 }
 
 template <typename T>
-umx::LayoutView<T> view_of(umx_layout* L, DevBuf<T>& eps, DevBuf<T>& nxt, DevBuf<T>& nng, DevBuf<T>& emb) {
+DevBuf<T>& split_delta_buf(umx_layout* L);
+template <>
+DevBuf<float>& split_delta_buf<float>(umx_layout* L) {
+    return L->split_df;
+}
+template <>
+DevBuf<double>& split_delta_buf<double>(umx_layout* L) {
+    return L->split_dd;
+}
+
+// Kernel view.  'emb' and 'emb_alt' are the two embedding buffers for the
+// double-buffered pull kernels (buffer 0 and buffer 1); kernels that update
+// in place get the current buffer as 'emb' (see in_place_view).
+template <typename T>
+umx::LayoutView<T> view_of(umx_layout* L, DevBuf<T>& eps, DevBuf<T>& nxt, DevBuf<T>& nng, DevBuf<T>& emb,
+                           DevBuf<T>& emb_alt) {
     umx::LayoutView<T> v{};  // value-initialised: padding is zero, so views compare bytewise
     v.n = L->n;
     v.nnz = L->nnz;
@@ -759,6 +790,7 @@ umx::LayoutView<T> view_of(umx_layout* L, DevBuf<T>& eps, DevBuf<T>& nxt, DevBuf
     v.nxt = nxt.p;
     v.nxt_neg = nng.p;
     v.emb = emb.p;
+    v.emb_alt = emb_alt.p;
     v.dim = L->dim;
     v.mode = L->mode;
     v.a = static_cast<T>(L->a);
@@ -778,6 +810,12 @@ umx::LayoutView<T> view_of(umx_layout* L, DevBuf<T>& eps, DevBuf<T>& nxt, DevBuf
     v.item_len = L->item_len.p;
     v.item_row = L->item_row.p;
     v.n_items = L->n_items;
+    v.item_aux = L->item_aux.p;
+    v.split_delta = split_delta_buf<T>(L).p;
+    v.split_row = L->split.row.p;
+    v.split_first = L->split.first.p;
+    v.split_count = L->split.count.p;
+    v.n_split_rows = L->n_items >= 0 ? L->split.rows : 0;
     v.item_inst = L->item_inst.p;
     v.inst = L->inst_tbl.p;
     v.alpha_tbl = alpha_buf<T>(L).p;
@@ -787,6 +825,28 @@ umx::LayoutView<T> view_of(umx_layout* L, DevBuf<T>& eps, DevBuf<T>& nxt, DevBuf
     v.trace_cap = L->trace_cap;
     v.trace_len = L->trace_len.p;
     return v;
+}
+
+umx::LayoutView<float> pull_view_f(umx_layout* L) { return view_of<float>(L, L->ef, L->nf, L->gf, L->mf, L->mf2); }
+umx::LayoutView<double> pull_view_d(umx_layout* L) { return view_of<double>(L, L->ed, L->nd, L->gd, L->md, L->md2); }
+
+// View for the kernels that update the embedding in place (serial replay,
+// push mode): the buffer of the current epoch.
+template <typename T>
+umx::LayoutView<T> in_place_view(umx_layout* L, DevBuf<T>& eps, DevBuf<T>& nxt, DevBuf<T>& nng, DevBuf<T>& b0,
+                                 DevBuf<T>& b1) {
+    umx::LayoutView<T> v = view_of<T>(L, eps, nxt, nng, b0, b1);
+    v.emb = (L->current_epoch & 1) ? b1.p : b0.p;
+    v.emb_alt = nullptr;
+    return v;
+}
+
+// After epochs that updated the current buffer in place from epoch 'from':
+// both buffers get its contents, so the embedding is in buffer (epoch & 1)
+// whatever the parity of the new epoch.
+void equalize_buffers(umx_layout* L, int from) {
+    const size_t bytes = L->emb_bytes();
+    if (bytes) UMX_HIP(hipMemcpyAsync(L->emb_buf((from & 1) ^ 1), L->emb_buf(from & 1), bytes, hipMemcpyDeviceToDevice, L->s));
 }
 
 template <typename T>
@@ -859,15 +919,28 @@ void build_items_t(umx_layout* L, const T* eps) {
         pl.group_of_inst = L->group_of_inst.p;
     }
     umx::build_work_items<T>(L->n, L->rp.p, eps, L->fires_per_item, L->edges_per_item, L->instances(), pl,
-                             L->item_begin, L->item_len, L->item_row, L->item_inst, L->n_items, L->s);
+                             L->item_begin, L->item_len, L->item_row, L->item_inst, L->item_aux, L->split,
+                             L->n_items, L->s);
     L->groups = groups;
+    L->split_dim = -1;  // split_delta is sized by ensure_split_delta
+}
+
+// Room for the moves of the pieces of split rows: slots x num_dim values.
+void ensure_split_delta(umx_layout* L) {
+    if (L->split_dim == L->dim) return;
+    const size_t count = static_cast<size_t>(L->split.slots) * static_cast<size_t>(L->dim);
+    if (L->dtype == UMX_F32) L->split_df.alloc(count);
+    else L->split_dd.alloc(count);
+    L->split_dim = L->dim;
 }
 
 void ensure_items(umx_layout* L) {
-    if (L->n_items >= 0) return;
-    if (L->dtype == UMX_F32) build_items_t<float>(L, L->ef.p);
-    else build_items_t<double>(L, L->ed.p);
-    L->sell_built = false;
+    if (L->n_items < 0) {
+        if (L->dtype == UMX_F32) build_items_t<float>(L, L->ef.p);
+        else build_items_t<double>(L, L->ed.p);
+        L->sell_built = false;
+    }
+    if (L->dim > 0) ensure_split_delta(L);
 }
 
 // Drops the work items (and the sliced-ELL copy built from them) after a
@@ -1079,13 +1152,15 @@ void launch_sell_once(umx_layout* L, int epoch, hipEvent_t start = nullptr, hipE
                       bool fold = true) {
     if (start) UMX_HIP(hipEventRecord(start, L->s));
     if (L->dtype == UMX_F32) {
-        auto v = view_of<float>(L, L->ef, L->nf, L->gf, L->mf);
+        auto v = pull_view_f(L);
         attach_sell(v, L->sell_f);
         umx::launch_layout_sell<float>(v, epoch, effective_tile(L), L->waves, L->s);
+        umx::launch_fold_split<float>(v, epoch, L->s);
     } else {
-        auto v = view_of<double>(L, L->ed, L->nd, L->gd, L->md);
+        auto v = pull_view_d(L);
         attach_sell(v, L->sell_d);
         umx::launch_layout_sell<double>(v, epoch, effective_tile(L), L->waves, L->s);
+        umx::launch_fold_split<double>(v, epoch, L->s);
     }
     if (stop) UMX_HIP(hipEventRecord(stop, L->s));
     if (fold) umx::finish_layout_epoch(L->tally_part.p, L->tally.p, epoch, L->epoch_ctr.p, L->s);
@@ -1116,11 +1191,11 @@ std::vector<unsigned char> launch_key(umx_layout* L, int block) {
         k.insert(k.end(), c, c + n);
     };
     if (L->dtype == UMX_F32) {
-        auto v = view_of<float>(L, L->ef, L->nf, L->gf, L->mf);
+        auto v = pull_view_f(L);
         attach_sell(v, L->sell_f);
         put(&v, sizeof(v));
     } else {
-        auto v = view_of<double>(L, L->ed, L->nd, L->gd, L->md);
+        auto v = pull_view_d(L);
         attach_sell(v, L->sell_d);
         put(&v, sizeof(v));
     }
@@ -1290,8 +1365,14 @@ int umx_layout_set_params(umx_layout* L, int num_dim, double a, double b, double
                 UMX_ERR_ARGUMENT, "a, b, gamma and learning_rate must be finite");
         if (L->dim != num_dim) {
             L->have_emb = false;
-            if (L->dtype == UMX_F32) L->mf.alloc(static_cast<size_t>(L->n) * static_cast<size_t>(num_dim));
-            else L->md.alloc(static_cast<size_t>(L->n) * static_cast<size_t>(num_dim));
+            const size_t count = static_cast<size_t>(L->n) * static_cast<size_t>(num_dim);
+            if (L->dtype == UMX_F32) {
+                L->mf.alloc(count);
+                L->mf2.alloc(count);
+            } else {
+                L->md.alloc(count);
+                L->md2.alloc(count);
+            }
         }
         L->dim = num_dim;
         L->a = a;
@@ -1308,8 +1389,13 @@ int umx_layout_set_embedding(umx_layout* L, const void* embedding, int device_pt
     return guard([&] {
         require(L != nullptr && embedding != nullptr, UMX_ERR_ARGUMENT, "NULL argument");
         require(L->params, UMX_ERR_STATE, "umx_layout_set_params must be called first");
-        const size_t bytes = dsize(L->dtype) * static_cast<size_t>(L->n) * static_cast<size_t>(L->dim);
-        if (bytes) UMX_HIP(hipMemcpyAsync(L->emb(), embedding, bytes, device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, L->s));
+        const size_t bytes = L->emb_bytes();
+        // both buffers: observations no work item writes (no edges) keep
+        // this position in every epoch
+        if (bytes) {
+            UMX_HIP(hipMemcpyAsync(L->emb_buf(0), embedding, bytes, device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, L->s));
+            UMX_HIP(hipMemcpyAsync(L->emb_buf(1), L->emb_buf(0), bytes, hipMemcpyDeviceToDevice, L->s));
+        }
         UMX_HIP(hipStreamSynchronize(L->s));
         L->have_emb = true;
     });
@@ -1319,7 +1405,7 @@ int umx_layout_get_embedding(const umx_layout* L, void* embedding, int device_pt
     return guard([&] {
         require(L != nullptr && embedding != nullptr, UMX_ERR_ARGUMENT, "NULL argument");
         require(L->have_emb, UMX_ERR_STATE, "no embedding has been set");
-        const size_t bytes = dsize(L->dtype) * static_cast<size_t>(L->n) * static_cast<size_t>(L->dim);
+        const size_t bytes = L->emb_bytes();
         if (bytes) UMX_HIP(hipMemcpyAsync(embedding, L->emb(), bytes, device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, L->s));
         UMX_HIP(hipStreamSynchronize(L->s));
     });
@@ -1342,6 +1428,9 @@ int umx_layout_run(umx_layout* L, int epoch_limit) {
             ensure_items(L);
             ensure_csr_current(L);
             prepare_events(L, count);
+            // push mode updates the current buffer in place; pull mode
+            // (num_dim > 4) is double-buffered like the sliced-ELL kernel
+            const bool push = L->mode == UMX_LAYOUT_PUSH;
             for (int q = 0; q < count; ++q) {
                 const int ep = first + q;
                 umx::EpochArgs ea;
@@ -1350,14 +1439,17 @@ int umx_layout_run(umx_layout* L, int epoch_limit) {
                 if (L->timing) UMX_HIP(hipEventRecord(L->events[2 * q], L->s));
                 if (L->dtype == UMX_F32) {
                     ea.alpha = epoch_alpha<float>(L->alpha0, ep, L->total_epochs);
-                    umx::launch_layout_epoch<float>(view_of<float>(L, L->ef, L->nf, L->gf, L->mf), ea, L->s);
+                    umx::launch_layout_epoch<float>(push ? in_place_view<float>(L, L->ef, L->nf, L->gf, L->mf, L->mf2)
+                                                         : pull_view_f(L), ea, L->s);
                 } else {
                     ea.alpha = epoch_alpha<double>(L->alpha0, ep, L->total_epochs);
-                    umx::launch_layout_epoch<double>(view_of<double>(L, L->ed, L->nd, L->gd, L->md), ea, L->s);
+                    umx::launch_layout_epoch<double>(push ? in_place_view<double>(L, L->ed, L->nd, L->gd, L->md, L->md2)
+                                                          : pull_view_d(L), ea, L->s);
                 }
                 if (L->timing) UMX_HIP(hipEventRecord(L->events[2 * q + 1], L->s));
                 umx::finish_layout_epoch(L->tally_part.p, L->tally.p, ep, L->epoch_ctr.p, L->s);
             }
+            if (push && count > 0) equalize_buffers(L, first);
             UMX_HIP(hipStreamSynchronize(L->s));
             collect_events(L, count);
         }
@@ -1383,20 +1475,23 @@ int umx_layout_run_serial(umx_layout* L, int epoch_limit, const int32_t* samples
         }
         UMX_HIP(hipMemsetAsync(L->cursor.p, 0, sizeof(unsigned long long) * 2, L->s));
         const int32_t* sp = samples ? (dsamples ? dsamples : reinterpret_cast<const int32_t*>(L->cursor.p)) : nullptr;
+        // in place on the current buffer (its epoch is L->current_epoch
+        // throughout the loop), then copied to the other
         for (int ep = L->current_epoch; ep < epoch_limit; ++ep) {
             umx::EpochArgs ea;
             ea.epoch = ep;
             ea.ekey = umx::counter_epoch_key(L->seed, ep);
             if (L->dtype == UMX_F32) {
                 ea.alpha = epoch_alpha<float>(L->alpha0, ep, L->total_epochs);
-                umx::launch_layout_serial<float>(view_of<float>(L, L->ef, L->nf, L->gf, L->mf), ea, sp, count,
-                                                 L->cursor.p, L->s);
+                umx::launch_layout_serial<float>(in_place_view<float>(L, L->ef, L->nf, L->gf, L->mf, L->mf2), ea, sp,
+                                                 count, L->cursor.p, L->s);
             } else {
                 ea.alpha = epoch_alpha<double>(L->alpha0, ep, L->total_epochs);
-                umx::launch_layout_serial<double>(view_of<double>(L, L->ed, L->nd, L->gd, L->md), ea, sp, count,
-                                                  L->cursor.p, L->s);
+                umx::launch_layout_serial<double>(in_place_view<double>(L, L->ed, L->nd, L->gd, L->md, L->md2), ea,
+                                                  sp, count, L->cursor.p, L->s);
             }
         }
+        if (epoch_limit > L->current_epoch) equalize_buffers(L, L->current_epoch);
         unsigned long long host_cursor[2] = {0, 0};
         UMX_HIP(hipMemcpyAsync(host_cursor, L->cursor.p, sizeof(host_cursor), hipMemcpyDeviceToHost, L->s));
         if (dsamples) UMX_HIP(hipFreeAsync(dsamples, L->s));
@@ -1583,7 +1678,8 @@ void restore_snapshot(umx_layout* L, const Snapshot& sn) {
     auto back = [&](void* to, const DevBuf<unsigned char>& from, size_t bytes) {
         if (bytes) UMX_HIP(hipMemcpyAsync(to, from.p, bytes, hipMemcpyDeviceToDevice, L->s));
     };
-    back(L->emb(), sn.emb, es * static_cast<size_t>(L->n) * L->dim);
+    back(L->emb_buf(0), sn.emb, L->emb_bytes());
+    back(L->emb_buf(1), sn.emb, L->emb_bytes());
     back(f ? static_cast<void*>(L->nf.p) : L->nd.p, sn.nx, es * static_cast<size_t>(L->nnz));
     back(f ? static_cast<void*>(L->gf.p) : L->gd.p, sn.nn, es * static_cast<size_t>(L->nnz));
     back(L->tally.p, sn.tally, sizeof(unsigned long long) * L->tally.n);
@@ -1696,6 +1792,13 @@ int umx_layout_autotune(umx_layout* L, int trial_epochs, int* tile_cols, int* wa
                 (void)hipEventDestroy(ev[1]);
                 L->timing = timing;
                 L->graph_block = gb;
+                // back to the state the call started from (best effort: the
+                // original error is the one reported)
+                try {
+                    ensure_csr_current(L);
+                    restore_snapshot(L, s0);
+                } catch (...) {
+                }
                 throw;
             }
             (void)hipEventDestroy(ev[0]);

@@ -266,154 +266,38 @@ __global__ void k_layout_serial(LayoutView<T> v, EpochConst ec, T ep, T alpha, c
     t[2] += self_hits;
 }
 
-constexpr int kMaskLoads = 8;     // schedule values fetched together when building a fire mask
-constexpr int kNegPrefetch = 8;   // negative samples whose coordinates are fetched together
-
-// Pull-mode epoch: the owner lane of observation i applies every move of i.
-// A stored edge i->t fires in exactly the same epochs as its reverse t->i
-// (the fuzzy union makes weights, hence epochs_per_sample, bit-symmetric), so
-// for each firing edge the lane applies i's head step of i->t and then the
-// tail step that t->i gives i (the reference computes it as the negative of
-// t's head move, -alpha*clip(c*(y_t - y_i)) = alpha*clip(c*(y_i - y_t))),
-// followed by i's negative samples.  Only the owner writes i, once, at the
-// end of its row (rows split over several work items add their moves with
-// atomics instead): the epoch is a pure gather.
-//
-// SIMD efficiency: a lane first turns up to 64 edges of its row into a fire
-// mask (schedule values fetched kMaskLoads at a time), then walks only the
-// firing edges with count-trailing-zeros, so on every trip of the loop each
-// lane that still has work applies a whole firing edge; items are sorted by
-// length, so lanes of a wavefront finish together.  The tail and
-// the first kNegPrefetch negative samples of an edge are fetched as one
-// batch of independent loads.  Steps are applied in the reference's order.
-template <typename T, int D, bool TRACE>
-__global__ void __launch_bounds__(kBlock) k_layout_pull(LayoutView<T> v, EpochConst ec, T ep, T alpha) {
-    const int64_t m = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    unsigned long long fires = 0, draws = 0, self_hits = 0;
-    if (m < v.n_items && v.item_len[m] > 0) {  // padding items have length 0
-        const uint32_t packed = static_cast<uint32_t>(v.item_row[m]);
-        const int64_t i = static_cast<int64_t>(packed & 0x7fffffffu);
-        const bool split = (packed >> 31) != 0;
-        const int64_t s = v.item_begin[m], e = s + v.item_len[m];
-        const Vec<T, D> h0 = load_point<T, D>(v.emb, i);
-        Vec<T, D> h = h0;
-        const T two_ab = -2 * v.a * v.b;
-        const T two_gb = 2 * v.gamma * v.b;
-        const uint32_t n32 = static_cast<uint32_t>(v.n);
-        bool moved = false;
-        for (int64_t seg = s; seg < e; seg += 64) {
-            const int nseg = e - seg < 64 ? static_cast<int>(e - seg) : 64;
-            unsigned long long mask = 0;
-            for (int c = 0; c < nseg; c += kMaskLoads) {
-                T nx[kMaskLoads];
-#pragma unroll
-                for (int u = 0; u < kMaskLoads; ++u) nx[u] = c + u < nseg ? ld_stream(v.nxt + seg + c + u) : ep + 1;
-#pragma unroll
-                for (int u = 0; u < kMaskLoads; ++u)
-                    if (!(nx[u] > ep)) mask |= 1ULL << (c + u);
-            }
-            while (mask) {
-                const int w = __builtin_ctzll(mask);
-                mask &= mask - 1;
-                const int64_t j = seg + w;
-                ++fires;
-                moved = true;
-                const int32_t t = ld_stream(v.tg + j);
-                const T nxj = ld_stream(v.nxt + j);
-                const T epsj = ld_stream(v.eps + j);
-                const T nnj = ld_stream(v.nxt_neg + j);
-                const T epns = epsj / v.nsr;
-                const int nneg = static_cast<int>((ep - nnj) / epns);
-                const uint64_t ej = edge_key(ec.ekey, static_cast<uint64_t>(j));
-                // tail and first batch of negatives in flight together
-                const Vec<T, D> r = load_point<T, D>(v.emb, t);
-                int32_t kk[kNegPrefetch];
-                Vec<T, D> q[kNegPrefetch];
-#pragma unroll
-                for (int u = 0; u < kNegPrefetch; ++u) {
-                    kk[u] = u < nneg ? draw_index(ej, static_cast<uint32_t>(u), n32) : -1;
-                    q[u] = load_point<T, D>(v.emb, kk[u] >= 0 ? kk[u] : i);
-                }
-#pragma unroll
-                for (int half = 0; half < 2; ++half) {
-                    const T d2 = sqdist<T, D>(h, r);
-                    const T pd2b = fast_pow(d2, v.b);
-                    const T gc = prod_div(two_ab * pd2b, d2 * (v.a * pd2b + 1));
-                    double* rec = TRACE ? trace_slot(v, half == 0 ? 0 : 2, i, t) : nullptr;
-#pragma unroll
-                    for (int d = 0; d < D; ++d) {
-                        const T g = alpha * clip4(gc * (h.c[d] - r.c[d]));
-                        if (TRACE && rec) {
-                            rec[3 + d] = h.c[d];
-                            rec[3 + D + d] = r.c[d];
-                            rec[3 + 2 * D + d] = g;
-                        }
-                        h.c[d] += g;
-                    }
-                }
-                for (int p0 = 0; p0 < nneg; p0 += kNegPrefetch) {
-                    if (p0 > 0) {
-#pragma unroll
-                        for (int u = 0; u < kNegPrefetch; ++u) {
-                            kk[u] = p0 + u < nneg ? draw_index(ej, static_cast<uint32_t>(p0 + u), n32) : -1;
-                            q[u] = load_point<T, D>(v.emb, kk[u] >= 0 ? kk[u] : i);
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < kNegPrefetch; ++u) {
-                        if (kk[u] < 0) break;
-                        ++draws;
-                        if (kk[u] == i) {
-                            ++self_hits;
-                            continue;
-                        }
-                        const T e2 = sqdist<T, D>(h, q[u]);
-                        const T rc = prod_div(two_gb, (static_cast<T>(0.001) + e2) * (v.a * fast_pow(e2, v.b) + 1));
-                        double* rec = TRACE ? trace_slot(v, 1, i, kk[u]) : nullptr;
-#pragma unroll
-                        for (int d = 0; d < D; ++d) {
-                            const T g = alpha * clip4(rc * (h.c[d] - q[u].c[d]));
-                            if (TRACE && rec) {
-                                rec[3 + d] = h.c[d];
-                                rec[3 + D + d] = q[u].c[d];
-                                rec[3 + 2 * D + d] = g;
-                            }
-                            h.c[d] += g;
-                        }
-                    }
-                }
-                st_stream(v.nxt + j, nxj + epsj);
-                st_stream(v.nxt_neg + j, nnj + static_cast<T>(nneg) * epns);
-            }
-        }
-        if (moved) {
-            if (split) {  // several items move this observation: merge their moves
-#pragma unroll
-                for (int d = 0; d < D; ++d) add_to(v.emb + i * D + d, h.c[d] - h0.c[d]);
-            } else {
-                store_point<T, D>(v.emb, i, h);
-            }
-        }
-    }
-    tally_add(v.tally_part, m / kWave, fires, draws, self_hits);
-}
-
-// Pull mode for any embedding dimension: the owner updates its point in place.
+// Pull mode for embeddings of more than four dimensions (the sliced-ELL
+// kernel covers one to four): the owner lane of observation i applies every
+// move of i.  A stored edge i->t fires in exactly the same epochs as its
+// reverse t->i (the fuzzy union makes weights, hence epochs_per_sample,
+// bit-symmetric), so for each firing edge the lane applies i's head step of
+// i->t, then the tail step that t->i gives i (the negative of t's head move,
+// -alpha*clip(c*(y_t - y_i)) = alpha*clip(c*(y_i - y_t))), then i's negative
+// samples, in the reference's order.  Double-buffered: every other point is
+// read from the epoch's input buffer, and i's working copy is its slot in the
+// output buffer (a whole row) or its piece's slot in split_delta (a piece of
+// a split row, turned into the piece's move at the end and summed onto the
+// head by launch_fold_split).  Nothing is shared, so the epoch is
+// deterministic.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_layout_pull_generic(LayoutView<T> v, EpochConst ec, T ep, T alpha) {
     const int64_t m = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     unsigned long long fires = 0, draws = 0, self_hits = 0;
-    if (m < v.n_items && v.item_len[m] > 0) {  // padding items have length 0
+    const int32_t aux = m < v.n_items ? v.item_aux[m] : kAuxPadding;
+    if (aux != kAuxPadding) {
         const int D = v.dim;
+        const T* src = (ec.epoch & 1) ? v.emb_alt : v.emb;
+        T* dst = (ec.epoch & 1) ? v.emb : v.emb_alt;
         const int64_t i = static_cast<int64_t>(static_cast<uint32_t>(v.item_row[m]) & 0x7fffffffu);
         const int64_t s = v.item_begin[m], e = s + v.item_len[m];
-        T* h = v.emb + i * D;
+        T* h = aux >= 0 ? v.split_delta + static_cast<int64_t>(aux) * D : dst + i * D;
+        for (int d = 0; d < D; ++d) h[d] = src[i * D + d];
         const uint32_t n32 = static_cast<uint32_t>(v.n);
         for (int64_t j = s; j < e; ++j) {
             const T nx = v.nxt[j];
             if (nx > ep) continue;
             ++fires;
-            const T* r = v.emb + static_cast<int64_t>(v.tg[j]) * D;
+            const T* r = src + static_cast<int64_t>(v.tg[j]) * D;
             for (int half = 0; half < 2; ++half) {
                 T d2 = 0;
                 for (int d = 0; d < D; ++d) {
@@ -423,7 +307,7 @@ __global__ void __launch_bounds__(kBlock) k_layout_pull_generic(LayoutView<T> v,
                 d2 = d2 > eps_of<T>() ? d2 : eps_of<T>();
                 const T pd2b = fast_pow(d2, v.b);
                 const T gc = (-2 * v.a * v.b * pd2b) / (d2 * (v.a * pd2b + 1));
-                for (int d = 0; d < D; ++d) add_to(h + d, alpha * clip4(gc * (h[d] - r[d])));
+                for (int d = 0; d < D; ++d) h[d] += alpha * clip4(gc * (h[d] - r[d]));
             }
             const T epns = v.eps[j] / v.nsr;
             const T nn = v.nxt_neg[j];
@@ -436,7 +320,7 @@ __global__ void __launch_bounds__(kBlock) k_layout_pull_generic(LayoutView<T> v,
                     ++self_hits;
                     continue;
                 }
-                const T* q = v.emb + static_cast<int64_t>(k) * D;
+                const T* q = src + static_cast<int64_t>(k) * D;
                 T e2 = 0;
                 for (int d = 0; d < D; ++d) {
                     const T z = h[d] - q[d];
@@ -444,13 +328,54 @@ __global__ void __launch_bounds__(kBlock) k_layout_pull_generic(LayoutView<T> v,
                 }
                 e2 = e2 > eps_of<T>() ? e2 : eps_of<T>();
                 const T rc = 2 * v.gamma * v.b / ((static_cast<T>(0.001) + e2) * (v.a * fast_pow(e2, v.b) + 1));
-                for (int d = 0; d < D; ++d) add_to(h + d, alpha * clip4(rc * (h[d] - q[d])));
+                for (int d = 0; d < D; ++d) h[d] += alpha * clip4(rc * (h[d] - q[d]));
             }
             v.nxt[j] = nx + v.eps[j];
             v.nxt_neg[j] = nn + static_cast<T>(nneg) * epns;
         }
+        if (aux >= 0)
+            for (int d = 0; d < D; ++d) h[d] -= src[i * D + d];
     }
     tally_add(v.tally_part, m / kWave, fires, draws, self_hits);
+}
+
+// The fold of split rows after a double-buffered pull epoch (one lane per
+// split row; see launch_fold_split).
+template <typename T, int D>
+__global__ void __launch_bounds__(kBlock) k_fold_split(LayoutView<T> v, int epoch_arg) {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (r >= v.n_split_rows) return;
+    const int epn = epoch_arg >= 0 ? epoch_arg : v.epoch_ctr[0] + (-1 - epoch_arg);
+    const T* src = (epn & 1) ? v.emb_alt : v.emb;
+    T* dst = (epn & 1) ? v.emb : v.emb_alt;
+    const int64_t i = v.split_row[r];
+    const int64_t first = v.split_first[r];
+    const int cnt = v.split_count[r];
+    Vec<T, D> h = load_point<T, D>(src, i);
+    for (int q = 0; q < cnt; ++q) {
+        const Vec<T, D> d = load_point<T, D>(v.split_delta, first + q);
+#pragma unroll
+        for (int c = 0; c < D; ++c) h.c[c] += d.c[c];
+    }
+    store_point<T, D>(dst, i, h);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_fold_split_generic(LayoutView<T> v, int epoch_arg) {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (r >= v.n_split_rows) return;
+    const int D = v.dim;
+    const int epn = epoch_arg >= 0 ? epoch_arg : v.epoch_ctr[0] + (-1 - epoch_arg);
+    const T* src = (epn & 1) ? v.emb_alt : v.emb;
+    T* dst = (epn & 1) ? v.emb : v.emb_alt;
+    const int64_t i = v.split_row[r];
+    const int64_t first = v.split_first[r];
+    const int cnt = v.split_count[r];
+    for (int c = 0; c < D; ++c) {
+        T h = src[i * D + c];
+        for (int q = 0; q < cnt; ++q) h += v.split_delta[(first + q) * D + c];
+        dst[i * D + c] = h;
+    }
 }
 
 // Work items (see build_work_items).  One thread per row: the expected
@@ -623,9 +548,10 @@ __global__ void __launch_bounds__(kBlock) k_item_scatter(const int64_t* __restri
                                                        int64_t n_items, const int32_t* __restrict__ of_row,
                                                        const int32_t* __restrict__ group_of_inst, int n_keys,
                                                        int64_t ntiles, const int64_t* __restrict__ offs, int waves,
-                                                       int n_groups, int64_t* __restrict__ obeg,
-                                                       int32_t* __restrict__ olen, int32_t* __restrict__ orow,
-                                                       int32_t* __restrict__ oinst) {
+                                                       int n_groups, const int32_t* __restrict__ iaux,
+                                                       int64_t* __restrict__ obeg, int32_t* __restrict__ olen,
+                                                       int32_t* __restrict__ orow, int32_t* __restrict__ oinst,
+                                                       int32_t* __restrict__ oaux) {
     extern __shared__ int wave_cnt[];  // [kBlock / kWave][n_keys]
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
@@ -660,22 +586,58 @@ __global__ void __launch_bounds__(kBlock) k_item_scatter(const int64_t* __restri
     obeg[to] = ibeg[m];
     olen[to] = ilen[m];
     orow[to] = irow[m];
+    oaux[to] = iaux[m];
     if (oinst) {
         const int64_t row = static_cast<int64_t>(static_cast<uint32_t>(irow[m]) & 0x7fffffffu);
         oinst[to] = of_row ? of_row[row] : 0;
     }
 }
 
-// Padding items: length 0, row 0, instance 0.
+// Padding items: length 0, row 0, instance 0, marked kAuxPadding.
 __global__ void __launch_bounds__(kBlock) k_clear_items(int64_t count, int64_t* __restrict__ obeg,
                                                       int32_t* __restrict__ olen, int32_t* __restrict__ orow,
-                                                      int32_t* __restrict__ oinst) {
+                                                      int32_t* __restrict__ oinst, int32_t* __restrict__ oaux) {
     const int64_t m = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (m >= count) return;
     obeg[m] = 0;
     olen[m] = 0;
     orow[m] = 0;
     oinst[m] = 0;
+    oaux[m] = kAuxPadding;
+}
+
+// Split rows (rows cut into more than one item).  Pass 1, per row: its
+// pieces (0 unless split) and whether it is split.
+__global__ void __launch_bounds__(kBlock) k_split_count(int64_t n, const int64_t* __restrict__ offs,
+                                                      int64_t* __restrict__ pieces, int64_t* __restrict__ is_split) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int64_t c = offs[i + 1] - offs[i];
+    pieces[i] = c > 1 ? c : 0;
+    is_split[i] = c > 1 ? 1 : 0;
+}
+
+// Pass 2, per row (after exclusive scans of pass 1): the aux value of each
+// of its items before the length sort (the pieces of a row are consecutive
+// there, in edge order) and, for a split row, its entry in the split list.
+__global__ void __launch_bounds__(kBlock) k_split_fill(int64_t n, const int64_t* __restrict__ offs,
+                                                     const int64_t* __restrict__ slot_off,
+                                                     const int64_t* __restrict__ row_off, int32_t* __restrict__ aux,
+                                                     int32_t* __restrict__ srow, int64_t* __restrict__ sfirst,
+                                                     int32_t* __restrict__ scount) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int64_t p0 = offs[i], c = offs[i + 1] - p0;
+    if (c == 1) {
+        aux[p0] = kAuxWholeRow;
+    } else if (c > 1) {
+        const int64_t s0 = slot_off[i];
+        for (int64_t q = 0; q < c; ++q) aux[p0 + q] = static_cast<int32_t>(s0 + q);
+        const int64_t r = row_off[i];
+        srow[r] = static_cast<int32_t>(i);
+        sfirst[r] = s0;
+        scount[r] = static_cast<int32_t>(c);
+    }
 }
 
 // Sum of 1 / eps over each instance's entries (blockIdx.y = instance).
@@ -725,21 +687,12 @@ void launch_layout_epoch(const LayoutView<T>& view, const EpochArgs& ea, hipStre
                 default: k_layout_rows_generic<T><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
             }
         }
-    } else if (v.trace && v.dim <= 4) {
-        switch (v.dim) {
-            case 1: k_layout_pull<T, 1, true><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
-            case 2: k_layout_pull<T, 2, true><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
-            case 3: k_layout_pull<T, 3, true><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
-            default: k_layout_pull<T, 4, true><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
-        }
     } else {
-        switch (v.dim) {
-            case 1: k_layout_pull<T, 1, false><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
-            case 2: k_layout_pull<T, 2, false><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
-            case 3: k_layout_pull<T, 3, false><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
-            case 4: k_layout_pull<T, 4, false><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
-            default: k_layout_pull_generic<T><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha); break;
-        }
+        // pull mode reaches this launcher only for more than four dimensions
+        // (one to four run the sliced-ELL kernel)
+        k_layout_pull_generic<T><<<grid, kBlock, 0, s>>>(v, ec, ep, alpha);
+        UMX_HIP(hipGetLastError());
+        launch_fold_split<T>(v, ea.epoch, s);
     }
     UMX_HIP(hipGetLastError());
 }
@@ -758,7 +711,7 @@ template <typename T>
 void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_split, int edges_cap,
                       const Instances& inst, const ItemPlacement& place, DevBuf<int64_t>& item_begin,
                       DevBuf<int32_t>& item_len, DevBuf<int32_t>& item_row, DevBuf<int32_t>& item_inst,
-                      int64_t& n_items, hipStream_t s) {
+                      DevBuf<int32_t>& item_aux, SplitPlan& split, int64_t& n_items, hipStream_t s) {
     if (!(fires_split > 0) || edges_cap < 1) throw Error(-1, "work split parameters must be positive");
     if (place.waves < 1 || place.waves > 4 || place.n_groups < 1 || place.n_groups > 32)
         throw Error(-1, "placement: waves must be 1..4 and groups 1..32");
@@ -785,6 +738,36 @@ void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_s
     if (n > 0)
         k_fill_items<T><<<grid, kBlock, 0, s>>>(n, rp, eps, fires_split, edges_cap, offs.p, ib.p, il.p, ir.p);
     UMX_HIP(hipGetLastError());
+
+    // split rows: piece slots numbered row by row in edge order
+    DevBuf<int32_t> ia;
+    ia.alloc(M);
+    {
+        DevBuf<int64_t> pieces, flags, slot_off, row_off;
+        pieces.alloc(static_cast<size_t>(n > 0 ? n : 1));
+        flags.alloc(static_cast<size_t>(n > 0 ? n : 1));
+        slot_off.alloc(static_cast<size_t>(n + 1));
+        row_off.alloc(static_cast<size_t>(n + 1));
+        if (n > 0) k_split_count<<<grid, kBlock, 0, s>>>(n, offs.p, pieces.p, flags.p);
+        UMX_HIP(hipGetLastError());
+        exclusive_scan_i64(pieces.p, slot_off.p, n, unused, s);
+        exclusive_scan_i64(flags.p, row_off.p, n, unused, s);
+        int64_t tot[2] = {0, 0};
+        UMX_HIP(hipMemcpyAsync(&tot[0], slot_off.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        UMX_HIP(hipMemcpyAsync(&tot[1], row_off.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        UMX_HIP(hipStreamSynchronize(s));
+        if (tot[0] > 0x7fffffffLL) throw Error(-5, "too many pieces of split rows");
+        split.slots = tot[0];
+        split.rows = tot[1];
+        split.row.alloc(static_cast<size_t>(split.rows));
+        split.first.alloc(static_cast<size_t>(split.rows));
+        split.count.alloc(static_cast<size_t>(split.rows));
+        if (n > 0)
+            k_split_fill<<<grid, kBlock, 0, s>>>(n, offs.p, slot_off.p, row_off.p, ia.p, split.row.p, split.first.p,
+                                                 split.count.p);
+        UMX_HIP(hipGetLastError());
+        UMX_HIP(hipStreamSynchronize(s));
+    }
 
     const int G = place.n_groups;
     const int n_keys = G * kLevels;
@@ -822,14 +805,15 @@ void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_s
     item_len.alloc(P);
     item_row.alloc(P);
     item_inst.alloc(P);
+    item_aux.alloc(P);
     if (padded > real)
         k_clear_items<<<static_cast<unsigned>((padded + kBlock - 1) / kBlock), kBlock, 0, s>>>(
-            padded, item_begin.p, item_len.p, item_row.p, item_inst.p);
+            padded, item_begin.p, item_len.p, item_row.p, item_inst.p, item_aux.p);
     if (real > 0) {
         const size_t lds = sizeof(int) * static_cast<size_t>((kBlock / kWave) * n_keys);
         k_item_scatter<<<static_cast<unsigned>(ntiles), kBlock, lds, s>>>(
-            ib.p, il.p, ir.p, real, of_row, goi, n_keys, ntiles, hoff.p, place.waves, G, item_begin.p,
-            item_len.p, item_row.p, item_inst.p);
+            ib.p, il.p, ir.p, real, of_row, goi, n_keys, ntiles, hoff.p, place.waves, G, ia.p, item_begin.p,
+            item_len.p, item_row.p, item_inst.p, item_aux.p);
         UMX_HIP(hipGetLastError());
     }
     UMX_HIP(hipStreamSynchronize(s));
@@ -842,12 +826,28 @@ void instance_fires(const int64_t* rp, const T* eps, const Instances& inst, doub
     UMX_HIP(hipGetLastError());
 }
 
+template <typename T>
+void launch_fold_split(const LayoutView<T>& v, int epoch, hipStream_t s) {
+    if (v.n_split_rows <= 0) return;
+    const unsigned grid = static_cast<unsigned>((v.n_split_rows + kBlock - 1) / kBlock);
+    switch (v.dim) {
+        case 1: k_fold_split<T, 1><<<grid, kBlock, 0, s>>>(v, epoch); break;
+        case 2: k_fold_split<T, 2><<<grid, kBlock, 0, s>>>(v, epoch); break;
+        case 3: k_fold_split<T, 3><<<grid, kBlock, 0, s>>>(v, epoch); break;
+        case 4: k_fold_split<T, 4><<<grid, kBlock, 0, s>>>(v, epoch); break;
+        default: k_fold_split_generic<T><<<grid, kBlock, 0, s>>>(v, epoch); break;
+    }
+    UMX_HIP(hipGetLastError());
+}
+
 template void build_work_items<float>(int64_t, const int64_t*, const float*, double, int, const Instances&,
                                       const ItemPlacement&, DevBuf<int64_t>&, DevBuf<int32_t>&, DevBuf<int32_t>&,
-                                      DevBuf<int32_t>&, int64_t&, hipStream_t);
+                                      DevBuf<int32_t>&, DevBuf<int32_t>&, SplitPlan&, int64_t&, hipStream_t);
 template void build_work_items<double>(int64_t, const int64_t*, const double*, double, int, const Instances&,
                                        const ItemPlacement&, DevBuf<int64_t>&, DevBuf<int32_t>&, DevBuf<int32_t>&,
-                                       DevBuf<int32_t>&, int64_t&, hipStream_t);
+                                       DevBuf<int32_t>&, DevBuf<int32_t>&, SplitPlan&, int64_t&, hipStream_t);
+template void launch_fold_split<float>(const LayoutView<float>&, int, hipStream_t);
+template void launch_fold_split<double>(const LayoutView<double>&, int, hipStream_t);
 template void instance_fires<float>(const int64_t*, const float*, const Instances&, double*, hipStream_t);
 template void instance_fires<double>(const int64_t*, const double*, const Instances&, double*, hipStream_t);
 

@@ -15,13 +15,24 @@
 // fire mask per lane, and walks the firing slots with count-trailing-zeros:
 // on every trip each lane applies one whole firing edge (pull mode: head
 // step, reverse-edge tail step, negative samples, exactly as in
-// k_layout_pull), fetching the tail and kNegFetch negative samples in one
-// batch of unconditional independent loads.  Updated schedule values go back to HBM
-// with coalesced stores.  Schedule loads and stores are non-temporal, so the
-// stream of schedule bytes does not evict from the L2 the embedding that the
-// gathers read.  The CSR arrays remain the canonical schedule state;
-// the host syncs the two copies when it switches between this kernel and the
-// CSR kernels (serial replay, push mode, export).
+// k_layout_pull_generic), fetching the tail and kNegFetch negative samples
+// in one batch of unconditional independent loads.  Updated schedule values
+// go back to HBM with coalesced stores.  Schedule loads and stores are
+// non-temporal, so the stream of schedule bytes does not evict from the L2
+// the embedding that the gathers read.  The CSR arrays remain the canonical
+// schedule state; the host syncs the two copies when it switches between
+// this kernel and the CSR kernels (serial replay, push mode, export).
+//
+// Determinism.  The embedding is double-buffered: epoch e reads every point
+// (its own head, tails, negative samples) from buffer e & 1 and writes its
+// result to the other buffer.  Every item writes, moved or not: a whole row
+// its head's new position, a piece of a split row its move into its own
+// slot, which a fold after the epoch adds to the head in edge order.  No
+// lane reads what another lane writes in the same epoch and nothing is
+// accumulated with atomics, so the result of an epoch does not depend on the
+// order the wavefronts run in, the tile width, the wavefronts per workgroup
+// or the placement: the same inputs give the same embedding bit for bit, and
+// a run that stops at any epoch and restarts equals one run.
 //
 // Batches.  The same kernel runs many independent problems (instances) in one
 // launch: each lane reads its item's instance constants (observation offset
@@ -185,22 +196,26 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
     const int64_t c = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + wid;
     const int epn = epoch_arg >= 0 ? epoch_arg : v.epoch_ctr[0] + (-1 - epoch_arg);
     const T ep = static_cast<T>(epn);
+    // double buffer: the epoch reads buffer epn & 1 and writes the other
+    const T* __restrict__ src = (epn & 1) ? v.emb_alt : v.emb;
+    T* __restrict__ dst = (epn & 1) ? v.emb : v.emb_alt;
 
     uint32_t fires = 0, draws = 0, self_hits = 0;  // one chunk's counts fit in 32 bits
     if (c < v.n_chunks) {
         const int64_t m = c * kWave + lane;
-        bool live = m < v.n_items;
+        const int32_t aux = m < v.n_items ? v.item_aux[m] : kAuxPadding;
+        const bool has_item = aux != kAuxPadding;
+        bool live = has_item;  // the item's instance runs this epoch
         const int width = v.chunk_width[c];
         const int64_t base = v.chunk_base[c];
 
         int64_t i = 0, e0 = 0, edge_off = 0, pt_off = 0;
         int32_t i32 = 0;
-        bool split = false;
         uint32_t n32 = 0;
         uint64_t ekey = 0;
         T alpha = 0;
         Vec<T, D> h0{}, h{};
-        if (live) {
+        if (has_item) {
             const int q = v.item_inst[m];
             const InstConst ic = v.inst[q];
             live = epn < ic.epochs;
@@ -209,17 +224,14 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
             n32 = ic.n;
             ekey = mix64(ic.seed + kGold * (static_cast<uint64_t>(epn) + 1));
             alpha = live ? v.alpha_tbl[static_cast<int64_t>(q) * v.alpha_stride + epn] : static_cast<T>(0);
-            const uint32_t packed = static_cast<uint32_t>(v.item_row[m]);
-            i = static_cast<int64_t>(packed & 0x7fffffffu);
+            i = static_cast<int64_t>(static_cast<uint32_t>(v.item_row[m]) & 0x7fffffffu);
             i32 = static_cast<int32_t>(i);
-            split = (packed >> 31) != 0;
             e0 = v.item_begin[m];
-            h0 = load_point<T, D>(v.emb, i);
+            h0 = load_point<T, D>(src, i);
             h = h0;
         }
         const T two_ab = -2 * v.a * v.b;
         const T two_gb = 2 * v.gamma * v.b;
-        bool moved = false;
         // The sampler's arithmetic, chosen once per wavefront: the 32-bit
         // path when every lane's instance has at most 2^24 observations (the
         // draw is the same function of the key, draw number and n either way;
@@ -237,7 +249,7 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
                     static_cast<int32_t>(pt_off + (small_n ? draw_index_small(ej, p, n32) : draw_index(ej, p, n32)));
                 const bool use = p0 + u < nneg;
                 kk[u] = use ? d : -1;
-                q[u] = load_point<T, D>(v.emb, use ? static_cast<int64_t>(d) : i);
+                q[u] = load_point<T, D>(src, use ? static_cast<int64_t>(d) : i);
             }
         };
 
@@ -265,7 +277,6 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
                 // instance-local entry number of this edge: keys the sampler
                 const int64_t j = e0 + t0 + k - edge_off;
                 ++fires;
-                moved = true;
                 const int32_t t = tile.tg[k][lane];
                 const T nxj = tile.nx[k][lane];
                 const T epsj = tile.eps[k][lane];
@@ -274,7 +285,7 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
                 const int nneg = static_cast<int>((ep - nnj) / epns);
                 draws += static_cast<uint32_t>(nneg > 0 ? nneg : 0);
                 const uint64_t ej = edge_key(ekey, static_cast<uint64_t>(j));
-                const Vec<T, D> r = load_point<T, D>(v.emb, t);
+                const Vec<T, D> r = load_point<T, D>(src, t);
                 int32_t kk[kNegFetch];
                 Vec<T, D> q[kNegFetch];
                 fetch(ej, 0, nneg, kk, q);
@@ -333,13 +344,16 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
                 }
             }
         }
-        if (live && moved) {
-            if (split) {
+        // every item writes, moved or not (the output buffer must hold the
+        // whole state): a whole row its new position, a piece of a split row
+        // its move, summed onto the head in edge order by the fold
+        if (aux >= 0) {
+            Vec<T, D> dlt;
 #pragma unroll
-                for (int d = 0; d < D; ++d) add_to(v.emb + i * D + d, h.c[d] - h0.c[d]);
-            } else {
-                store_point<T, D>(v.emb, i, h);
-            }
+            for (int d = 0; d < D; ++d) dlt.c[d] = h.c[d] - h0.c[d];
+            store_point<T, D>(v.split_delta, aux, dlt);
+        } else if (has_item) {
+            store_point<T, D>(dst, i, h);
         }
     }
     tally_add(v.tally_part + tally_ring_offset(epn), c, fires, draws, self_hits);

@@ -210,7 +210,13 @@ struct LayoutView {
     const T* eps;
     T* nxt;
     T* nxt_neg;
+    // Embedding.  The push and serial kernels update 'emb' in place.  The
+    // pull kernels are double-buffered: epoch e reads every observation from
+    // buffer e & 1 (emb for even epochs, emb_alt for odd ones) and writes its
+    // result to the other, so an epoch is a pure function of the state at
+    // its start, whatever order the wavefronts run in.
     T* emb;
+    T* emb_alt;
     int dim;
     int mode;
     T a, b, gamma, nsr;
@@ -228,6 +234,17 @@ struct LayoutView {
     const int32_t* item_len;
     const int32_t* item_row;
     int64_t n_items;
+    // item_aux[m]: kAuxPadding for padding items, kAuxWholeRow for an item
+    // that is a whole row, else the piece's slot in split_delta (D values:
+    // the piece's move of its head, summed onto the head by a fold after
+    // the epoch).  Split row r is observation split_row[r], whose pieces
+    // hold slots split_first[r] .. + split_count[r] - 1 in edge order.
+    const int32_t* item_aux;
+    T* split_delta;
+    const int32_t* split_row;
+    const int64_t* split_first;
+    const int32_t* split_count;
+    int64_t n_split_rows;
     // Sliced-ELL copy of the schedule for the production pull kernel (see
     // sell_kernels.hip): chunk c of 64 items has chunk_width[c] columns
     // starting at slot chunk_base[c]; slot (k, lane) = base + k * 64 + lane.
@@ -312,11 +329,31 @@ struct ItemPlacement {
     const int32_t* group_of_inst = nullptr;  // device, n_inst entries (n_groups > 1 only)
 };
 
+//
+// Split rows: item_aux (see LayoutView) and the list of split rows with the
+// slots of their pieces, numbered in edge order.
+constexpr int32_t kAuxPadding = -2;
+constexpr int32_t kAuxWholeRow = -1;
+struct SplitPlan {
+    DevBuf<int32_t> row;
+    DevBuf<int64_t> first;
+    DevBuf<int32_t> count;
+    int64_t rows = 0;   // split rows
+    int64_t slots = 0;  // pieces of split rows
+};
+
 template <typename T>
 void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_split, int edges_cap,
                       const Instances& inst, const ItemPlacement& place, DevBuf<int64_t>& item_begin,
                       DevBuf<int32_t>& item_len, DevBuf<int32_t>& item_row, DevBuf<int32_t>& item_inst,
-                      int64_t& n_items, hipStream_t s);
+                      DevBuf<int32_t>& item_aux, SplitPlan& split, int64_t& n_items, hipStream_t s);
+
+// After an epoch of a double-buffered pull kernel: observation split_row[r]
+// of every split row r gets its state at the start of the epoch plus the
+// moves of its pieces, added in edge order, written to the epoch's output
+// buffer.  'epoch' as for launch_layout_sell.
+template <typename T>
+void launch_fold_split(const LayoutView<T>& v, int epoch, hipStream_t s);
 
 // Expected firings per epoch of each instance (sum of 1 / eps over its
 // entries), written to fires[n_inst] (device).
