@@ -336,6 +336,23 @@ int umx_loss_backward(int64_t n, int num_dim, const float* embedding, int64_t nn
                       const double* ab, double gamma, uint64_t seed, int step, double clip,
                       const double* grad_out, float* grad_embedding, double* grad_ab, void* stream);
 
+/* umx_loss_backward without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving or training loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and errors as umx_loss_backward; its
+ * working memory (the double gradient accumulator and the index-error flag)
+ * is a fixed array in the library's device memory, held by each GPU the
+ * library runs on until the process ends (32 MiB), so n * num_dim may be at
+ * most 4194304 (UMX_ERR_CAPACITY beyond).  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (other threads or streams) race, and
+ * must be serialised by the caller.  Like umx_loss_backward it waits for its
+ * stream before returning (to report index errors). */
+int umx_loss_backward_noalloc(int64_t n, int num_dim, const float* embedding, int64_t nnz,
+                              const int32_t* heads, const int32_t* tails, const int64_t* keys,
+                              const float* coef, const int32_t* num_neg, int m, const double* ab,
+                              double gamma, uint64_t seed, int step, double clip, const double* grad_out,
+                              float* grad_embedding, double* grad_ab, void* stream);
+
 /* ------------------------------------------------------------------ */
 /* Host helpers (no GPU work)                                          */
 /* ------------------------------------------------------------------ */

@@ -35,6 +35,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <type_traits>
 
@@ -418,28 +419,88 @@ void loss_forward(const LossArgs& x, double* loss, hipStream_t s) {
     bad.check();
 }
 
-void loss_backward(const LossArgs& x, double clip, const double* grad_out, float* grad_emb, double* grad_ab,
-                   hipStream_t s) {
+void check_backward_args(const LossArgs& x, double clip, float* grad_emb, double* grad_ab) {
     check_args(x);
     if (grad_ab == nullptr || (x.n > 0 && grad_emb == nullptr)) throw Error(UMX_ERR_ARGUMENT, "output is NULL");
     if (!(clip >= 0)) throw Error(UMX_ERR_ARGUMENT, "clip must be >= 0");
-    BadFlag bad(s);
+}
+
+// The backward kernels: g64 (n * dim doubles) accumulates the embedding
+// gradient and is cleared here first; 'bad' must already be cleared.
+void backward_launch(const LossArgs& x, double clip, const double* grad_out, float* grad_emb, double* grad_ab,
+                     double* g64, int* bad, hipStream_t s) {
     const int64_t count = x.n * x.dim;
-    double* g64 = nullptr;
-    UMX_HIP(hipMallocAsync(reinterpret_cast<void**>(&g64), sizeof(double) * static_cast<size_t>(count > 0 ? count : 1), s));
     const unsigned zg = static_cast<unsigned>((count + kLossBlock - 1) / kLossBlock);
     if (count > 0) k_zero_f64<<<zg, kLossBlock, 0, s>>>(g64, count);
     k_zero_f64<<<1, kLossBlock, 0, s>>>(grad_ab, 2);
-    const LossView v = view_of(x, bad.p);
+    const LossView v = view_of(x, bad);
     if (x.nnz > 0)
         dispatch_dim(x.dim, [&](auto Dc) {
             k_loss_backward<decltype(Dc)::value><<<loss_grid(x.nnz), kLossBlock, 0, s>>>(v, clip, g64, grad_ab);
         });
     k_loss_finish<<<count > 0 ? zg : 1, kLossBlock, 0, s>>>(count, g64, grad_out, grad_emb, grad_ab);
+}
+
+void loss_backward(const LossArgs& x, double clip, const double* grad_out, float* grad_emb, double* grad_ab,
+                   hipStream_t s) {
+    check_backward_args(x, clip, grad_emb, grad_ab);
+    BadFlag bad(s);
+    const int64_t count = x.n * x.dim;
+    double* g64 = nullptr;
+    UMX_HIP(hipMallocAsync(reinterpret_cast<void**>(&g64), sizeof(double) * static_cast<size_t>(count > 0 ? count : 1), s));
+    backward_launch(x, clip, grad_out, grad_emb, grad_ab, g64, bad.p, s);
     const hipError_t e = hipGetLastError();
     (void)hipFreeAsync(g64, s);
     UMX_HIP(e);
     bad.check();
+}
+
+// ---- the backward pass without allocation ----
+//
+// Device memory of the no-allocation entry point: the index-error flag (the
+// first 256 bytes) and the double gradient accumulator, for at most
+// kNoAllocValues = n * num_dim values.  Each device holds its own copy at its
+// own address until the process ends; every call on a device shares it.
+constexpr int64_t kNoAllocValues = int64_t(1) << 22;  // 32 MiB of doubles
+constexpr int64_t kNoAllocFlagWords = 32;             // 256 bytes
+static __device__ __align__(256) double scratch[kNoAllocFlagWords + kNoAllocValues];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of 'scratch' on the current device, looked up once per
+// device (host code must never pass the array's own name to a kernel: it is
+// a host-side stand-in, not a device address).
+double* scratch_address() {
+    int dev = 0;
+    UMX_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw Error(UMX_ERR_ARGUMENT, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr)
+            throw Error(UMX_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+void loss_backward_noalloc(const LossArgs& x, double clip, const double* grad_out, float* grad_emb, double* grad_ab,
+                           hipStream_t s) {
+    check_backward_args(x, clip, grad_emb, grad_ab);
+    if (x.n * x.dim > kNoAllocValues)
+        throw Error(UMX_ERR_CAPACITY, "n * num_dim exceeds the no-allocation limit of 4194304 gradient values");
+    double* base = scratch_address();
+    int* bad = reinterpret_cast<int*>(base);
+    k_zero_i32<<<1, 1, 0, s>>>(bad);
+    backward_launch(x, clip, grad_out, grad_emb, grad_ab, base + kNoAllocFlagWords, bad, s);
+    UMX_HIP(hipGetLastError());
+    int h = 0;
+    UMX_HIP(hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    UMX_HIP(hipStreamSynchronize(s));
+    if (h) throw Error(UMX_ERR_GRAPH, "an edge index lies outside [0, n)");
 }
 
 }  // namespace
@@ -453,6 +514,16 @@ int umx_loss_forward(int64_t n, int num_dim, const float* embedding, int64_t nnz
     return umx::guarded([&] {
         const umx::LossArgs x{n, num_dim, embedding, nnz, heads, tails, keys, coef, num_neg, m, ab, gamma, seed, step};
         umx::loss_forward(x, loss, static_cast<hipStream_t>(stream));
+    });
+}
+
+int umx_loss_backward_noalloc(int64_t n, int num_dim, const float* embedding, int64_t nnz, const int32_t* heads,
+                              const int32_t* tails, const int64_t* keys, const float* coef, const int32_t* num_neg,
+                              int m, const double* ab, double gamma, uint64_t seed, int step, double clip,
+                              const double* grad_out, float* grad_embedding, double* grad_ab, void* stream) {
+    return umx::guarded([&] {
+        const umx::LossArgs x{n, num_dim, embedding, nnz, heads, tails, keys, coef, num_neg, m, ab, gamma, seed, step};
+        umx::loss_backward_noalloc(x, clip, grad_out, grad_embedding, grad_ab, static_cast<hipStream_t>(stream));
     });
 }
 
