@@ -364,7 +364,7 @@ int umx_choose_num_epochs(int num_epochs, int64_t size);
 /* Symmetric tridiagonal eigen-decomposition used by umx_graph_spectral_init
  * (host code): diagonal d[k] (overwritten with the eigenvalues, unsorted),
  * off-diagonal e[k - 1], eigenvectors into the columns of z (row-major
- * k x k).  UMX_ERR_STATE if the QL iteration does not converge. */
+ * k x k).  UMX_ERR_STATE if the QR iteration does not converge. */
 int umx_tridiagonal_eigen(int k, double* d, const double* e, double* z);
 
 #ifdef __cplusplus

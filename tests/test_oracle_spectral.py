@@ -7,7 +7,7 @@
 * the reference's component known answers (spectral_init.cpp:160-200) and
   scipy's connected_components on random graphs;
 * a multi-component graph fails (spectral_init.cpp:130-149);
-* the library's tridiagonal QL solver (host code) against
+* the library's tridiagonal QR solver (host code) against
   scipy.linalg.eigh_tridiagonal.
 """
 import ctypes as C
@@ -126,15 +126,20 @@ def test_multiple_components_fail():
     assert not ok and Y is None
 
 
-@pytest.mark.parametrize("k", [1, 2, 3, 8, 60, 300])
+@pytest.mark.parametrize("k", [1, 2, 3, 8, 60, 300, 512])
 def test_tridiagonal_solver_vs_scipy(umx, k):
     rng = np.random.default_rng(k)
-    for trial in range(3):
+    for trial in range(5):
         d = rng.normal(size=k)
         e = rng.normal(size=max(k - 1, 0))
         if trial == 2 and k > 3:
             d[:] = 1.0  # clustered spectrum
             e *= 1e-3
+        if trial == 3 and k > 3:
+            e[k // 2] = 0.0  # already split into two blocks
+            e[0] = 0.0
+        if trial == 4:
+            d = 2.0 - np.abs(d) * 1e-3 ** np.linspace(0, 1, k)  # graded, near 2 like the Lanczos T
         dd = d.copy()
         z = np.zeros((k, k))
         rc = umx.lib().umx_tridiagonal_eigen(k, dd.ctypes.data_as(C.c_void_p),

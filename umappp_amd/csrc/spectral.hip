@@ -12,7 +12,7 @@
 //    classical Gram-Schmidt pass against all previous vectors, and a second
 //    pass where the first cancelled most of the direction) builds V (k
 //    vectors of length n, resident in HBM) and
-//    the tridiagonal T; T's eigen-decomposition (implicit-shift QL, host)
+//    the tridiagonal T; T's eigen-decomposition (implicit symmetric QR, host)
 //    gives Ritz pairs; the Krylov dimension doubles until the num_dim + 1
 //    largest Ritz pairs have residuals below tol * theta, else the
 //    initialisation fails (the reference fails when irlba does not converge);
