@@ -224,17 +224,27 @@ int umx_layout_run(umx_layout* L, int epoch_limit);
 /* Update scheme of umx_layout_run.  UMX_LAYOUT_PULL (default): every
  * observation's owner lane applies all of its moves — its head steps, the
  * tail steps of the reverse edges (a stored edge and its reverse fire in the
- * same epochs) and its negative samples — and writes the observation once per
- * epoch; no atomics.  UMX_LAYOUT_PUSH: each head step also moves the tail
- * with float atomics, as the reference's loop does. */
+ * same epochs) and its negative samples.  Each epoch reads the embedding as
+ * it was at the epoch's start and writes a new one (two buffers), with no
+ * atomics, so results are bit-reproducible: the same for any launch shape,
+ * and a run stopped and restarted at any epoch equals one run
+ * (Options.hpp:203-205).  UMX_LAYOUT_PUSH: each head step also moves the
+ * tail with float atomics, as the reference's loop does; not reproducible
+ * bit for bit.  Pull mode needs every entry's reverse with the same weight,
+ * as combine_neighbor_sets leaves it: a layout made from a graph with
+ * one-way entries starts in push mode and refuses pull mode
+ * (UMX_ERR_STATE); a batched one is refused (UMX_ERR_GRAPH). */
 enum umx_layout_mode { UMX_LAYOUT_PULL = 0, UMX_LAYOUT_PUSH = 1 };
 int umx_layout_set_mode(umx_layout* L, int mode);
+int umx_layout_get_mode(const umx_layout* L, int* mode);
 /* Load balance of umx_layout_run: one lane runs one work item, normally a
  * whole row (its steps stay sequential, as in the reference).  Rows expected
  * to fire more than fires_per_item edges per epoch (sum of
  * 1 / epochs_per_sample), or longer than edges_per_item, are cut into items
- * of about equal expected firings whose moves of the head are merged with
- * atomics; items are run longest first.  Defaults 32.0 and 2^31 - 1. */
+ * of about equal expected firings; each starts from the head's position at
+ * the epoch's start, and their moves are added to the head in edge order
+ * after the epoch.  Items are run longest first.  Defaults 32.0 and
+ * 2^31 - 1. */
 int umx_layout_set_work_split(umx_layout* L, double fires_per_item, int edges_per_item);
 /* LDS tile of the pull kernel: schedule columns staged per wavefront at a
  * time (4, 8, 16 or 32), or 0 (the default) to choose from the number of

@@ -66,7 +66,7 @@ def test_loss_and_gradient_match_oracle(torch_dev, dim):
     P = oracle.LossProblem(emb.astype(np.float64), heads, tails, A, B, 1.3, seed=77, step=4, keys=keys,
                            coef=coef.astype(np.float64), num_neg=nneg)
     want = P.loss()
-    assert abs(float(loss) - want) <= 1e-12 * abs(want)
+    assert abs(float(loss.detach()) - want) <= 1e-12 * abs(want)
     g_ref, gab_ref = P.grad()
     g = y.grad.cpu().numpy().astype(np.float64)
     scale = np.max(np.abs(g_ref))
@@ -96,6 +96,30 @@ def test_gradient_with_shuffled_edges(torch_dev):
     g = y.grad.cpu().numpy().astype(np.float64)
     scale = np.max(np.abs(g_ref))
     assert np.all(np.abs(g - g_ref) <= 1e-6 * np.abs(g_ref) + 1e-7 * scale)
+
+
+@pytest.mark.parametrize("seed", [2**63 + 5, 2**64 - 1])
+def test_seed_with_top_bit(torch_dev, seed):
+    """Seeds of 2^63 and more reach the sampler unchanged (as the layout's
+    uint64 optimize_seed does): the loss and gradient equal the oracle's at
+    that seed, not at the seed with its top bit cleared."""
+    torch, dev = torch_dev
+    from umappp_amd.autograd import umap_loss
+
+    emb, heads, tails, coef, nneg, keys = _random_problem(12, n=150, dim=2, deg=6)
+    y, h, t, c, nn, k = _to(torch, dev, emb, heads, tails, coef, nneg, keys)
+    y.requires_grad_(True)
+    loss = umap_loss(y, A, B, h, t, keys=k, coef=c, num_neg=nn, seed=seed, step=2)
+    loss.backward()
+    P = oracle.LossProblem(emb.astype(np.float64), heads, tails, A, B, 1.0, seed=seed, step=2, keys=keys,
+                           coef=coef.astype(np.float64), num_neg=nneg)
+    assert abs(float(loss.detach()) - P.loss()) <= 1e-12 * abs(P.loss())
+    Q = oracle.LossProblem(emb.astype(np.float64), heads, tails, A, B, 1.0, seed=seed & (2**63 - 1), step=2,
+                           keys=keys, coef=coef.astype(np.float64), num_neg=nneg)
+    assert Q.loss() != P.loss()
+    g_ref, _ = P.grad()
+    g = y.grad.cpu().numpy().astype(np.float64)
+    assert np.all(np.abs(g - g_ref) <= 1e-6 * np.abs(g_ref) + 1e-7 * np.max(np.abs(g_ref)))
 
 
 @pytest.mark.parametrize("n,deg", [(401, 7), (333, 5), (70, 3)])

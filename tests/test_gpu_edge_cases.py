@@ -109,3 +109,54 @@ def test_batch_with_degenerate_and_ragged_instances(umx):
     lonely = np.diff(ed.cumulative_num_edges) == 0
     assert lonely[off[3]:].all()
     assert np.array_equal(emb[lonely], emb0[lonely]) and np.all(np.isfinite(emb))
+
+
+def test_one_way_edges_run_in_push_mode(umx):
+    """A graph that did not go through the fuzzy union may hold an entry
+    without its reverse.  Pull mode would apply a reverse step that does not
+    exist, so such a layout runs in push mode (every stored edge moves its
+    head and its tail, as in the reference's loop) and refuses pull mode;
+    per epoch its steps equal the oracle's; a batched layout of it is
+    refused.  Combined graphs and symmetric raw CSR stay in pull mode."""
+    rp, idx, d = random_ragged_graph(7, 600, 12, np.float32, 0)
+    w, _, _ = oracle.neighbor_similarities(rp, d)
+    crp, cidx, cw = oracle.combine_neighbor_sets(rp, idx, w, 1.0)
+    sym = umx.Layout(csr=umx.NeighborList(crp, cidx, cw), num_epochs=30, negative_sample_rate=5.0)
+    assert sym.mode() == umx.Layout.PULL
+    # one-way: drop the reverse of one entry from a symmetric graph
+    r = int(np.argmax(np.diff(crp) > 2))
+    t = int(cidx[crp[r]])
+    rev = int(crp[t] + np.nonzero(cidx[crp[t]:crp[t + 1]] == r)[0][0])
+    keep = np.ones(cidx.size, bool)
+    keep[rev] = False
+    orp = crp.copy()
+    orp[t + 1:] -= 1
+    oidx, ow = cidx[keep], cw[keep]
+    T, nsr, s = 30, 5.0, 3
+    lay = umx.Layout(csr=umx.NeighborList(orp, oidx, ow), num_epochs=T, negative_sample_rate=nsr)
+    assert lay.mode() == umx.Layout.PUSH
+    with pytest.raises(umx.UmxError) as e:
+        lay.set_mode(umx.Layout.PULL)
+    assert e.value.code == -4
+    ep = oracle.similarities_to_epochs(orp, oidx, ow, T, nsr)
+    orc = oracle.Layout(ep, T, nsr, rng_kind=oracle.RNG_COUNTER, seed=s, dtype=np.float32)
+    emb0 = np.random.default_rng(1).uniform(-5, 5, size=(600, 2)).astype(np.float32)
+    lay.set_params(2, A, B, 1.0, 1.0, s)
+    lay.set_embedding(emb0)
+    lay.set_trace(400_000)
+    orc.trace(400_000)
+    eo = emb0.copy()
+    key = lambda a: a[np.lexsort(tuple(a[:, c] for c in range(a.shape[1] - 1, -1, -1)))]
+    for limit in (5, 12, T):
+        lay.run(limit)
+        orc.run(eo, A, B, epoch_limit=limit)
+        assert np.array_equal(key(lay.get_trace()[:, :3].astype(np.int64)), key(orc.traced()))
+        lay.set_trace(400_000)
+        orc.trace(400_000)
+    ed = lay.epoch_data()
+    nx, nn, _, tal = orc.state()
+    assert np.array_equal(ed.epoch_of_next_sample, nx) and np.array_equal(lay.tallies(), tal)
+    with pytest.raises(umx.UmxError) as e:
+        g = umx.Graph.batch([umx.NeighborList(orp, oidx, ow), umx.NeighborList(crp, cidx, cw)])
+        umx.Layout(graph=g, num_epochs=np.array([T, T], np.int32), negative_sample_rate=nsr)
+    assert e.value.code == -2

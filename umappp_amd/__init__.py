@@ -88,6 +88,7 @@ def _declare(L):
         "umx_layout_run_serial": (C.c_int, [_vp, C.c_int, _vp, C.c_int64]),
         "umx_layout_tallies": (C.c_int, [_vp, _vp, C.c_int64]),
         "umx_layout_set_mode": (C.c_int, [_vp, C.c_int]),
+        "umx_layout_get_mode": (C.c_int, [_vp, C.c_void_p]),
         "umx_layout_set_work_split": (C.c_int, [_vp, C.c_double, C.c_int]),
         "umx_layout_set_tile": (C.c_int, [_vp, C.c_int]),
         "umx_layout_work_items": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int64, _i64p]),
@@ -456,8 +457,15 @@ class Layout:
     PULL, PUSH = 0, 1
 
     def set_mode(self, mode: int):
-        """Layout.PULL (default, no atomics) or Layout.PUSH (tails via atomics)."""
+        """Layout.PULL (default: double-buffered, no atomics, bit-reproducible)
+        or Layout.PUSH (tails via atomics).  Graphs with one-way entries run in
+        push mode only."""
         _check(lib().umx_layout_set_mode(self._h, int(mode)))
+
+    def mode(self) -> int:
+        m = C.c_int()
+        _check(lib().umx_layout_get_mode(self._h, C.byref(m)))
+        return m.value
 
     def set_work_split(self, fires_per_item: float = 32.0, edges_per_item: int = 2**31 - 1):
         _check(lib().umx_layout_set_work_split(self._h, float(fires_per_item), int(edges_per_item)))

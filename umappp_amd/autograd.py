@@ -87,7 +87,14 @@ def umap_loss(emb, a, b, heads, tails, *, keys=None, coef=None, num_neg=None, m=
         keys if keys is not None else empty(torch.int64),
         coef if coef is not None else empty(torch.float32),
         num_neg if num_neg is not None else empty(torch.int32),
-        int(m), float(gamma), int(seed) & (2**63 - 1), int(step), float(clip), int(stream), bool(value))
+        int(m), float(gamma), _as_int64(seed), int(step), float(clip), int(stream), bool(value))
+
+
+def _as_int64(seed) -> int:
+    """A uint64 seed as the int64 with the same bits (two's complement), which
+    the extension casts back to uint64 unchanged."""
+    s = int(seed) & (2**64 - 1)
+    return s - 2**64 if s >= 2**63 else s
 
 
 def edge_heads(row_ptr):

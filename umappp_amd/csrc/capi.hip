@@ -254,6 +254,9 @@ struct umx_graph {
     DevBuf<double> vd;
     DevBuf<double> rho, sigma;
     bool have_rho_sigma = false;
+    // weights known to be symmetric (set by combine: its output holds every
+    // entry's reverse with the same value); otherwise a layout checks
+    bool symmetric = false;
     mutable umx::SpectralReport spectral;  // diagnostics of the last spectral initialisation
     // instances (a plain graph is one instance)
     std::vector<int64_t> inst_off;  // host, n_inst + 1
@@ -557,6 +560,7 @@ int umx_graph_neighbor_similarities(umx_graph* g, double local_connectivity, dou
                                              g->max_degree, g->s);
         UMX_HIP(hipStreamSynchronize(g->s));
         g->have_rho_sigma = true;
+        g->symmetric = false;  // per-row weights
     });
 }
 
@@ -581,6 +585,7 @@ int umx_graph_combine(umx_graph* g, double mix_ratio) {
         UMX_HIP(hipMemcpyAsync(&g->nnz, g->rp.p + g->n, sizeof(int64_t), hipMemcpyDeviceToHost, g->s));
         DevBuf<int64_t> unused;
         g->max_degree = umx::max_row_length(g->rp.p, g->n, unused, g->s);
+        g->symmetric = true;
     });
 }
 
@@ -675,6 +680,7 @@ struct umx_layout {
     DevBuf<unsigned long long> cursor;  // serial replay cursor + overrun flag
     int dim = 0;
     int mode = UMX_LAYOUT_PULL;
+    bool one_way = false;  // some entry lacks its reverse: push mode only
     DevBuf<int64_t> item_begin;  // work items of the production kernels
     DevBuf<int32_t> item_len;
     DevBuf<int32_t> item_row;
@@ -1076,6 +1082,21 @@ void layout_from_graph(umx_layout* L, const umx_graph* g, const std::vector<int3
     DevBuf<int64_t> unused;
     L->max_degree = umx::max_row_length(L->rp.p, L->n, unused, L->s);
     UMX_HIP(hipStreamSynchronize(L->s));
+    // Pull mode applies the tail step of each edge t -> i as the reverse of
+    // i -> t; a graph that did not come out of combine_neighbor_sets is
+    // checked, and one with an entry lacking its reverse (same epochs per
+    // sample) runs in push mode, which handles any graph as the reference
+    // does.
+    if (!g->symmetric) {
+        const bool ok = L->dtype == UMX_F32 ? umx::has_reverse_edges<float>(L->n, L->rp.p, L->tg.p, L->ef.p, L->s)
+                                            : umx::has_reverse_edges<double>(L->n, L->rp.p, L->tg.p, L->ed.p, L->s);
+        if (!ok) {
+            require(ni == 1, UMX_ERR_GRAPH,
+                    "batched layouts need every entry's reverse with the same weight (combine the graph first)");
+            L->one_way = true;
+            L->mode = UMX_LAYOUT_PUSH;
+        }
+    }
 }
 
 // Timing events are kept in a process-wide pool per device and handed back
@@ -1551,7 +1572,16 @@ int umx_layout_set_mode(umx_layout* L, int mode) {
     return guard([&] {
         require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
         require(mode == UMX_LAYOUT_PULL || mode == UMX_LAYOUT_PUSH, UMX_ERR_ARGUMENT, "unknown layout mode");
+        require(mode == UMX_LAYOUT_PUSH || !L->one_way, UMX_ERR_STATE,
+                "pull mode needs every entry's reverse with the same weight; this graph has one-way entries");
         L->mode = mode;
+    });
+}
+
+int umx_layout_get_mode(const umx_layout* L, int* mode) {
+    return guard([&] {
+        require(L != nullptr && mode != nullptr, UMX_ERR_ARGUMENT, "NULL argument");
+        *mode = L->mode;
     });
 }
 

@@ -826,6 +826,46 @@ void instance_fires(const int64_t* rp, const T* eps, const Instances& inst, doub
     UMX_HIP(hipGetLastError());
 }
 
+// Pull mode needs every stored edge i -> t to have a reverse t -> i with the
+// same epochs_per_sample (then both fire in the same epochs).  One lane per
+// row; a linear search of row t (rows need not be sorted); *bad is set when
+// an edge has no such reverse.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_check_reverse(int64_t n, const int64_t* __restrict__ rp,
+                                                        const int32_t* __restrict__ tg, const T* __restrict__ eps,
+                                                        int* __restrict__ bad) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    for (int64_t e = rp[i]; e < rp[i + 1]; ++e) {
+        const int64_t t = tg[e];
+        const T w = eps[e];
+        bool found = false;
+        if (t >= 0 && t < n) {
+            for (int64_t f = rp[t]; f < rp[t + 1] && !found; ++f) found = tg[f] == i && eps[f] == w;
+        }
+        if (!found) {
+            atomicOr(bad, 1);
+            return;
+        }
+    }
+}
+
+__global__ void k_clear_flag(int* p) { *p = 0; }
+
+template <typename T>
+bool has_reverse_edges(int64_t n, const int64_t* rp, const int32_t* tg, const T* eps, hipStream_t s) {
+    if (n <= 0) return true;
+    DevBuf<int> flag;
+    flag.alloc(1);
+    k_clear_flag<<<1, 1, 0, s>>>(flag.p);
+    k_check_reverse<T><<<static_cast<unsigned>((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(n, rp, tg, eps, flag.p);
+    UMX_HIP(hipGetLastError());
+    int h = 0;
+    UMX_HIP(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    UMX_HIP(hipStreamSynchronize(s));
+    return h == 0;
+}
+
 template <typename T>
 void launch_fold_split(const LayoutView<T>& v, int epoch, hipStream_t s) {
     if (v.n_split_rows <= 0) return;
@@ -846,6 +886,8 @@ template void build_work_items<float>(int64_t, const int64_t*, const float*, dou
 template void build_work_items<double>(int64_t, const int64_t*, const double*, double, int, const Instances&,
                                        const ItemPlacement&, DevBuf<int64_t>&, DevBuf<int32_t>&, DevBuf<int32_t>&,
                                        DevBuf<int32_t>&, DevBuf<int32_t>&, SplitPlan&, int64_t&, hipStream_t);
+template bool has_reverse_edges<float>(int64_t, const int64_t*, const int32_t*, const float*, hipStream_t);
+template bool has_reverse_edges<double>(int64_t, const int64_t*, const int32_t*, const double*, hipStream_t);
 template void launch_fold_split<float>(const LayoutView<float>&, int, hipStream_t);
 template void launch_fold_split<double>(const LayoutView<double>&, int, hipStream_t);
 template void instance_fires<float>(const int64_t*, const float*, const Instances&, double*, hipStream_t);

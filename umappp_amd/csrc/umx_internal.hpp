@@ -348,6 +348,11 @@ void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_s
                       DevBuf<int32_t>& item_len, DevBuf<int32_t>& item_row, DevBuf<int32_t>& item_inst,
                       DevBuf<int32_t>& item_aux, SplitPlan& split, int64_t& n_items, hipStream_t s);
 
+// Whether every entry i -> t of an EpochData CSR has a reverse entry t -> i
+// with the same epochs_per_sample (what pull mode needs); synchronises s.
+template <typename T>
+bool has_reverse_edges(int64_t n, const int64_t* rp, const int32_t* tg, const T* eps, hipStream_t s);
+
 // After an epoch of a double-buffered pull kernel: observation split_row[r]
 // of every split row r gets its state at the start of the epoch plus the
 // moves of its pieces, added in edge order, written to the epoch's output
