@@ -491,16 +491,18 @@ class Layout:
         _check(lib().umx_layout_set_trace(self._h, int(max_records)))
         self._trace_cap = int(max_records)
 
-    def get_trace(self) -> np.ndarray:
+    def get_trace(self, partial: bool = False) -> np.ndarray:
         """Recorded steps since the last call, as an (m, 3 + 3*num_dim) array:
-        kind, head, other, head before, other before, head move."""
+        kind, head, other, head before, other before, head move.  More steps
+        than the capacity is an error unless ``partial``, which returns the
+        steps that were recorded (the first ones to claim a record)."""
         width = 3 + 3 * self.num_dim
         out = np.zeros((self._trace_cap, width))
         total = C.c_int64()
         _check(lib().umx_layout_get_trace(self._h, _ptr(out), self._trace_cap, C.byref(total)))
-        if total.value > self._trace_cap:
+        if total.value > self._trace_cap and not partial:
             raise UmxError(-6, f"trace overflow: {total.value} steps, capacity {self._trace_cap}")
-        return out[: total.value]
+        return out[: min(total.value, self._trace_cap)]
 
     def instances(self):
         """(offsets[n_inst + 1], num_epochs[n_inst]) of the layout's instances."""
