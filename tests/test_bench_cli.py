@@ -13,3 +13,17 @@ def test_more_than_one_gpu_is_refused():
     assert r.returncode == 2
     assert "one GPU" in r.stderr
     assert r.stdout == ""
+
+
+def test_host_cores_is_the_affinity_capped_by_the_share():
+    sys.path.insert(0, str(ROOT))
+    import os
+
+    import bench
+
+    cores, info = bench.host_cores()
+    assert info["affinity"] == len(os.sched_getaffinity(0))
+    assert 1 <= cores <= info["affinity"]
+    for cap in (info["cgroup_quota"], info["omp_num_threads"]):
+        if cap is not None:
+            assert cores <= cap

@@ -20,6 +20,7 @@ import pytest
 import oracle
 from _data import knn_lists, mixture, normal_data
 from _quality import knn_preservation, trustworthiness
+from _steps import check_reverse_steps, check_steps
 
 pytestmark = pytest.mark.gpu
 
@@ -204,11 +205,8 @@ def test_batched_samples_match_plain_runs(umx):
             assert np.array_equal(key(mine), key(pt)), (limit, q)
         alpha = oracle.epoch_alpha(1.0, limit - 1, 30, np.float32)
         kinds = tr[:, 0].astype(np.int64)
-        for r in tr[kinds != 2][::3]:
-            h = r[3:5].astype(np.float32)
-            o = r[5:7].astype(np.float32)
-            ref = oracle.step_delta(int(r[0]), h, o, A, B, 1.0, alpha, np.float32).astype(np.float64)
-            assert np.all(np.abs(r[7:9] - ref) <= 1e-5 * np.abs(ref) + 1e-6)
+        check_steps(tr[kinds != 2], [int((kinds != 2).sum())], 2, np.float32, A, B, 1.0, 30, limit - 1, 1e-5, 1e-6)
+        check_reverse_steps(tr[kinds == 2], 2, np.float32, A, B, 1.0, alpha, 1e-5, 1e-6)
 
 
 def test_batched_autotune_restores_state(umx):

@@ -3,9 +3,10 @@
 The reference guarantees that a run's embedding does not depend on how it is
 split into calls or on the thread count (tests/src/optimize_layout.cpp:89-125,
 tests/src/umappp.cpp:104-138).  The production kernels here are
-double-buffered (each epoch reads the state at its start and writes a new
-one) and fold the pieces of split rows in edge order, so the same holds on
-the GPU, bit for bit:
+double-buffered (each epoch, phase after phase, reads the state at its start
+or as earlier phases left it, and writes a new one) and fold the pieces of
+split rows in edge order, so the same holds on the GPU, bit for bit, for a
+given number of phases:
 
 * repeated runs, runs stopped and restarted at any epoch, runs replayed from
   captured epoch blocks, and every launch shape (LDS tile, wavefronts per
@@ -34,7 +35,7 @@ def _c1_graph(dtype=np.float32):
     return crp, cidx, cw.astype(dtype)
 
 
-def _run(umx, g, dim=2, dtype=np.float32, tile=0, waves=1, split=32.0, block=0, stops=(T,), seed=4242):
+def _run(umx, g, dim=2, dtype=np.float32, tile=0, waves=1, split=32.0, block=0, stops=(T,), seed=4242, phases=4):
     crp, cidx, cw = g
     n = crp.size - 1
     lay = umx.Layout(csr=umx.NeighborList(crp, cidx, cw), num_epochs=T, negative_sample_rate=5.0)
@@ -42,6 +43,7 @@ def _run(umx, g, dim=2, dtype=np.float32, tile=0, waves=1, split=32.0, block=0, 
     lay.set_tile(tile)
     lay.set_placement(waves, 1)
     lay.set_work_split(split)
+    lay.set_phases(phases)
     lay.set_graph_block(block)
     lay.set_embedding(np.random.default_rng(5).uniform(-10, 10, size=(n, dim)).astype(dtype))
     for s in stops:
@@ -50,17 +52,27 @@ def _run(umx, g, dim=2, dtype=np.float32, tile=0, waves=1, split=32.0, block=0, 
     return lay.get_embedding(), ed.epoch_of_next_sample, lay.tallies()
 
 
-@pytest.mark.parametrize("split", [32.0, 2.0])
-def test_every_launch_shape_gives_the_same_embedding(umx, split):
+@pytest.mark.parametrize("split,phases", [(32.0, 4), (2.0, 4), (32.0, 1), (32.0, 8)])
+def test_every_launch_shape_gives_the_same_embedding(umx, split, phases):
     g = _c1_graph()
-    base = _run(umx, g, split=split)
+    base = _run(umx, g, split=split, phases=phases)
     assert np.all(np.isfinite(base[0]))
     variants = [dict(tile=t, waves=w) for t in (4, 8, 16, 32) for w in (1, 2, 4) if t * w <= 64]
     variants += [dict(block=25), dict(block=64), dict(stops=(1, 37, 38, 90, T)), dict(stops=(T,))]
     for v in variants:
-        got = _run(umx, g, split=split, **v)
+        got = _run(umx, g, split=split, phases=phases, **v)
         assert np.array_equal(got[0], base[0]), v
         assert np.array_equal(got[1], base[1]) and np.array_equal(got[2], base[2]), v
+
+
+def test_phases_change_the_steps_not_the_schedule(umx):
+    """The number of phases decides which positions a step reads, so the
+    embeddings differ; the schedule and tallies do not depend on it."""
+    g = _c1_graph()
+    runs = [_run(umx, g, phases=p) for p in (1, 2, 4, 8)]
+    for r in runs[1:]:
+        assert not np.array_equal(r[0], runs[0][0])
+        assert np.array_equal(r[1], runs[0][1]) and np.array_equal(r[2], runs[0][2])
 
 
 def test_split_rows_are_folded(umx):

@@ -13,8 +13,9 @@
   epoch_of_next_negative_sample and the per-epoch tallies bit for bit against
   the oracle's counter-sampler run;
 * every step among the first 2,000,000 trace records of epoch 10 equal to
-  the reference's update from its traced inputs within 1e-5 relative, reading
-  other points as they were at the epoch's start.
+  the reference's update from its traced inputs within 1e-5 relative, head
+  and negative-sample steps reading other points of earlier phases as the
+  epoch left them and all others as they were at its start.
 
 64 instances (sizes log-uniform in [2,000, 200,000], seed 4; each 30 dims
 float32 from its own 10-component mixture, seed 4000 + i; kNN k = 15;
@@ -33,6 +34,7 @@ import pytest
 
 import oracle
 from _data import mixture
+from _steps import phase_of
 
 pytestmark = pytest.mark.gpu
 
@@ -113,10 +115,15 @@ def test_headline_1m_graph_schedule_and_steps(umx):
     lay.set_trace(0)
     assert tr.shape[0] == cap
     kinds = tr[:, 0].astype(np.int64)
+    heads = tr[:, 1].astype(np.int64)
     others = tr[:, 2].astype(np.int64)
-    assert np.array_equal(tr[:, 5:7].astype(np.float32), before[others])
-    alpha = oracle.epoch_alpha(1.0, 10, T, np.float32)
     fwd = kinds != 2
+    P, _ = lay.phases()
+    after = lay.get_embedding()
+    fresh = phase_of(others, P) < phase_of(heads, P)
+    read = np.where(fresh[:, None], after[others], before[others])
+    assert np.array_equal(tr[fwd, 5:7].astype(np.float32), read[fwd])
+    alpha = oracle.epoch_alpha(1.0, 10, T, np.float32)
     h = tr[fwd, 3:5].astype(np.float32)
     o = tr[fwd, 5:7].astype(np.float32)
     ref = oracle.step_delta_many(kinds[fwd].astype(np.int32), h, o, a, b, 1.0, alpha, np.float32).astype(np.float64)

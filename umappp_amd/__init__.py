@@ -89,6 +89,9 @@ def _declare(L):
         "umx_layout_tallies": (C.c_int, [_vp, _vp, C.c_int64]),
         "umx_layout_set_mode": (C.c_int, [_vp, C.c_int]),
         "umx_layout_get_mode": (C.c_int, [_vp, C.c_void_p]),
+        "umx_layout_launch_shape": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+        "umx_layout_set_phases": (C.c_int, [_vp, C.c_int]),
+        "umx_layout_get_phases": (C.c_int, [_vp, _vp, _vp]),
         "umx_layout_set_work_split": (C.c_int, [_vp, C.c_double, C.c_int]),
         "umx_layout_set_tile": (C.c_int, [_vp, C.c_int]),
         "umx_layout_work_items": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int64, _i64p]),
@@ -462,21 +465,38 @@ class Layout:
         push mode only."""
         _check(lib().umx_layout_set_mode(self._h, int(mode)))
 
+    def launch_shape(self):
+        """(tile columns, wavefronts per workgroup, placement groups, chunks,
+        slots) of the pull kernel's next run."""
+        t, w, g = C.c_int(), C.c_int(), C.c_int()
+        c, sl = C.c_int64(), C.c_int64()
+        _check(lib().umx_layout_launch_shape(self._h, C.byref(t), C.byref(w), C.byref(g), C.byref(c),
+                                             C.byref(sl)))
+        return t.value, w.value, g.value, c.value, sl.value
+
     def mode(self) -> int:
         m = C.c_int()
         _check(lib().umx_layout_get_mode(self._h, C.byref(m)))
         return m.value
 
-    def set_work_split(self, fires_per_item: float = 32.0, edges_per_item: int = 2**31 - 1):
+    def set_work_split(self, fires_per_item: float = 32.0, edges_per_item: int = 48):
         _check(lib().umx_layout_set_work_split(self._h, float(fires_per_item), int(edges_per_item)))
+        self._split = (float(fires_per_item), int(edges_per_item))
+
+    def work_split(self):
+        """(fires_per_item, edges_per_item) last set on this layout (the
+        library's defaults otherwise)."""
+        return getattr(self, "_split", (32.0, 48))
 
     def set_tile(self, tile_cols: int = 0):
         """LDS tile of the pull kernel: 4, 8, 16 or 32 columns, or 0 to choose
         from the number of chunks (the default)."""
         _check(lib().umx_layout_set_tile(self._h, int(tile_cols)))
 
-    def work_items(self):
-        """(begin, length, row, split) arrays of the production kernels' work items, in run order."""
+    def work_items(self, with_phase: bool = False):
+        """(begin, length, row, split) arrays of the production kernels' work
+        items in run order, padding left out; with ``with_phase`` also each
+        item's phase."""
         cnt = C.c_int64()
         _check(lib().umx_layout_work_items(self._h, None, None, None, 0, C.byref(cnt)))
         m = cnt.value
@@ -485,7 +505,23 @@ class Layout:
         r = np.empty(m, np.int32)
         _check(lib().umx_layout_work_items(self._h, _ptr(b), _ptr(ln), _ptr(r), m, C.byref(cnt)))
         ru = r.view(np.uint32)
-        return b, ln, (ru & 0x7FFFFFFF).astype(np.int64), (ru >> 31).astype(bool)
+        P, off = self.phases()
+        ph = np.repeat(np.arange(P), np.diff(off))
+        real = ln > 0
+        out = (b[real], ln[real], (ru[real] & 0x7FFFFFFF).astype(np.int64), (ru[real] >> 31).astype(bool))
+        return out + (ph[real],) if with_phase else out
+
+    def set_phases(self, phases: int):
+        """Phases of a pull-mode epoch: 1, 2, 4 or 8 (default 4)."""
+        _check(lib().umx_layout_set_phases(self._h, int(phases)))
+
+    def phases(self):
+        """(number of phases, work-item offsets of the phases)."""
+        p = C.c_int()
+        _check(lib().umx_layout_get_phases(self._h, C.byref(p), None))
+        off = np.zeros(p.value + 1, np.int64)
+        _check(lib().umx_layout_get_phases(self._h, C.byref(p), _ptr(off)))
+        return p.value, off
 
     def set_trace(self, max_records: int):
         _check(lib().umx_layout_set_trace(self._h, int(max_records)))

@@ -18,6 +18,9 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "libumappp_amd.so"
+# floor probes of the layout epoch (used by bench.py only, not part of the library)
+PROBES_SRC = "probes.hip"
+PROBES_LIB = PKG / "libumx_probes.so"
 TORCH_DIR = PKG / "_torch_build"
 TORCH_NAME = "umappp_amd_torch"
 
@@ -66,9 +69,12 @@ def build(verbose: bool = False) -> Path:
         objs.append(obj)
         cmds.append([CXX, "-O2", "-fPIC", "-std=c++17", "-Wall", "-Wextra", f"-I{ROOT / 'include'}",
                      "-c", str(CSRC / src), "-o", str(obj)])
+    probes_tmp = PROBES_LIB.with_suffix(".so.tmp")
+    cmds.append([HIPCC, *FLAGS, "-shared", str(CSRC / PROBES_SRC), "-o", str(probes_tmp)])
     with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
         for fut in [ex.submit(_run, c) for c in cmds]:
             fut.result()
+    os.replace(probes_tmp, PROBES_LIB)
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-soname,libumappp_amd.so", "-o", str(tmp),
           *map(str, objs)])

@@ -690,12 +690,17 @@ struct umx_layout {
     DevBuf<double> split_dd;
     int64_t n_items = -1;        // -1: not built yet
     int split_dim = -1;          // num_dim split_delta was sized for (-1: not sized)
-    // Only heavy rows are split: a row's steps should see each other
-    // sequentially (items of a split row start from one snapshot of the head
-    // and their summed moves overshoot), but one very heavy row would
-    // otherwise set the length of the whole epoch.
+    // phases of the pull kernels' epochs (a power of two, 1 to 8): phase p's
+    // items are [phase_items[p], phase_items[p + 1]) (whole workgroups), its
+    // split rows [phase_split[p], phase_split[p + 1])
+    int phases = 4;
+    std::vector<int64_t> phase_items, phase_split;
+    // Only heavy and long rows are split: a row's steps should see each
+    // other sequentially (items of a split row start from one snapshot of
+    // the head and their summed moves overshoot), but every phase of an epoch
+    // waits for its longest item.
     double fires_per_item = 32.0;
-    int edges_per_item = 0x7fffffff;
+    int edges_per_item = 48;
     double a = 0, b = 0, gamma = 1, alpha0 = 1;
     uint64_t seed = 0;
     bool params = false, have_emb = false;
@@ -797,6 +802,8 @@ umx::LayoutView<T> view_of(umx_layout* L, DevBuf<T>& eps, DevBuf<T>& nxt, DevBuf
     v.nxt_neg = nng.p;
     v.emb = emb.p;
     v.emb_alt = emb_alt.p;
+    v.phases = 1;
+    v.phase = 0;
     v.dim = L->dim;
     v.mode = L->mode;
     v.a = static_cast<T>(L->a);
@@ -924,11 +931,101 @@ void build_items_t(umx_layout* L, const T* eps) {
         UMX_HIP(hipMemcpyAsync(L->group_of_inst.p, gof.data(), sizeof(int32_t) * ni, hipMemcpyHostToDevice, L->s));
         pl.group_of_inst = L->group_of_inst.p;
     }
-    umx::build_work_items<T>(L->n, L->rp.p, eps, L->fires_per_item, L->edges_per_item, L->instances(), pl,
-                             L->item_begin, L->item_len, L->item_row, L->item_inst, L->item_aux, L->split,
-                             L->n_items, L->s);
+    const int P = L->phases;
+    L->phase_items.assign(static_cast<size_t>(P) + 1, 0);
+    L->phase_split.assign(static_cast<size_t>(P) + 1, 0);
+    if (P == 1) {
+        umx::build_work_items<T>(L->n, L->rp.p, eps, L->fires_per_item, L->edges_per_item, L->instances(), pl, 1, 0,
+                                 L->item_begin, L->item_len, L->item_row, L->item_inst, L->item_aux, L->split,
+                                 L->n_items, L->s);
+        L->phase_items[1] = L->n_items;
+        L->phase_split[1] = L->split.rows;
+    } else {
+        // each phase's items (whole workgroups) one after the other; split
+        // slots and split rows numbered across the phases
+        struct Part {
+            DevBuf<int64_t> beg;
+            DevBuf<int32_t> len, row, inst, aux;
+            umx::SplitPlan sp;
+            int64_t n = 0;
+        };
+        std::vector<std::unique_ptr<Part>> parts;
+        int64_t items = 0, rows = 0, slots = 0;
+        for (int p = 0; p < P; ++p) {
+            parts.emplace_back(new Part());
+            Part& q = *parts.back();
+            umx::build_work_items<T>(L->n, L->rp.p, eps, L->fires_per_item, L->edges_per_item, L->instances(), pl, P,
+                                     p, q.beg, q.len, q.row, q.inst, q.aux, q.sp, q.n, L->s);
+            items += q.n;
+            rows += q.sp.rows;
+            slots += q.sp.slots;
+            L->phase_items[p + 1] = items;
+            L->phase_split[p + 1] = rows;
+        }
+        if (items > 0x7fffffffLL || slots > 0x7fffffffLL) throw umx::Error(UMX_ERR_OVERFLOW, "too many work items");
+        const size_t M = static_cast<size_t>(items > 0 ? items : 1);
+        L->item_begin.alloc(M);
+        L->item_len.alloc(M);
+        L->item_row.alloc(M);
+        L->item_inst.alloc(M);
+        L->item_aux.alloc(M);
+        L->split.row.alloc(static_cast<size_t>(rows));
+        L->split.first.alloc(static_cast<size_t>(rows));
+        L->split.count.alloc(static_cast<size_t>(rows));
+        int64_t slot_base = 0;
+        for (int p = 0; p < P; ++p) {
+            Part& q = *parts[static_cast<size_t>(p)];
+            const int64_t i0 = L->phase_items[p], r0 = L->phase_split[p];
+            auto cp = [&](void* to, const void* from, size_t bytes) {
+                if (bytes) UMX_HIP(hipMemcpyAsync(to, from, bytes, hipMemcpyDeviceToDevice, L->s));
+            };
+            const size_t qn = static_cast<size_t>(q.n), qr = static_cast<size_t>(q.sp.rows);
+            cp(L->item_begin.p + i0, q.beg.p, sizeof(int64_t) * qn);
+            cp(L->item_len.p + i0, q.len.p, sizeof(int32_t) * qn);
+            cp(L->item_row.p + i0, q.row.p, sizeof(int32_t) * qn);
+            cp(L->item_inst.p + i0, q.inst.p, sizeof(int32_t) * qn);
+            cp(L->item_aux.p + i0, q.aux.p, sizeof(int32_t) * qn);
+            cp(L->split.row.p + r0, q.sp.row.p, sizeof(int32_t) * qr);
+            cp(L->split.first.p + r0, q.sp.first.p, sizeof(int64_t) * qr);
+            cp(L->split.count.p + r0, q.sp.count.p, sizeof(int32_t) * qr);
+            umx::offset_nonnegative(L->item_aux.p + i0, q.n, slot_base, L->s);
+            umx::offset_all(L->split.first.p + r0, q.sp.rows, slot_base, L->s);
+            slot_base += q.sp.slots;
+        }
+        L->split.rows = rows;
+        L->split.slots = slots;
+        L->n_items = items;
+        UMX_HIP(hipStreamSynchronize(L->s));
+    }
     L->groups = groups;
     L->split_dim = -1;  // split_delta is sized by ensure_split_delta
+}
+
+// The view of phase p's items, chunks and split rows (phases > 1), or the
+// whole view (one phase).
+template <typename T>
+umx::LayoutView<T> phase_view(const umx_layout* L, const umx::LayoutView<T>& v, int p) {
+    umx::LayoutView<T> w = v;
+    w.phases = L->phases;
+    w.phase = p;
+    const int64_t i0 = L->phase_items[static_cast<size_t>(p)], i1 = L->phase_items[static_cast<size_t>(p) + 1];
+    const int64_t s0 = L->phase_split[static_cast<size_t>(p)], s1 = L->phase_split[static_cast<size_t>(p) + 1];
+    w.item_begin += i0;
+    w.item_len += i0;
+    w.item_row += i0;
+    w.item_inst += i0;
+    w.item_aux += i0;
+    w.n_items = i1 - i0;
+    if (w.chunk_width) {
+        w.chunk_width += i0 / umx::kWave;
+        w.chunk_base += i0 / umx::kWave;
+        w.n_chunks = (i1 - i0 + umx::kWave - 1) / umx::kWave;
+    }
+    w.split_row += s0;
+    w.split_first += s0;
+    w.split_count += s0;
+    w.n_split_rows = s1 - s0;
+    return w;
 }
 
 // Room for the moves of the pieces of split rows: slots x num_dim values.
@@ -1172,16 +1269,21 @@ int effective_tile(const umx_layout* L) {
 void launch_sell_once(umx_layout* L, int epoch, hipEvent_t start = nullptr, hipEvent_t stop = nullptr,
                       bool fold = true) {
     if (start) UMX_HIP(hipEventRecord(start, L->s));
-    if (L->dtype == UMX_F32) {
-        auto v = pull_view_f(L);
-        attach_sell(v, L->sell_f);
-        umx::launch_layout_sell<float>(v, epoch, effective_tile(L), L->waves, L->s);
-        umx::launch_fold_split<float>(v, epoch, L->s);
-    } else {
-        auto v = pull_view_d(L);
-        attach_sell(v, L->sell_d);
-        umx::launch_layout_sell<double>(v, epoch, effective_tile(L), L->waves, L->s);
-        umx::launch_fold_split<double>(v, epoch, L->s);
+    const int tile = effective_tile(L);
+    for (int p = 0; p < L->phases; ++p) {
+        if (L->dtype == UMX_F32) {
+            auto v = pull_view_f(L);
+            attach_sell(v, L->sell_f);
+            const auto w = phase_view<float>(L, v, p);
+            umx::launch_layout_sell<float>(w, epoch, tile, L->waves, L->s);
+            umx::launch_fold_split<float>(w, epoch, L->s);
+        } else {
+            auto v = pull_view_d(L);
+            attach_sell(v, L->sell_d);
+            const auto w = phase_view<double>(L, v, p);
+            umx::launch_layout_sell<double>(w, epoch, tile, L->waves, L->s);
+            umx::launch_fold_split<double>(w, epoch, L->s);
+        }
     }
     if (stop) UMX_HIP(hipEventRecord(stop, L->s));
     if (fold) umx::finish_layout_epoch(L->tally_part.p, L->tally.p, epoch, L->epoch_ctr.p, L->s);
@@ -1220,7 +1322,7 @@ std::vector<unsigned char> launch_key(umx_layout* L, int block) {
         attach_sell(v, L->sell_d);
         put(&v, sizeof(v));
     }
-    const int extra[3] = {effective_tile(L), L->waves, block};
+    const int extra[4] = {effective_tile(L), L->waves, block, L->phases};
     put(extra, sizeof(extra));
     return k;
 }
@@ -1460,12 +1562,20 @@ int umx_layout_run(umx_layout* L, int epoch_limit) {
                 if (L->timing) UMX_HIP(hipEventRecord(L->events[2 * q], L->s));
                 if (L->dtype == UMX_F32) {
                     ea.alpha = epoch_alpha<float>(L->alpha0, ep, L->total_epochs);
-                    umx::launch_layout_epoch<float>(push ? in_place_view<float>(L, L->ef, L->nf, L->gf, L->mf, L->mf2)
-                                                         : pull_view_f(L), ea, L->s);
+                    if (push)
+                        umx::launch_layout_epoch<float>(in_place_view<float>(L, L->ef, L->nf, L->gf, L->mf, L->mf2), ea,
+                                                        L->s);
+                    else
+                        for (int p = 0; p < L->phases; ++p)
+                            umx::launch_layout_epoch<float>(phase_view<float>(L, pull_view_f(L), p), ea, L->s);
                 } else {
                     ea.alpha = epoch_alpha<double>(L->alpha0, ep, L->total_epochs);
-                    umx::launch_layout_epoch<double>(push ? in_place_view<double>(L, L->ed, L->nd, L->gd, L->md, L->md2)
-                                                          : pull_view_d(L), ea, L->s);
+                    if (push)
+                        umx::launch_layout_epoch<double>(in_place_view<double>(L, L->ed, L->nd, L->gd, L->md, L->md2),
+                                                         ea, L->s);
+                    else
+                        for (int p = 0; p < L->phases; ++p)
+                            umx::launch_layout_epoch<double>(phase_view<double>(L, pull_view_d(L), p), ea, L->s);
                 }
                 if (L->timing) UMX_HIP(hipEventRecord(L->events[2 * q + 1], L->s));
                 umx::finish_layout_epoch(L->tally_part.p, L->tally.p, ep, L->epoch_ctr.p, L->s);
@@ -1575,6 +1685,43 @@ int umx_layout_set_mode(umx_layout* L, int mode) {
         require(mode == UMX_LAYOUT_PUSH || !L->one_way, UMX_ERR_STATE,
                 "pull mode needs every entry's reverse with the same weight; this graph has one-way entries");
         L->mode = mode;
+    });
+}
+
+int umx_layout_launch_shape(umx_layout* L, int* tile_cols, int* waves, int* groups, int64_t* chunks,
+                            int64_t* slots) {
+    return guard([&] {
+        require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
+        ensure_sell_current(L);
+        const bool f = L->dtype == UMX_F32;
+        if (tile_cols) *tile_cols = effective_tile(L);
+        if (waves) *waves = L->waves;
+        if (groups) *groups = L->groups;
+        if (chunks) *chunks = f ? L->sell_f.n_chunks : L->sell_d.n_chunks;
+        if (slots) *slots = f ? L->sell_f.n_slots : L->sell_d.n_slots;
+    });
+}
+
+int umx_layout_set_phases(umx_layout* L, int phases) {
+    return guard([&] {
+        require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
+        require(phases >= 1 && phases <= umx::kMaxPhases && (phases & (phases - 1)) == 0, UMX_ERR_ARGUMENT,
+                "phases must be 1, 2, 4 or 8");
+        if (phases != L->phases) {
+            invalidate_items(L);  // rebuilt at the next run
+            L->phases = phases;
+        }
+    });
+}
+
+int umx_layout_get_phases(umx_layout* L, int* phases, int64_t* item_offsets) {
+    return guard([&] {
+        require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
+        if (phases) *phases = L->phases;
+        if (item_offsets) {
+            ensure_items(L);
+            for (int p = 0; p <= L->phases; ++p) item_offsets[p] = L->phase_items[static_cast<size_t>(p)];
+        }
     });
 }
 

@@ -270,15 +270,21 @@ __global__ void k_layout_serial(LayoutView<T> v, EpochConst ec, T ep, T alpha, c
 // kernel covers one to four): the owner lane of observation i applies every
 // move of i.  A stored edge i->t fires in exactly the same epochs as its
 // reverse t->i (the fuzzy union makes weights, hence epochs_per_sample,
-// bit-symmetric), so for each firing edge the lane applies i's head step of
-// i->t, then the tail step that t->i gives i (the negative of t's head move,
-// -alpha*clip(c*(y_t - y_i)) = alpha*clip(c*(y_i - y_t))), then i's negative
-// samples, in the reference's order.  Double-buffered: every other point is
-// read from the epoch's input buffer, and i's working copy is its slot in the
-// output buffer (a whole row) or its piece's slot in split_delta (a piece of
-// a split row, turned into the piece's move at the end and summed onto the
-// head by launch_fold_split).  Nothing is shared, so the epoch is
-// deterministic.
+// bit-symmetric), so for each firing edge the lane applies i's head step g of
+// i->t, then the tail step that t->i gives i, then i's negative samples, in
+// the reference's order.  The tail step is the negative of t's head move of
+// t->i, -alpha*clip(c*(y_t' - y_i')) = alpha*clip(c*(y_i' - y_t')), taken at
+// the positions the pair's first event leaves, y_i' = y_i + g and
+// y_t' = y_t - g: in the reference the event i->t moves the tail by -g, and
+// the reverse event sees that.
+//
+// Double-buffered, in phases (see phase_of): a point of an earlier phase is
+// read from the epoch's output buffer (this epoch's position), any other
+// from its input buffer.  i's working copy is its slot in the output buffer
+// (a whole row) or its piece's slot in split_delta (a piece of a split row,
+// turned into the piece's move at the end and summed onto the head by
+// launch_fold_split).  No lane reads what another lane of the launch writes,
+// so the epoch is deterministic.  Plain layouts only (local index = index).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_layout_pull_generic(LayoutView<T> v, EpochConst ec, T ep, T alpha) {
     const int64_t m = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
@@ -288,6 +294,9 @@ __global__ void __launch_bounds__(kBlock) k_layout_pull_generic(LayoutView<T> v,
         const int D = v.dim;
         const T* src = (ec.epoch & 1) ? v.emb_alt : v.emb;
         T* dst = (ec.epoch & 1) ? v.emb : v.emb_alt;
+        auto at = [&](int64_t j) -> const T* {
+            return (phase_of(static_cast<uint64_t>(j), v.phases) < v.phase ? dst : src) + j * D;
+        };
         const int64_t i = static_cast<int64_t>(static_cast<uint32_t>(v.item_row[m]) & 0x7fffffffu);
         const int64_t s = v.item_begin[m], e = s + v.item_len[m];
         T* h = aux >= 0 ? v.split_delta + static_cast<int64_t>(aux) * D : dst + i * D;
@@ -297,17 +306,31 @@ __global__ void __launch_bounds__(kBlock) k_layout_pull_generic(LayoutView<T> v,
             const T nx = v.nxt[j];
             if (nx > ep) continue;
             ++fires;
-            const T* r = src + static_cast<int64_t>(v.tg[j]) * D;
-            for (int half = 0; half < 2; ++half) {
-                T d2 = 0;
-                for (int d = 0; d < D; ++d) {
-                    const T q = h[d] - r[d];
-                    d2 += q * q;
-                }
-                d2 = d2 > eps_of<T>() ? d2 : eps_of<T>();
-                const T pd2b = fast_pow(d2, v.b);
-                const T gc = (-2 * v.a * v.b * pd2b) / (d2 * (v.a * pd2b + 1));
-                for (int d = 0; d < D; ++d) h[d] += alpha * clip4(gc * (h[d] - r[d]));
+            const T* r = at(v.tg[j]);
+            // head step g (coefficient c1), then the reverse edge's step at
+            // (y_i + g, y_t - g) (coefficient c2); g is recomputed per
+            // coordinate rather than stored, so any num_dim works
+            T d2 = 0;
+            for (int d = 0; d < D; ++d) {
+                const T q = h[d] - r[d];
+                d2 += q * q;
+            }
+            d2 = d2 > eps_of<T>() ? d2 : eps_of<T>();
+            T pd2b = fast_pow(d2, v.b);
+            const T c1 = (-2 * v.a * v.b * pd2b) / (d2 * (v.a * pd2b + 1));
+            T e2 = 0;
+            for (int d = 0; d < D; ++d) {
+                const T g = alpha * clip4(c1 * (h[d] - r[d]));
+                const T q = (h[d] + g) - (r[d] - g);
+                e2 += q * q;
+            }
+            e2 = e2 > eps_of<T>() ? e2 : eps_of<T>();
+            pd2b = fast_pow(e2, v.b);
+            const T c2 = (-2 * v.a * v.b * pd2b) / (e2 * (v.a * pd2b + 1));
+            for (int d = 0; d < D; ++d) {
+                const T g = alpha * clip4(c1 * (h[d] - r[d]));
+                const T h1 = h[d] + g, r1 = r[d] - g;
+                h[d] = h1 + alpha * clip4(c2 * (h1 - r1));
             }
             const T epns = v.eps[j] / v.nsr;
             const T nn = v.nxt_neg[j];
@@ -320,14 +343,14 @@ __global__ void __launch_bounds__(kBlock) k_layout_pull_generic(LayoutView<T> v,
                     ++self_hits;
                     continue;
                 }
-                const T* q = src + static_cast<int64_t>(k) * D;
-                T e2 = 0;
+                const T* q = at(k);
+                T f2 = 0;
                 for (int d = 0; d < D; ++d) {
                     const T z = h[d] - q[d];
-                    e2 += z * z;
+                    f2 += z * z;
                 }
-                e2 = e2 > eps_of<T>() ? e2 : eps_of<T>();
-                const T rc = 2 * v.gamma * v.b / ((static_cast<T>(0.001) + e2) * (v.a * fast_pow(e2, v.b) + 1));
+                f2 = f2 > eps_of<T>() ? f2 : eps_of<T>();
+                const T rc = 2 * v.gamma * v.b / ((static_cast<T>(0.001) + f2) * (v.a * fast_pow(f2, v.b) + 1));
                 for (int d = 0; d < D; ++d) h[d] += alpha * clip4(rc * (h[d] - q[d]));
             }
             v.nxt[j] = nx + v.eps[j];
@@ -462,13 +485,26 @@ __device__ inline int64_t split_row(const int64_t* __restrict__ rp, const T* __r
     return items + 1;
 }
 
+// Whether row i belongs to phase 'phase' (its index within its instance
+// decides).
+__device__ inline bool in_phase(int64_t i, const int64_t* __restrict__ pt_off, const int32_t* __restrict__ of_row,
+                                int phases, int phase) {
+    if (phases <= 1) return true;
+    const int64_t local = i - (of_row ? pt_off[of_row[i]] : pt_off[0]);
+    return phase_of(static_cast<uint64_t>(local), phases) == phase;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_count_items(int64_t n, const int64_t* __restrict__ rp,
                                                       const T* __restrict__ eps, double fires_split, int edges_cap,
+                                                      const int64_t* __restrict__ pt_off,
+                                                      const int32_t* __restrict__ of_row, int phases, int phase,
                                                       int64_t* __restrict__ count) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;
-    count[i] = split_row<T, false>(rp, eps, i, fires_split, edges_cap, 0, nullptr, nullptr, nullptr, false);
+    count[i] = in_phase(i, pt_off, of_row, phases, phase)
+                   ? split_row<T, false>(rp, eps, i, fires_split, edges_cap, 0, nullptr, nullptr, nullptr, false)
+                   : 0;
 }
 
 template <typename T>
@@ -709,9 +745,10 @@ void launch_layout_serial(const LayoutView<T>& v, const EpochArgs& ea, const int
 
 template <typename T>
 void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_split, int edges_cap,
-                      const Instances& inst, const ItemPlacement& place, DevBuf<int64_t>& item_begin,
-                      DevBuf<int32_t>& item_len, DevBuf<int32_t>& item_row, DevBuf<int32_t>& item_inst,
-                      DevBuf<int32_t>& item_aux, SplitPlan& split, int64_t& n_items, hipStream_t s) {
+                      const Instances& inst, const ItemPlacement& place, int phases, int phase,
+                      DevBuf<int64_t>& item_begin, DevBuf<int32_t>& item_len, DevBuf<int32_t>& item_row,
+                      DevBuf<int32_t>& item_inst, DevBuf<int32_t>& item_aux, SplitPlan& split, int64_t& n_items,
+                      hipStream_t s) {
     if (!(fires_split > 0) || edges_cap < 1) throw Error(-1, "work split parameters must be positive");
     if (place.waves < 1 || place.waves > 4 || place.n_groups < 1 || place.n_groups > 32)
         throw Error(-1, "placement: waves must be 1..4 and groups 1..32");
@@ -721,7 +758,11 @@ void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_s
     count.alloc(static_cast<size_t>(n > 0 ? n : 1));
     offs.alloc(static_cast<size_t>(n + 1));
     const unsigned grid = static_cast<unsigned>((n + kBlock - 1) / kBlock > 0 ? (n + kBlock - 1) / kBlock : 1);
-    if (n > 0) k_count_items<T><<<grid, kBlock, 0, s>>>(n, rp, eps, fires_split, edges_cap, count.p);
+    if (phases < 1 || phases > kMaxPhases || (phases & (phases - 1)) != 0 || phase < 0 || phase >= phases)
+        throw Error(-1, "phases must be a power of two in [1, 8] and phase in [0, phases)");
+    if (n > 0)
+        k_count_items<T><<<grid, kBlock, 0, s>>>(n, rp, eps, fires_split, edges_cap, inst.pt_off,
+                                                 inst.n_inst > 1 ? inst.of_row : nullptr, phases, phase, count.p);
     UMX_HIP(hipGetLastError());
     DevBuf<int64_t> unused;
     exclusive_scan_i64(count.p, offs.p, n, unused, s);
@@ -778,21 +819,22 @@ void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_s
     hist.alloc(static_cast<size_t>(n_keys * (ntiles > 0 ? ntiles : 1)));
     hoff.alloc(static_cast<size_t>(n_keys * ntiles + 1));
     // Final item count: each group padded to whole workgroups of
-    // waves * 64 items, every group given as many workgroups as the largest.
-    int64_t padded = real;
+    // waves * 64 items, every group given as many workgroups as the largest
+    // (one group: the items padded to whole workgroups).
+    const int64_t per_blk = static_cast<int64_t>(place.waves) * kWave;
+    int64_t padded = (real + per_blk - 1) / per_blk * per_blk;
     if (real > 0) {
         k_item_key_hist<<<static_cast<unsigned>(ntiles), kBlock, sizeof(int) * n_keys, s>>>(
             il.p, ir.p, real, of_row, goi, n_keys, ntiles, hist.p);
         UMX_HIP(hipGetLastError());
         exclusive_scan_i64(hist.p, hoff.p, n_keys * ntiles, unused, s);
-        if (G > 1 || place.waves > 1) {
+        if (G > 1) {
             std::vector<int64_t> starts(static_cast<size_t>(G) + 1);
             for (int g = 0; g < G; ++g)
                 UMX_HIP(hipMemcpyAsync(&starts[g], hoff.p + static_cast<int64_t>(g * kLevels) * ntiles,
                                        sizeof(int64_t), hipMemcpyDeviceToHost, s));
             UMX_HIP(hipStreamSynchronize(s));
             starts[G] = real;
-            const int64_t per_blk = static_cast<int64_t>(place.waves) * kWave;
             int64_t most = 0;
             for (int g = 0; g < G; ++g) most = std::max(most, (starts[g + 1] - starts[g] + per_blk - 1) / per_blk);
             padded = most * G * per_blk;
@@ -824,6 +866,15 @@ void instance_fires(const int64_t* rp, const T* eps, const Instances& inst, doub
     UMX_HIP(hipMemsetAsync(fires, 0, sizeof(double) * static_cast<size_t>(inst.n_inst), s));
     k_instance_fires<T><<<dim3(64, static_cast<unsigned>(inst.n_inst)), kBlock, 0, s>>>(rp, eps, inst.pt_off, fires);
     UMX_HIP(hipGetLastError());
+}
+
+__global__ void __launch_bounds__(kBlock) k_offset_nonneg(int32_t* __restrict__ a, int64_t count, int64_t by) {
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (q < count && a[q] >= 0) a[q] = static_cast<int32_t>(a[q] + by);
+}
+__global__ void __launch_bounds__(kBlock) k_offset_all(int64_t* __restrict__ a, int64_t count, int64_t by) {
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (q < count) a[q] += by;
 }
 
 // Pull mode needs every stored edge i -> t to have a reverse t -> i with the
@@ -866,6 +917,17 @@ bool has_reverse_edges(int64_t n, const int64_t* rp, const int32_t* tg, const T*
     return h == 0;
 }
 
+void offset_nonnegative(int32_t* a, int64_t count, int64_t by, hipStream_t s) {
+    if (count <= 0 || by == 0) return;
+    k_offset_nonneg<<<static_cast<unsigned>((count + kBlock - 1) / kBlock), kBlock, 0, s>>>(a, count, by);
+    UMX_HIP(hipGetLastError());
+}
+void offset_all(int64_t* a, int64_t count, int64_t by, hipStream_t s) {
+    if (count <= 0 || by == 0) return;
+    k_offset_all<<<static_cast<unsigned>((count + kBlock - 1) / kBlock), kBlock, 0, s>>>(a, count, by);
+    UMX_HIP(hipGetLastError());
+}
+
 template <typename T>
 void launch_fold_split(const LayoutView<T>& v, int epoch, hipStream_t s) {
     if (v.n_split_rows <= 0) return;
@@ -881,11 +943,13 @@ void launch_fold_split(const LayoutView<T>& v, int epoch, hipStream_t s) {
 }
 
 template void build_work_items<float>(int64_t, const int64_t*, const float*, double, int, const Instances&,
-                                      const ItemPlacement&, DevBuf<int64_t>&, DevBuf<int32_t>&, DevBuf<int32_t>&,
-                                      DevBuf<int32_t>&, DevBuf<int32_t>&, SplitPlan&, int64_t&, hipStream_t);
+                                      const ItemPlacement&, int, int, DevBuf<int64_t>&, DevBuf<int32_t>&,
+                                      DevBuf<int32_t>&, DevBuf<int32_t>&, DevBuf<int32_t>&, SplitPlan&, int64_t&,
+                                      hipStream_t);
 template void build_work_items<double>(int64_t, const int64_t*, const double*, double, int, const Instances&,
-                                       const ItemPlacement&, DevBuf<int64_t>&, DevBuf<int32_t>&, DevBuf<int32_t>&,
-                                       DevBuf<int32_t>&, DevBuf<int32_t>&, SplitPlan&, int64_t&, hipStream_t);
+                                       const ItemPlacement&, int, int, DevBuf<int64_t>&, DevBuf<int32_t>&,
+                                       DevBuf<int32_t>&, DevBuf<int32_t>&, DevBuf<int32_t>&, SplitPlan&, int64_t&,
+                                       hipStream_t);
 template bool has_reverse_edges<float>(int64_t, const int64_t*, const int32_t*, const float*, hipStream_t);
 template bool has_reverse_edges<double>(int64_t, const int64_t*, const int32_t*, const double*, hipStream_t);
 template void launch_fold_split<float>(const LayoutView<float>&, int, hipStream_t);

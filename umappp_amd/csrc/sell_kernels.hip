@@ -23,16 +23,20 @@
 // schedule state; the host syncs the two copies when it switches between
 // this kernel and the CSR kernels (serial replay, push mode, export).
 //
-// Determinism.  The embedding is double-buffered: epoch e reads every point
-// (its own head, tails, negative samples) from buffer e & 1 and writes its
-// result to the other buffer.  Every item writes, moved or not: a whole row
-// its head's new position, a piece of a split row its move into its own
-// slot, which a fold after the epoch adds to the head in edge order.  No
-// lane reads what another lane writes in the same epoch and nothing is
-// accumulated with atomics, so the result of an epoch does not depend on the
-// order the wavefronts run in, the tile width, the wavefronts per workgroup
-// or the placement: the same inputs give the same embedding bit for bit, and
-// a run that stops at any epoch and restarts equals one run.
+// Determinism.  The embedding is double-buffered: epoch e reads from buffer
+// e & 1 and writes every point's result to the other buffer.  The epoch
+// runs in phases, one launch each (observations dealt to phases by a hash of
+// their index, see phase_of): phase p reads the points of earlier phases
+// from the output buffer, as this epoch left them, and every other point
+// (its own head included) from the input buffer.  Every item writes, moved
+// or not: a whole row its head's new position, a piece of a split row its
+// move into its own slot, which a fold after the phase adds to the head in
+// edge order.  No lane reads what another lane of its launch writes and
+// nothing is accumulated with atomics, so the result of an epoch does not
+// depend on the order the wavefronts run in, the tile width, the wavefronts
+// per workgroup or the placement: the same inputs and number of phases give
+// the same embedding bit for bit, and a run that stops at any epoch and
+// restarts equals one run.
 //
 // Batches.  The same kernel runs many independent problems (instances) in one
 // launch: each lane reads its item's instance constants (observation offset
@@ -196,9 +200,11 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
     const int64_t c = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + wid;
     const int epn = epoch_arg >= 0 ? epoch_arg : v.epoch_ctr[0] + (-1 - epoch_arg);
     const T ep = static_cast<T>(epn);
-    // double buffer: the epoch reads buffer epn & 1 and writes the other
+    // double buffer: the epoch reads buffer epn & 1 and writes the other;
+    // points of earlier phases are read from the output buffer
     const T* __restrict__ src = (epn & 1) ? v.emb_alt : v.emb;
     T* __restrict__ dst = (epn & 1) ? v.emb : v.emb_alt;
+    const bool phased = v.phases > 1 && v.phase > 0;
 
     uint32_t fires = 0, draws = 0, self_hits = 0;  // one chunk's counts fit in 32 bits
     if (c < v.n_chunks) {
@@ -237,6 +243,10 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
         // draw is the same function of the key, draw number and n either way;
         // a per-lane choice would run both paths under divergence).
         const bool small_n = __ballot(n32 > kSmallDraw) == 0;
+        // where point j (global index, in this lane's instance) is read from
+        auto from = [&](int64_t j) -> const T* {
+            return phased && phase_of(static_cast<uint64_t>(j - pt_off), v.phases) < v.phase ? dst : src;
+        };
         // Draws p0 .. p0 + kNegFetch - 1 of an edge and their points, as one
         // batch of unconditional loads (slots past nneg read the head's own
         // point and are marked -1); conditional loads would each be followed
@@ -249,7 +259,8 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
                     static_cast<int32_t>(pt_off + (small_n ? draw_index_small(ej, p, n32) : draw_index(ej, p, n32)));
                 const bool use = p0 + u < nneg;
                 kk[u] = use ? d : -1;
-                q[u] = load_point<T, D>(src, use ? static_cast<int64_t>(d) : i);
+                const int64_t pt = use ? static_cast<int64_t>(d) : i;
+                q[u] = load_point<T, D>(from(pt), pt);
             }
         };
 
@@ -285,7 +296,10 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
                 const int nneg = static_cast<int>((ep - nnj) / epns);
                 draws += static_cast<uint32_t>(nneg > 0 ? nneg : 0);
                 const uint64_t ej = edge_key(ekey, static_cast<uint64_t>(j));
-                const Vec<T, D> r = load_point<T, D>(src, t);
+                // the reverse edge's step is taken where the pair's first
+                // event leaves the two points: the head step g of i -> t
+                // moves the tail by -g (as in the reference)
+                Vec<T, D> r = load_point<T, D>(from(t), t);
                 int32_t kk[kNegFetch];
                 Vec<T, D> q[kNegFetch];
                 fetch(ej, 0, nneg, kk, q);
@@ -304,6 +318,7 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
                             rec[3 + 2 * D + d] = g;
                         }
                         h.c[d] += g;
+                        if (half == 0) r.c[d] -= g;
                     }
                 }
                 for (int p0 = 0; p0 < nneg; p0 += kNegFetch) {

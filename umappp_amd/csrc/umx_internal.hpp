@@ -192,6 +192,22 @@ struct EpochArgs {
 constexpr int kModePull = 0;
 constexpr int kModePush = 1;
 
+// Phases of a pull-mode epoch.  The observations are dealt to 'phases'
+// groups (a power of two, 1 to 8) by a hash of their index within their
+// instance; phase p's work items run in a launch after phase p - 1's, and
+// read the observations of earlier phases as this epoch left them (the
+// others as they were at its start), so a point sees fresh positions of
+// part of its neighbours, as in the reference's loop, which moves one point
+// after another.  Deterministic for a given number of phases.
+__host__ __device__ inline int phase_of(uint64_t local_index, int phases) {
+    uint64_t z = local_index + 0x632be59bd9b4e019ULL;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    z ^= z >> 31;
+    return static_cast<int>(z >> 32) & (phases - 1);
+}
+constexpr int kMaxPhases = 8;
+
 // Per-instance constants of the production layout kernel.
 struct InstConst {
     int64_t pt_off;    // first observation
@@ -217,6 +233,10 @@ struct LayoutView {
     // its start, whatever order the wavefronts run in.
     T* emb;
     T* emb_alt;
+    // phases of the pull kernels: this launch runs phase 'phase' of
+    // 'phases' (its items, chunks and split rows are the view's)
+    int phases;
+    int phase;
     int dim;
     int mode;
     T a, b, gamma, nsr;
@@ -342,11 +362,21 @@ struct SplitPlan {
     int64_t slots = 0;  // pieces of split rows
 };
 
+//
+// Only the rows of phase 'phase' of 'phases' get items; their count is
+// padded to whole workgroups (waves * 64 items), so that a phase's items
+// are whole chunks and whole workgroups.
 template <typename T>
 void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_split, int edges_cap,
-                      const Instances& inst, const ItemPlacement& place, DevBuf<int64_t>& item_begin,
-                      DevBuf<int32_t>& item_len, DevBuf<int32_t>& item_row, DevBuf<int32_t>& item_inst,
-                      DevBuf<int32_t>& item_aux, SplitPlan& split, int64_t& n_items, hipStream_t s);
+                      const Instances& inst, const ItemPlacement& place, int phases, int phase,
+                      DevBuf<int64_t>& item_begin, DevBuf<int32_t>& item_len, DevBuf<int32_t>& item_row,
+                      DevBuf<int32_t>& item_inst, DevBuf<int32_t>& item_aux, SplitPlan& split, int64_t& n_items,
+                      hipStream_t s);
+
+// Adds 'by' to every non-negative value of a[0..count) (split-piece slots
+// of items built phase by phase).
+void offset_nonnegative(int32_t* a, int64_t count, int64_t by, hipStream_t s);
+void offset_all(int64_t* a, int64_t count, int64_t by, hipStream_t s);
 
 // Whether every entry i -> t of an EpochData CSR has a reverse entry t -> i
 // with the same epochs_per_sample (what pull mode needs); synchronises s.
