@@ -262,7 +262,7 @@ def main():
     traffic = read_traffic(args.config)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": (traffic or {}).get("bytes_per_launch"),
+                "traffic": (traffic or {}).get("bytes_per_epoch"),
                 "traffic_source": None if traffic is None else
                 f"profiles/layout_epoch_{args.config}_traffic.json ({traffic.get('kernel')}, "
                 f"{traffic.get('measured', 'undated')})",
