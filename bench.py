@@ -257,7 +257,7 @@ def main():
     avg_launch_s = (kernel_ms / 1000.0) / max(1, launches)
     achieved = avg_bytes / avg_launch_s / 1e9
     tile, waves, groups, chunks, slots = lay.launch_shape()
-    kernel = f"k_layout_sell<float,2,{tile},false>"
+    kernel = f"k_layout_sell<float,2,{tile},false,true>"
     floor = epoch_floor(umx, torch, dev, n, chunks, slots, float(F.mean()), float(Se.mean()))
     traffic = read_traffic(args.config)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -760,7 +760,7 @@ def main_batched(args, umx, torch, dev):
                    "stored_edges": int(E), "edge_updates": U, "launch_shape_tile_waves_groups": list(shape)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": "k_layout_sell<float,2,*,false> (batched)",
+                     "kernel": "k_layout_sell<float,2,*,false,true> (batched)",
                      "avg_launch_us": round(avg_launch_s * 1e6, 2), "algorithmic_bytes_per_launch": round(avg_bytes)},
         "cpu_baseline": None,
         "sequential_plain_ms": None if seq_ms is None else round(seq_ms, 3),

@@ -29,15 +29,10 @@ def check_steps(trace, counts, dim, dtype, a, b, gamma, T, first_epoch, rtol, at
 
 
 def phase_of(index, phases):
-    """The phase of observations with these (instance-local) indices: the
-    high half of a splitmix64 finaliser of index + 0x632be59bd9b4e019, modulo
-    the number of phases (a power of two)."""
-    with np.errstate(over="ignore"):
-        z = np.asarray(index, dtype=np.uint64) + np.uint64(0x632BE59BD9B4E019)
-        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
-        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
-        z = z ^ (z >> np.uint64(31))
-    return ((z >> np.uint64(32)) & np.uint64(phases - 1)).astype(np.int64)
+    """The phase of observations with these (instance-local) indices: blocks
+    of 16 consecutive indices, block b in phase b modulo the number of phases
+    (a power of two)."""
+    return ((np.asarray(index, dtype=np.int64) >> 4) & (phases - 1)).astype(np.int64)
 
 
 def check_pull_inputs(trace, before, after, dim, dtype, phases):

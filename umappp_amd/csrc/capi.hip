@@ -831,6 +831,9 @@ umx::LayoutView<T> view_of(umx_layout* L, DevBuf<T>& eps, DevBuf<T>& nxt, DevBuf
     v.n_split_rows = L->n_items >= 0 ? L->split.rows : 0;
     v.item_inst = L->item_inst.p;
     v.inst = L->inst_tbl.p;
+    int64_t most = 0;
+    for (size_t q = 0; q + 1 < L->inst_off.size(); ++q) most = std::max(most, L->inst_off[q + 1] - L->inst_off[q]);
+    v.max_inst_n = static_cast<uint32_t>(most);
     v.alpha_tbl = alpha_buf<T>(L).p;
     v.alpha_stride = L->total_epochs;
     v.epoch_ctr = L->epoch_ctr.p;

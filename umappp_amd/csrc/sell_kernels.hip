@@ -63,6 +63,9 @@ using namespace umx::dev;
 // Negative samples fetched per batch: the reference's default rate (5) is
 // one batch for most firing edges; a few draw a sixth in a second batch.
 constexpr int kNegFetch = 5;
+#ifndef UMX_LONG_PRIO
+#define UMX_LONG_PRIO 1
+#endif
 
 // ---------------------------------------------------------------------------
 // Build: chunk widths, chunk offsets, constant slot data and the slot map.
@@ -191,7 +194,7 @@ struct SellTile {
 // or v.epoch_ctr[0] + (-1 - epoch_arg) when epoch_arg < 0 (captured blocks
 // of epochs: launch q of a block passes -1 - q, and a one-wavefront kernel
 // after the block's launches advances the counter by the block length).
-template <typename T, int D, int TW, bool TRACE>
+template <typename T, int D, int TW, bool TRACE, bool SMALL>
 __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_arg) {
     extern __shared__ __align__(16) unsigned char sell_lds[];
     const int lane = threadIdx.x & (kWave - 1);
@@ -204,7 +207,6 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
     // points of earlier phases are read from the output buffer
     const T* __restrict__ src = (epn & 1) ? v.emb_alt : v.emb;
     T* __restrict__ dst = (epn & 1) ? v.emb : v.emb_alt;
-    const bool phased = v.phases > 1 && v.phase > 0;
 
     uint32_t fires = 0, draws = 0, self_hits = 0;  // one chunk's counts fit in 32 bits
     if (c < v.n_chunks) {
@@ -213,6 +215,17 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
         const bool has_item = aux != kAuxPadding;
         bool live = has_item;  // the item's instance runs this epoch
         const int width = v.chunk_width[c];
+#if UMX_LONG_PRIO
+        // Wide chunks (the pieces of heavy rows) are the longest chains of
+        // sequential steps of a launch, and wavefronts of short chunks share
+        // their SIMD: the wider the chunk, the higher its wavefront's issue
+        // priority, so that the launch does not wait for a long chain that ran
+        // at a fraction of the SIMD's rate.  (Priority orders issue among the
+        // wavefronts of a SIMD; it changes no result.)
+        if (width >= 40) __builtin_amdgcn_s_setprio(3);
+        else if (width >= 32) __builtin_amdgcn_s_setprio(2);
+        else if (width >= 24) __builtin_amdgcn_s_setprio(1);
+#endif
         const int64_t base = v.chunk_base[c];
 
         int64_t i = 0, e0 = 0, edge_off = 0, pt_off = 0;
@@ -238,14 +251,12 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
         }
         const T two_ab = -2 * v.a * v.b;
         const T two_gb = 2 * v.gamma * v.b;
-        // The sampler's arithmetic, chosen once per wavefront: the 32-bit
-        // path when every lane's instance has at most 2^24 observations (the
-        // draw is the same function of the key, draw number and n either way;
-        // a per-lane choice would run both paths under divergence).
-        const bool small_n = __ballot(n32 > kSmallDraw) == 0;
-        // where point j (global index, in this lane's instance) is read from
+        // where point j (global index, in this lane's instance) is read from:
+        // points of earlier phases from the output buffer (in the first phase,
+        // and with one phase, the comparison never holds)
         auto from = [&](int64_t j) -> const T* {
-            return phased && phase_of(static_cast<uint64_t>(j - pt_off), v.phases) < v.phase ? dst : src;
+            const uint32_t local = static_cast<uint32_t>(j) - static_cast<uint32_t>(pt_off);
+            return phase_of(local, v.phases) < v.phase ? dst : src;
         };
         // Draws p0 .. p0 + kNegFetch - 1 of an edge and their points, as one
         // batch of unconditional loads (slots past nneg read the head's own
@@ -256,7 +267,7 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
             for (int u = 0; u < kNegFetch; ++u) {
                 const uint32_t p = static_cast<uint32_t>(p0 + u);
                 const int32_t d =
-                    static_cast<int32_t>(pt_off + (small_n ? draw_index_small(ej, p, n32) : draw_index(ej, p, n32)));
+                    static_cast<int32_t>(pt_off + (SMALL ? draw_index_small(ej, p, n32) : draw_index(ej, p, n32)));
                 const bool use = p0 + u < nneg;
                 kk[u] = use ? d : -1;
                 const int64_t pt = use ? static_cast<int64_t>(d) : i;
@@ -296,13 +307,17 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
                 const int nneg = static_cast<int>((ep - nnj) / epns);
                 draws += static_cast<uint32_t>(nneg > 0 ? nneg : 0);
                 const uint64_t ej = edge_key(ekey, static_cast<uint64_t>(j));
-                // the reverse edge's step is taken where the pair's first
-                // event leaves the two points: the head step g of i -> t
-                // moves the tail by -g (as in the reference)
                 Vec<T, D> r = load_point<T, D>(from(t), t);
                 int32_t kk[kNegFetch];
                 Vec<T, D> q[kNegFetch];
                 fetch(ej, 0, nneg, kk, q);
+                // every load of the edge issued before the first step waits
+                // for the tail (the scheduler would otherwise interleave the
+                // later draws' address arithmetic and loads with the steps)
+                __builtin_amdgcn_sched_barrier(0);
+                // the reverse edge's step is taken where the pair's first
+                // event leaves the two points: the head step g of i -> t
+                // moves the tail by -g (as in the reference)
 #pragma unroll
                 for (int half = 0; half < 2; ++half) {
                     const T d2 = sqdist<T, D>(h, r);
@@ -321,18 +336,23 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
                         if (half == 0) r.c[d] -= g;
                     }
                 }
-                for (int p0 = 0; p0 < nneg; p0 += kNegFetch) {
-                    if (p0 > 0) fetch(ej, p0, nneg, kk, q);
-                    // every slot is visited (no early exit), so that every
-                    // load of the batch is waited for within the trip; unused
-                    // slots hold -1, never equal to the row
+                // Negative samples, a batch of kNegFetch at a time.  Every
+                // slot of a batch is evaluated and its step kept by a select
+                // (slots past the edge's count hold -1, never equal to the
+                // row; a draw equal to the row is skipped, as in the
+                // reference).  The first batch is applied in straight-line
+                // code with no branch around its uses, so its loads, issued
+                // with the tail's, stay ahead of the attractive steps, all in
+                // flight together (a use inside a branch or a loop lets the
+                // compiler move the load there, after the steps before it).
+                auto apply_batch = [&]() {
 #pragma unroll
                     for (int u = 0; u < kNegFetch; ++u) {
                         self_hits += kk[u] == i32 ? 1u : 0u;
-                        if (kk[u] < 0 || kk[u] == i32) continue;
+                        const bool apply = kk[u] >= 0 && kk[u] != i32;
                         const T e2 = sqdist<T, D>(h, q[u]);
                         const T rc = prod_div(two_gb, (static_cast<T>(0.001) + e2) * (v.a * fast_pow(e2, v.b) + 1));
-                        double* rec = TRACE ? trace_slot(v, 1, i, kk[u]) : nullptr;
+                        double* rec = TRACE && apply ? trace_slot(v, 1, i, kk[u]) : nullptr;
 #pragma unroll
                         for (int d = 0; d < D; ++d) {
                             const T g = alpha * clip4(rc * (h.c[d] - q[u].c[d]));
@@ -341,9 +361,14 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
                                 rec[3 + D + d] = q[u].c[d];
                                 rec[3 + 2 * D + d] = g;
                             }
-                            h.c[d] += g;
+                            h.c[d] = apply ? h.c[d] + g : h.c[d];
                         }
                     }
+                };
+                apply_batch();
+                for (int p0 = kNegFetch; p0 < nneg; p0 += kNegFetch) {
+                    fetch(ej, p0, nneg, kk, q);
+                    apply_batch();
                 }
                 tile.nx[k][lane] = nxj + epsj;
                 tile.nn[k][lane] = nnj + static_cast<T>(nneg) * epns;
@@ -459,6 +484,16 @@ __global__ void __launch_bounds__(kTallySlots) k_fold_epochs(unsigned long long*
     }
 }
 
+template <typename T, int TW, bool TRACE, bool SMALL>
+void launch_sell_dim(const LayoutView<T>& v, int epoch, unsigned grid, unsigned block, size_t lds, hipStream_t s) {
+    switch (v.dim) {
+        case 1: k_layout_sell<T, 1, TW, TRACE, SMALL><<<grid, block, lds, s>>>(v, epoch); break;
+        case 2: k_layout_sell<T, 2, TW, TRACE, SMALL><<<grid, block, lds, s>>>(v, epoch); break;
+        case 3: k_layout_sell<T, 3, TW, TRACE, SMALL><<<grid, block, lds, s>>>(v, epoch); break;
+        default: k_layout_sell<T, 4, TW, TRACE, SMALL><<<grid, block, lds, s>>>(v, epoch); break;
+    }
+}
+
 template <typename T, int TW>
 void launch_sell_tw(const LayoutView<T>& v, int epoch, int waves, hipStream_t s) {
     const unsigned grid = static_cast<unsigned>((v.n_chunks + waves - 1) / waves);
@@ -466,20 +501,18 @@ void launch_sell_tw(const LayoutView<T>& v, int epoch, int waves, hipStream_t s)
     const size_t lds = sizeof(SellTile<T, TW>) * static_cast<size_t>(waves);
     if (lds > static_cast<size_t>(kSellLdsMax))
         throw Error(-1, "LDS tile times waves per workgroup exceeds 64 KiB; use a smaller tile or fewer waves");
+    // the sampler's arithmetic: 32-bit when every instance has at most 2^24
+    // observations (the draw is the same function of key, draw number and n
+    // either way; the choice is made here, once, so that the kernel carries
+    // one of the two)
+    const bool small = v.max_inst_n <= kSmallDraw;
     if (v.trace) {
-        switch (v.dim) {
-            case 1: k_layout_sell<T, 1, TW, true><<<grid, block, lds, s>>>(v, epoch); break;
-            case 2: k_layout_sell<T, 2, TW, true><<<grid, block, lds, s>>>(v, epoch); break;
-            case 3: k_layout_sell<T, 3, TW, true><<<grid, block, lds, s>>>(v, epoch); break;
-            default: k_layout_sell<T, 4, TW, true><<<grid, block, lds, s>>>(v, epoch); break;
-        }
+        if (!small) throw Error(-1, "the update trace needs instances of at most 2^24 observations");
+        launch_sell_dim<T, TW, true, true>(v, epoch, grid, block, lds, s);
+    } else if (small) {
+        launch_sell_dim<T, TW, false, true>(v, epoch, grid, block, lds, s);
     } else {
-        switch (v.dim) {
-            case 1: k_layout_sell<T, 1, TW, false><<<grid, block, lds, s>>>(v, epoch); break;
-            case 2: k_layout_sell<T, 2, TW, false><<<grid, block, lds, s>>>(v, epoch); break;
-            case 3: k_layout_sell<T, 3, TW, false><<<grid, block, lds, s>>>(v, epoch); break;
-            default: k_layout_sell<T, 4, TW, false><<<grid, block, lds, s>>>(v, epoch); break;
-        }
+        launch_sell_dim<T, TW, false, false>(v, epoch, grid, block, lds, s);
     }
 }
 

@@ -193,18 +193,21 @@ constexpr int kModePull = 0;
 constexpr int kModePush = 1;
 
 // Phases of a pull-mode epoch.  The observations are dealt to 'phases'
-// groups (a power of two, 1 to 8) by a hash of their index within their
-// instance; phase p's work items run in a launch after phase p - 1's, and
-// read the observations of earlier phases as this epoch left them (the
-// others as they were at its start), so a point sees fresh positions of
-// part of its neighbours, as in the reference's loop, which moves one point
-// after another.  Deterministic for a given number of phases.
+// groups (a power of two, 1 to 8) in blocks of 16 consecutive indices within
+// their instance, block b to phase b mod phases; phase p's work items run in
+// a launch after phase p - 1's, and read the observations of earlier phases
+// as this epoch left them (the others as they were at its start), so a point
+// sees fresh positions of part of its neighbours, as in the reference's loop,
+// which moves one point after another.  Deterministic for a given number of
+// phases.  Blocks of 16 points keep each 128-byte line of a two-dimensional
+// float embedding in one phase, so a gather reads a line from one of the two
+// buffers only and the set of lines an epoch reads is one embedding's worth,
+// not two; dealing blocks in turn takes two instructions, which every gather
+// of the epoch kernel evaluates, and spreads every run of indices (a cluster
+// of points stored together, say) evenly over the phases.
+constexpr int kPhaseBlockShift = 4;
 __host__ __device__ inline int phase_of(uint64_t local_index, int phases) {
-    uint64_t z = local_index + 0x632be59bd9b4e019ULL;
-    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-    z ^= z >> 31;
-    return static_cast<int>(z >> 32) & (phases - 1);
+    return static_cast<int>(static_cast<uint32_t>(local_index >> kPhaseBlockShift) & static_cast<uint32_t>(phases - 1));
 }
 constexpr int kMaxPhases = 8;
 
@@ -281,6 +284,7 @@ struct LayoutView {
     // captured block.
     const int32_t* item_inst;
     const InstConst* inst;
+    uint32_t max_inst_n;  // observations of the largest instance (picks the sampler's arithmetic)
     const T* alpha_tbl;
     int alpha_stride;
     int* epoch_ctr;
