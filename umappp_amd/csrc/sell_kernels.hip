@@ -10,23 +10,29 @@
 // is fetched once per epoch, fully coalesced, instead of each lane walking its
 // own CSR row and touching a different cache line per load.
 //
-// Kernel.  A wavefront stages its chunk TW columns at a time in LDS (each
-// lane only reads back its own entries, so no barriers are needed), builds a
-// fire mask per lane, and walks the firing slots with count-trailing-zeros:
-// on every trip each lane applies one whole firing edge (pull mode: head
-// step, reverse-edge tail step, negative samples, exactly as in
-// k_layout_pull_generic), fetching the tail and kNegFetch negative samples
-// in one batch of unconditional independent loads.  Updated schedule values
-// go back to HBM with coalesced stores.  Schedule loads and stores are
-// non-temporal, so the stream of schedule bytes does not evict from the L2
-// the embedding that the gathers read.  The CSR arrays remain the canonical
-// schedule state; the host syncs the two copies when it switches between
-// this kernel and the CSR kernels (serial replay, push mode, export).
+// Kernel.  A wavefront reads its chunk TW columns at a time (coalesced, all
+// loads of a tile in flight together) and advances the schedule of the
+// firing slots at once: the fire test, the number of negative samples and
+// both updated epochs depend on the schedule alone, so they are computed in
+// this pass and stored straight away, and the stores complete while the
+// slots are walked (the vector memory counter retires in order, so stores
+// issued just before the next tile's loads would hold those loads back).
+// Each lane keeps its slots' targets and sample counts in LDS (it reads back
+// only its own entries, so no barriers are needed) and walks its firing
+// slots with count-trailing-zeros: on every trip each lane applies one whole
+// firing edge (pull mode: head step, reverse-edge tail step, negative
+// samples, exactly as in k_layout_pull_generic), fetching the tail and
+// kNegFetch negative samples in one batch of unconditional independent loads.
+// Schedule loads and stores are non-temporal, so the stream of schedule
+// bytes does not evict from the L2 the embedding that the gathers read.  The
+// CSR arrays remain the canonical schedule state; the host syncs the two
+// copies when it switches between this kernel and the CSR kernels (serial
+// replay, push mode, export).
 //
 // Determinism.  The embedding is double-buffered: epoch e reads from buffer
 // e & 1 and writes every point's result to the other buffer.  The epoch
-// runs in phases, one launch each (observations dealt to phases by a hash of
-// their index, see phase_of): phase p reads the points of earlier phases
+// runs in phases, one launch each (observations dealt to phases in blocks of
+// 16 indices, see phase_of): phase p reads the points of earlier phases
 // from the output buffer, as this epoch left them, and every other point
 // (its own head included) from the input buffer.  Every item writes, moved
 // or not: a whole row its head's new position, a piece of a split row its
@@ -181,9 +187,7 @@ __global__ void __launch_bounds__(256) k_sell_to_csr(int64_t E, const int64_t* _
 // ---------------------------------------------------------------------------
 template <typename T, int TW>
 struct SellTile {
-    T nx[TW][kWave];
-    T nn[TW][kWave];
-    T eps[TW][kWave];
+    int32_t nneg[TW][kWave];
     int32_t tg[TW][kWave];
 };
 
@@ -277,22 +281,39 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
 
         for (int t0 = 0; t0 < width; t0 += TW) {
             const int tw = width - t0 < TW ? width - t0 : TW;
-            // coalesced fetch of the tile (all columns in flight together)
+            // coalesced fetch of the tile (all columns in flight together);
+            // the schedule of the firing slots is advanced here and stored at
+            // once (it depends on the schedule only), so the stores complete
+            // while the slots are walked
             unsigned int mask = 0;
+            T xs[TW], ys[TW], es[TW];
+            int32_t ts[TW];
 #pragma unroll
             for (int k = 0; k < TW; ++k) {
                 if (k < tw) {
                     const int64_t slot = base + static_cast<int64_t>(t0 + k) * kWave + lane;
-                    const T x = ld_stream(v.sell_nx + slot);
-                    tile.nx[k][lane] = x;
-                    tile.nn[k][lane] = ld_stream(v.sell_nn + slot);
-                    tile.eps[k][lane] = ld_stream(v.sell_eps + slot);
-                    tile.tg[k][lane] = ld_stream(v.sell_tg + slot);
-                    if (!(x > ep)) mask |= 1u << k;
+                    xs[k] = ld_stream(v.sell_nx + slot);
+                    ys[k] = ld_stream(v.sell_nn + slot);
+                    es[k] = ld_stream(v.sell_eps + slot);
+                    ts[k] = ld_stream(v.sell_tg + slot);
                 }
             }
-            if (!live) mask = 0;
-            const unsigned int fired = mask;
+#pragma unroll
+            for (int k = 0; k < TW; ++k) {
+                if (k < tw) {
+                    const bool fire = live && !(xs[k] > ep);
+                    tile.tg[k][lane] = ts[k];
+                    if (fire) {
+                        const int64_t slot = base + static_cast<int64_t>(t0 + k) * kWave + lane;
+                        const T epns = es[k] / v.nsr;
+                        const int nneg = static_cast<int>((ep - ys[k]) / epns);
+                        tile.nneg[k][lane] = nneg;
+                        st_stream(v.sell_nx + slot, xs[k] + es[k]);
+                        st_stream(v.sell_nn + slot, ys[k] + static_cast<T>(nneg) * epns);
+                        mask |= 1u << k;
+                    }
+                }
+            }
             while (mask) {
                 const int k = __builtin_ctz(mask);
                 mask &= mask - 1;
@@ -300,11 +321,7 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
                 const int64_t j = e0 + t0 + k - edge_off;
                 ++fires;
                 const int32_t t = tile.tg[k][lane];
-                const T nxj = tile.nx[k][lane];
-                const T epsj = tile.eps[k][lane];
-                const T nnj = tile.nn[k][lane];
-                const T epns = epsj / v.nsr;
-                const int nneg = static_cast<int>((ep - nnj) / epns);
+                const int nneg = tile.nneg[k][lane];
                 draws += static_cast<uint32_t>(nneg > 0 ? nneg : 0);
                 const uint64_t ej = edge_key(ekey, static_cast<uint64_t>(j));
                 Vec<T, D> r = load_point<T, D>(from(t), t);
@@ -369,18 +386,6 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
                 for (int p0 = kNegFetch; p0 < nneg; p0 += kNegFetch) {
                     fetch(ej, p0, nneg, kk, q);
                     apply_batch();
-                }
-                tile.nx[k][lane] = nxj + epsj;
-                tile.nn[k][lane] = nnj + static_cast<T>(nneg) * epns;
-            }
-            // write-back of this lane's fired columns (coalesced across the
-            // lanes that fired in the same column)
-#pragma unroll
-            for (int k = 0; k < TW; ++k) {
-                if (k < tw && (fired >> k) & 1u) {
-                    const int64_t slot = base + static_cast<int64_t>(t0 + k) * kWave + lane;
-                    st_stream(v.sell_nx + slot, tile.nx[k][lane]);
-                    st_stream(v.sell_nn + slot, tile.nn[k][lane]);
                 }
             }
         }

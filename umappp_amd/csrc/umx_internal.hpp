@@ -418,10 +418,11 @@ void sync_csr_to_sell(const T* nx, const T* nn, SellState<T>& st, hipStream_t s)
 template <typename T>
 void sync_sell_to_csr(T* nx, T* nn, const SellState<T>& st, hipStream_t s);
 // LDS budget of one workgroup of the sliced-ELL kernel, and what a
-// configuration needs: waves * tile columns * 64 lanes * (3 values + 1 index).
+// configuration needs: waves * tile columns * 64 lanes * (target index +
+// negative-sample count), 4 bytes each whatever the value type.
 constexpr int kSellLdsMax = 65536;
-inline size_t sell_lds_bytes(size_t value_bytes, int tile_cols, int waves) {
-    return static_cast<size_t>(waves) * tile_cols * kWave * (3 * value_bytes + 4);
+inline size_t sell_lds_bytes(size_t /*value_bytes*/, int tile_cols, int waves) {
+    return static_cast<size_t>(waves) * tile_cols * kWave * (4 + 4);
 }
 
 // One epoch of the sliced-ELL kernel.  epoch >= 0: that epoch.  epoch < 0:
