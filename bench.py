@@ -248,29 +248,37 @@ def main():
     value = U / (ms_per_step / 1000.0)
 
     # roofline of the layout epoch kernel: algorithmic bytes per launch over
-    # its HIP-event launch time, beside the floor of what it cannot avoid
+    # its HIP-event launch time (an epoch of P phases is P launches, each a
+    # phase's share of the epoch's bytes), beside the floor of what an epoch
+    # cannot avoid
     D = 2
+    P = lay.phases()[0]
     F = tal[:, 0].astype(np.float64)
     Se = tal[:, 1].astype(np.float64)
     bytes_per_epoch = 4.0 * E + F * 20.0 + F * 2 * 4 * D + F * 2 * 4 * D + Se * 4 * D
-    avg_bytes = float(bytes_per_epoch.mean())
+    avg_bytes = float(bytes_per_epoch.mean()) / P
     avg_launch_s = (kernel_ms / 1000.0) / max(1, launches)
+    epoch_kernel_us = avg_launch_s * P * 1e6
     achieved = avg_bytes / avg_launch_s / 1e9
     tile, waves, groups, chunks, slots = lay.launch_shape()
     kernel = f"k_layout_sell<float,2,{tile},false,true>"
     floor = epoch_floor(umx, torch, dev, n, chunks, slots, float(F.mean()), float(Se.mean()))
     traffic = read_traffic(args.config)
+    traffic_launch = None
+    if traffic is not None and traffic.get("kernel") == kernel and traffic.get("launches_per_epoch") == P:
+        traffic_launch = round(traffic["bytes_per_epoch"] / P)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": (traffic or {}).get("bytes_per_epoch"),
-                "traffic_source": None if traffic is None else
+                "traffic": traffic_launch,
+                "traffic_source": None if traffic_launch is None else
                 f"profiles/layout_epoch_{args.config}_traffic.json ({traffic.get('kernel')}, "
                 f"{traffic.get('measured', 'undated')})",
                 "kernel": kernel, "launch_shape": {"tile": tile, "waves": waves, "groups": groups,
-                                                   "chunks": chunks, "slots": slots, "phases": lay.phases()[0]},
+                                                   "chunks": chunks, "slots": slots, "phases": P},
                 "avg_launch_us": round(avg_launch_s * 1e6, 2),
                 "algorithmic_bytes_per_launch": round(avg_bytes),
-                "floor_us": floor["floor_us"], "floor_frac": round(floor["floor_us"] / (avg_launch_s * 1e6), 4),
+                "launches_per_epoch": P, "epoch_kernel_us": round(epoch_kernel_us, 2),
+                "floor_epoch_us": floor["floor_us"], "floor_frac": round(floor["floor_us"] / epoch_kernel_us, 4),
                 "floor": floor}
 
     # The fast setting (one phase per epoch): throughput and, with the
@@ -322,9 +330,11 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "stages_ms_per_step": {key: round(1000 * v / args.steps, 3) for key, v in stage.items()},
-        "knn_input": {"ms": round(knn_t["knn_ms"], 1), "kernel": "k_knn_qlds<16,32,16> (exact, FP64 distances)",
-                      "fp64_tflops": round(3.0 * n * n * cfg["dim"] / (knn_t["knn_ms"] / 1000.0) / 1e12, 2),
-                      "fp64_peak_tflops": 78.6},
+        "knn_input": {"ms": round(knn_t["knn_ms"], 1),
+                      "kernel": f"k_knn_mfma<{(cfg['dim'] + 3) // 4}> + k_knn_rerank (exact: float32 matrix-core "
+                                f"candidates, FP64 distances)",
+                      "mfma_tflops": round(2.0 * n * n * cfg["dim"] / (knn_t["knn_ms"] / 1000.0) / 1e12, 2),
+                      "fp32_mfma_peak_tflops": 157.3},
         "layout_kernel_ms_per_step": round(kernel_ms / args.steps, 3),
     }
     if cpu and cpu.get("value"):
@@ -731,7 +741,9 @@ def main_batched(args, umx, torch, dev):
     F = tal[:, 0].astype(np.float64)
     Se = tal[:, 1].astype(np.float64)
     D = 2
-    avg_bytes = float((4.0 * E + F * 20.0 + F * 4 * 4 * D + Se * 4 * D).mean())
+    # an epoch of P phases is P launches, each a phase's share of the bytes
+    P = lay_t.phases()[0]
+    avg_bytes = float((4.0 * E + F * 20.0 + F * 4 * 4 * D + Se * 4 * D).mean()) / P
     avg_launch_s = (kernel_ms / 1000.0) / max(1, launches)
     achieved = avg_bytes / avg_launch_s / 1e9
 
@@ -917,18 +929,19 @@ def main_autograd(args, umx, torch, dev):
 
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (FMA counted as two operations)
+FP32_MFMA_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix cores
 
 
 def main_knn(args, umx, torch, dev):
     """The kNN producer (umx_knn_bruteforce) at the headline's input size.
 
-    One step = one exact search of all 1M queries against all 1M points.
-    value = queries / s.  Roofline: FP64 arithmetic, 3 operations per
-    (query, point, dimension) -- a subtract, a multiply and an add, kept
-    separate so that each squared distance is that of a sequential double
-    loop -- against the published FP64 vector peak; without FMA the issue
-    ceiling for this instruction mix is half of it.  The kernel is one launch
-    per search, timed by the wall clock around the blocking call.
+    One step = one exact search of all 1M queries against all 1M points
+    (float input, k <= 16: candidates from float32 matrix-core dot products
+    of centred points, within a proven margin of the exact distances, then
+    exact sequential double distances for the candidates; rows bit-identical
+    to the brute force).  value = queries / s.  Roofline: the candidate
+    kernel's 2 n^2 dim useful flops over the search's wall time against the
+    dense FP32 matrix-core peak.
     cpu_baseline: the oracle's brute force (16 threads) on a bounded range of
     queries; those rows must equal the GPU's bit for bit."""
     cfg = CONFIGS["knn"]
@@ -957,7 +970,7 @@ def main_knn(args, umx, torch, dev):
         times.append(time.perf_counter() - t0)
         log(f"step {s + 1}/{args.steps}: {times[-1]:.3f}s")
     sec = float(np.mean(times))
-    tflops = 3.0 * n * n * dim / sec / 1e12
+    tflops = 2.0 * n * n * dim / sec / 1e12
 
     cpu = None
     if not args.no_cpu:
@@ -994,13 +1007,13 @@ def main_knn(args, umx, torch, dev):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (f32 input)",
+        "dtype": "f32 matrix-core filter, f64 exact distances (f32 input)",
         "data": "synthetic",
         "config": {"workload": cfg["label"], "n": n, "dim": dim, "k": k},
-        "roofline": {"bound": "fp64-valu", "achieved": round(tflops, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(tflops / FP64_PEAK_TFLOPS, 4), "traffic": None,
-                     "kernel": "k_knn_qlds<16,32,16>", "avg_launch_us": round(sec * 1e6, 1),
-                     "ops_per_pair_dim": 3, "no_fma_issue_ceiling_tflops": FP64_PEAK_TFLOPS / 2},
+        "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": FP32_MFMA_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(tflops / FP32_MFMA_PEAK_TFLOPS, 4), "traffic": None,
+                     "kernel": f"k_knn_mfma<{(dim + 3) // 4}> (+ k_knn_rerank, k_knn_one)",
+                     "avg_launch_us": round(sec * 1e6, 1), "flops_per_pair_dim": 2},
         "cpu_baseline": cpu,
     }
     if cpu:

@@ -324,7 +324,11 @@ int umx_layout_tallies(const umx_layout* L, int64_t* out, int64_t cap);
  * resets the count.  m = 0 disables tracing. */
 int umx_layout_set_trace(umx_layout* L, int64_t max_records);
 int umx_layout_get_trace(umx_layout* L, double* out, int64_t cap_records, int64_t* total_records);
-/* HIP-event timing of the epoch kernels (on the handle's stream). */
+/* HIP-event timing of the epoch kernels (on the handle's stream): the
+ * summed time and number of kernel launches since timing was switched on.
+ * The sliced-ELL kernel is timed launch by launch (an epoch of P phases is P
+ * launches; the folds of split rows between them are not counted); the other
+ * kernels epoch by epoch. */
 int umx_layout_set_timing(umx_layout* L, int on);
 int umx_layout_timing(const umx_layout* L, double* kernel_ms, int64_t* launches);
 void umx_layout_destroy(umx_layout* L);

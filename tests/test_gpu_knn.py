@@ -31,6 +31,35 @@ def test_knn_matches_oracle(umx, n, dim, k, dtype):
     assert np.all(idx != np.arange(n)[:, None])
 
 
+@pytest.mark.parametrize("case", ["normal", "grid_ties", "offset", "duplicates"])
+@pytest.mark.parametrize("n,dim,k", [(512, 1, 16), (1001, 4, 15), (1500, 5, 1), (777, 63, 10), (2049, 64, 16),
+                                     (3000, 50, 15)])
+def test_knn_matrix_core_path(umx, case, n, dim, k):
+    """float32 input with at most 64 coordinates, k <= 16 and n >= 512: the
+    candidates come from the matrix cores (float32 dot products of centred
+    points within a proven margin of the exact distances), exact distances
+    rank them, and queries whose candidate lists cannot prove completeness
+    (ties on a grid, duplicates, a large common offset that widens the
+    margin) are searched again exactly.  Output bit-identical to the brute
+    force in every case."""
+    rng = np.random.default_rng(n * 131 + dim)
+    if case == "normal":
+        x = rng.normal(size=(n, dim))
+    elif case == "grid_ties":
+        x = rng.integers(0, 3, size=(n, dim)).astype(np.float64)
+    elif case == "offset":
+        x = rng.normal(size=(n, dim)) + 1000.0
+    else:
+        base = rng.normal(size=(n // 3 + 1, dim))
+        x = base[rng.integers(0, base.shape[0], size=n)]
+    x = x.astype(np.float32)
+    idx, dist = umx.find_knn(x, k)
+    ridx, rdist = oracle.knn_bruteforce(x.astype(np.float64), k)
+    assert np.array_equal(idx, ridx)
+    assert np.array_equal(dist, rdist.astype(np.float32))
+    assert np.all(idx != np.arange(n)[:, None])
+
+
 def test_knn_ties_go_to_smaller_index(umx):
     rng = np.random.default_rng(4)
     x = rng.integers(0, 3, size=(400, 3)).astype(np.float64)  # many duplicates and equal distances

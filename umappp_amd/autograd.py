@@ -120,7 +120,17 @@ class AutogradLayout:
     ``-alpha_n * grad`` with the reference's learning-rate decay (:131-133),
     the gradient's terms clipped one by one at the reference's clip value
     (:105-110) in the backward pass, so that a step is the reference's epoch
-    of clipped per-edge steps, all taken from the epoch's starting positions.
+    of clipped per-edge steps.
+
+    The epoch runs in ``phases`` steps, as the plain layout's does: the
+    firing edges are dealt to phases by their head (blocks of 16
+    observations, block b to phase b mod phases, the plain kernel's
+    deal), and each phase's loss is differentiated at the positions the
+    earlier phases' steps left, so later phases see fresh positions of the
+    points the earlier ones moved (with one phase every term of an epoch
+    reads the epoch's starting positions, and heavily connected points
+    overshoot: 15-NN preservation 0.0061 against the plain layout's 0.0117
+    on the 200k-point k = 30 graph).
 
     Unlike the reference, whose negative samples move only the head, the
     loss's repulsive terms also move the sampled point (they are terms of one
@@ -130,7 +140,7 @@ class AutogradLayout:
     """
 
     def __init__(self, row_ptr, targets, epochs_per_sample, num_epochs, negative_sample_rate=5.0, a=None, b=None,
-                 gamma=1.0, learning_rate=1.0, seed=1234567890, device="cuda"):
+                 gamma=1.0, learning_rate=1.0, seed=1234567890, device="cuda", phases=4):
         import torch
 
         self.dev = torch.device(device)
@@ -149,6 +159,11 @@ class AutogradLayout:
         self.gamma = float(gamma)
         self.alpha0 = float(learning_rate)
         self.seed = int(seed)
+        if int(phases) not in (1, 2, 4, 8):
+            raise ValueError("phases must be 1, 2, 4 or 8")
+        self.phases = int(phases)
+        # phase of every entry's head: blocks of 16 observations dealt in turn
+        self.head_phase = ((self.heads >> 4) & (self.phases - 1)).to(torch.int64)
         self._fire_counts = []
 
     def _alpha(self, n):
@@ -160,9 +175,10 @@ class AutogradLayout:
     def run(self, emb, epoch_limit=None):
         """Runs epochs [current, epoch_limit) on ``emb`` (float32 (n, dim) GPU
         tensor, updated in place).  Each epoch compacts its firing edges
-        before the loss: passing the whole edge list with a 0/1 mask instead
-        leaves most lanes of every wavefront idle (about 1 edge in 5 fires)
-        and was slower."""
+        before the loss (passing the whole edge list with a 0/1 mask instead
+        leaves most lanes of every wavefront idle, since about 1 edge in 5
+        fires, and was slower) and orders them by phase (a stable sort, so
+        each phase keeps the edges' schedule order)."""
         import torch
 
         limit = self.total if epoch_limit is None else int(epoch_limit)
@@ -178,13 +194,25 @@ class AutogradLayout:
             if idx.numel():
                 epns = epns_all[idx]
                 nneg = torch.trunc((ep - self.next_neg[idx]) / epns).to(torch.int32)
-                y = emb.detach().requires_grad_(True)
-                loss = umap_loss(y, self.a, self.b, self.heads[idx].contiguous(), self.tails[idx].contiguous(),
-                                 keys=self.keys[idx].contiguous(), num_neg=nneg.contiguous(),
-                                 gamma=self.gamma, seed=self.seed, step=n, clip=4.0, value=False)
-                (g,) = torch.autograd.grad(loss, [y])
-                with torch.no_grad():
-                    emb.add_(g, alpha=-self._alpha(n))
+                if self.phases > 1:
+                    ph = self.head_phase[idx]
+                    order = torch.sort(ph, stable=True).indices
+                    sizes = torch.bincount(ph, minlength=self.phases).tolist()
+                    pidx, pneg = idx[order], nneg[order]
+                else:
+                    sizes, pidx, pneg = [int(idx.numel())], idx, nneg
+                lo = 0
+                for size in sizes:
+                    if size:
+                        sel = pidx[lo:lo + size]
+                        y = emb.detach().requires_grad_(True)
+                        loss = umap_loss(y, self.a, self.b, self.heads[sel].contiguous(), self.tails[sel].contiguous(),
+                                         keys=self.keys[sel].contiguous(), num_neg=pneg[lo:lo + size].contiguous(),
+                                         gamma=self.gamma, seed=self.seed, step=n, clip=4.0, value=False)
+                        (g,) = torch.autograd.grad(loss, [y])
+                        with torch.no_grad():
+                            emb.add_(g, alpha=-self._alpha(n))
+                    lo += size
                 self.next[idx] += self.eps[idx]
                 self.next_neg[idx] += nneg.to(torch.float32) * epns
             self.current = n + 1

@@ -1212,6 +1212,7 @@ EventPool& event_pool() {
     return *p;
 }
 
+// Room for 'count' timed launches (an event pair each).
 void prepare_events(umx_layout* L, int count) {
     if (!L->timing) return;
     const size_t want = 2 * static_cast<size_t>(count);
@@ -1266,41 +1267,41 @@ int effective_tile(const umx_layout* L) {
 // One epoch of the sliced-ELL kernel: 'epoch' >= 0, or -1 - q for launch q
 // of a captured block (the device counter + q; the block's fold advances
 // the counter, as does this call's own fold for epoch -1).  Optional events
-// bracket the epoch kernel alone; the tally fold follows them unless 'fold'
-// is false (the caller then folds the epoch with fold_layout_epochs or
-// fold_layout_block before its ring entry is reused).
-void launch_sell_once(umx_layout* L, int epoch, hipEvent_t start = nullptr, hipEvent_t stop = nullptr,
-                      bool fold = true) {
-    if (start) UMX_HIP(hipEventRecord(start, L->s));
+// (an array of 2 * phases): pair p brackets the epoch kernel of phase p
+// alone; the tally fold follows them unless 'fold' is false (the caller then
+// folds the epoch with fold_layout_epochs or fold_layout_block before its
+// ring entry is reused).
+void launch_sell_once(umx_layout* L, int epoch, const hipEvent_t* ev = nullptr, bool fold = true) {
     const int tile = effective_tile(L);
     for (int p = 0; p < L->phases; ++p) {
+        if (ev) UMX_HIP(hipEventRecord(ev[2 * p], L->s));
         if (L->dtype == UMX_F32) {
             auto v = pull_view_f(L);
             attach_sell(v, L->sell_f);
             const auto w = phase_view<float>(L, v, p);
             umx::launch_layout_sell<float>(w, epoch, tile, L->waves, L->s);
+            if (ev) UMX_HIP(hipEventRecord(ev[2 * p + 1], L->s));
             umx::launch_fold_split<float>(w, epoch, L->s);
         } else {
             auto v = pull_view_d(L);
             attach_sell(v, L->sell_d);
             const auto w = phase_view<double>(L, v, p);
             umx::launch_layout_sell<double>(w, epoch, tile, L->waves, L->s);
+            if (ev) UMX_HIP(hipEventRecord(ev[2 * p + 1], L->s));
             umx::launch_fold_split<double>(w, epoch, L->s);
         }
     }
-    if (stop) UMX_HIP(hipEventRecord(stop, L->s));
     if (fold) umx::finish_layout_epoch(L->tally_part.p, L->tally.p, epoch, L->epoch_ctr.p, L->s);
 }
 
 // Epochs [first, first + count) launched one by one, their tallies folded by
 // one launch per kTallyRing epochs (a fold launch after every epoch cost a
-// few microseconds of kernel and launch gap each).  With 'timed', event pair
-// q brackets epoch kernel q.
+// few microseconds of kernel and launch gap each).  With 'timed', event pairs
+// q * phases .. (q + 1) * phases - 1 bracket the phase kernels of epoch q.
 void launch_sell_run(umx_layout* L, int first, int count, bool timed) {
     int pending = 0;
     for (int q = 0; q < count; ++q) {
-        launch_sell_once(L, first + q, timed ? L->events[2 * q] : nullptr, timed ? L->events[2 * q + 1] : nullptr,
-                         false);
+        launch_sell_once(L, first + q, timed ? &L->events[2 * static_cast<size_t>(q) * L->phases] : nullptr, false);
         if (++pending == umx::kTallyRing || q == count - 1) {
             umx::fold_layout_epochs(L->tally_part.p, L->tally.p, first + q + 1 - pending, pending, L->s);
             pending = 0;
@@ -1347,7 +1348,7 @@ void replay_block(umx_layout* L, int block) {
             // tallies and advances the counter
             for (int q0 = 0; q0 < block; q0 += umx::kTallyRing) {
                 const int nb = std::min(block - q0, umx::kTallyRing);
-                for (int u = 0; u < nb; ++u) launch_sell_once(L, -1 - u, nullptr, nullptr, false);
+                for (int u = 0; u < nb; ++u) launch_sell_once(L, -1 - u, nullptr, false);
                 umx::fold_layout_block(L->tally_part.p, L->tally.p, nb, L->epoch_ctr.p, L->s);
             }
         } catch (...) {
@@ -1370,7 +1371,7 @@ void run_sell(umx_layout* L, int first, int count) {
     ensure_sell_current(L);
     ensure_tables(L);
     if (L->timing) {
-        prepare_events(L, count);
+        prepare_events(L, count * L->phases);
         launch_sell_run(L, first, count, true);
     } else if (L->graph_block > 0 && L->trace_cap == 0 && count >= L->graph_block) {
         umx::set_epoch_counter(L->epoch_ctr.p, first, L->s);
@@ -1381,7 +1382,7 @@ void run_sell(umx_layout* L, int first, int count) {
         launch_sell_run(L, first, count, false);
     }
     UMX_HIP(hipStreamSynchronize(L->s));
-    collect_events(L, count);
+    collect_events(L, count * L->phases);
     if (count > 0) L->home = kHomeSell;
 }
 
