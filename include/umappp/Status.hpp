@@ -1,8 +1,9 @@
 // Drop-in for umappp/Status.hpp (reference :25-142).  The schedule and the
 // SGD epochs live on the GPU (a umx_layout handle); run() reads the caller's
 // embedding, runs the epochs there and writes the result back, as the
-// reference's run() does on the host.  Unlike the reference the object is
-// move-only: it owns device memory.
+// reference's run() does on the host.  Copies are deep, as the reference's
+// (its members are values): a copy owns a clone of the device state
+// (umx_layout_clone) and runs independently of the original.
 #ifndef UMAPPP_AMD_STATUS_HPP
 #define UMAPPP_AMD_STATUS_HPP
 
@@ -26,6 +27,17 @@ public:
                                             my_options.optimize_seed));
     }
 
+    Status(const Status& other)
+        : my_layout(clone(other.my_layout.get()), &umx_layout_destroy),
+          my_options(other.my_options),
+          my_num_dim(other.my_num_dim) {}
+    Status& operator=(const Status& other) {
+        if (this != &other) {
+            Status copy(other);
+            *this = std::move(copy);
+        }
+        return *this;
+    }
     Status(Status&&) = default;
     Status& operator=(Status&&) = default;
 
@@ -63,6 +75,12 @@ public:
     void run(Float_* const embedding) { run(embedding, num_epochs()); }
 
 private:
+    static umx_layout* clone(const umx_layout* layout) {
+        umx_layout* copy = nullptr;
+        detail::check(umx_layout_clone(layout, &copy));
+        return copy;
+    }
+
     std::unique_ptr<umx_layout, void (*)(umx_layout*)> my_layout;
     Options my_options;
     std::size_t my_num_dim;

@@ -331,6 +331,14 @@ int umx_layout_get_trace(umx_layout* L, double* out, int64_t cap_records, int64_
  * kernels epoch by epoch. */
 int umx_layout_set_timing(umx_layout* L, int on);
 int umx_layout_timing(const umx_layout* L, double* kernel_ms, int64_t* launches);
+/* Deep copy of a layout: schedule (epochs per sample, next epochs), current
+ * epoch, tallies, parameters, settings and embedding, on the same device; the
+ * copy and the source then run independently and, from the same state, give
+ * the same results bit for bit.  Replaces the copy of umappp::Status, whose
+ * members are values (Status.hpp:25-142, members :42-45); the drop-in
+ * umappp::Status copy constructor calls it.  Work items, the sliced-ELL copy
+ * and captured blocks are built again on the copy's first run. */
+int umx_layout_clone(const umx_layout* src, umx_layout** out);
 void umx_layout_destroy(umx_layout* L);
 
 /* ------------------------------------------------------------------ */

@@ -198,3 +198,33 @@ def test_status_restart_equals_single_run(umx):
     assert st.epoch() == st2.epoch() == 150
     assert np.array_equal(e1, e2)
     assert not np.array_equal(e1, e0)
+
+
+def test_copied_status_runs_independently(umx):
+    """The reference's Status is copyable (its members are values,
+    Status.hpp:42-45): a copy taken at epoch 37 and run to the end leaves
+    the original at epoch 37, and gives the original's final embedding and
+    schedule bit for bit, which equal a single uninterrupted run's."""
+    import copy
+
+    _, (rp, idx, d) = c1_knn()
+    nl = umx.NeighborList(rp, idx, d.astype(np.float32))
+    opt = umx.Options(initialize_method=umx.InitializeMethod.RANDOM, num_epochs=150)
+    e0 = np.zeros((2000, 2), np.float32)
+    st = umx.initialize(nl.copy(), 2, e0, opt)
+    e1 = e0.copy()
+    st.run(e1, 37)
+    st_copy = copy.copy(st)
+    assert st_copy.epoch() == 37
+    e2 = e1.copy()
+    st_copy.run(e2)
+    assert st.epoch() == 37 and st_copy.epoch() == 150
+    st.run(e1)
+    assert np.array_equal(e1, e2)
+    a, b = st.get_epoch_data(), st_copy.get_epoch_data()
+    assert np.array_equal(a.epoch_of_next_sample, b.epoch_of_next_sample)
+    assert np.array_equal(a.epoch_of_next_negative_sample, b.epoch_of_next_negative_sample)
+    st3 = umx.initialize(nl.copy(), 2, np.zeros((2000, 2), np.float32), opt)
+    e3 = e0.copy()
+    st3.run(e3)
+    assert np.array_equal(e1, e3)

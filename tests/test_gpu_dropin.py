@@ -1,8 +1,10 @@
 """The C++ drop-in headers end to end on the GPU: a program written against
 umappp's public surface (initialize with a data matrix and the default
-spectral initialisation, then Status::run for all epochs), compiled with g++
-against libumappp_amd, gives a finite embedding whose 15-NN preservation
-matches the Python path's on the same input."""
+spectral initialisation, then Status::run for all epochs, with a copy of the
+Status taken half-way and run to the end beside the original), compiled with
+g++ against libumappp_amd, gives a finite embedding whose 15-NN preservation
+matches the Python path's on the same input; the copy's embedding equals the
+original's bit for bit."""
 import shutil
 import subprocess
 from pathlib import Path
@@ -32,8 +34,14 @@ int main(int argc, char** argv) {
     umappp::Options opt;                       // SPECTRAL initialisation, 500 epochs at this size
     auto status = umappp::initialize(dim, n, data.data(), 2, emb.data(), opt);
     if (status.num_observations() != n || status.num_epochs() != 500) return 3;
+    status.run(emb.data(), 250);
+    auto copy = status;                        // a deep copy, as the reference's Status copies by value
+    std::vector<float> emb2 = emb;
+    copy.run(emb2.data());
+    if (status.epoch() != 250 || copy.epoch() != 500) return 5;
     status.run(emb.data());
     if (status.epoch() != 500) return 4;
+    if (emb != emb2) return 6;                 // the same state gives the same layout, bit for bit
     FILE* o = std::fopen(argv[2], "wb");
     std::fwrite(emb.data(), sizeof(float), emb.size(), o);
     std::fclose(o);

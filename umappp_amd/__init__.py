@@ -100,6 +100,7 @@ def _declare(L):
         "umx_layout_set_timing": (C.c_int, [_vp, C.c_int]),
         "umx_layout_timing": (C.c_int, [_vp, _f64p, _i64p]),
         "umx_layout_destroy": (None, [_vp]),
+        "umx_layout_clone": (C.c_int, [_vp, C.POINTER(C.c_void_p)]),
         "umx_knn_bruteforce": (C.c_int, [C.c_int, C.c_int64, C.c_int, _vp, C.c_int, _vp, _vp, C.c_int]),
         "umx_graph_create_knn": (C.c_int, [C.c_int, C.c_int64, C.c_int, _vp, C.c_int, C.c_int, C.POINTER(_vp)]),
         "umx_graph_components": (C.c_int, [_vp, _i64p]),
@@ -403,6 +404,18 @@ class Layout:
     @property
     def handle(self):
         return self._h
+
+    def clone(self) -> "Layout":
+        """Deep copy (umx_layout_clone): schedule, current epoch, tallies,
+        parameters, settings and embedding; the copy runs independently of
+        this layout and, from the same state, gives the same results bit for
+        bit (the reference's Status is copyable, Status.hpp:42-45)."""
+        h = C.c_void_p()
+        _check(lib().umx_layout_clone(self._h, C.byref(h)))
+        other = Layout.__new__(Layout)
+        other.__dict__.update({key: val for key, val in self.__dict__.items() if key != "_h"})
+        other._h = h
+        return other
 
     def info(self):
         n, nnz, te, ce = C.c_int64(), C.c_int64(), C.c_int(), C.c_int()
@@ -720,6 +733,18 @@ class Status:
     @property
     def layout(self) -> Layout:
         return self._layout
+
+    def __copy__(self) -> "Status":
+        """A deep copy, as the reference's value copy (Status.hpp:42-45): the
+        copy owns a clone of the device state and runs independently."""
+        other = Status.__new__(Status)
+        other._layout = self._layout.clone()
+        other._options = self._options
+        other._num_dim = self._num_dim
+        return other
+
+    def __deepcopy__(self, memo) -> "Status":
+        return self.__copy__()
 
     def run(self, embedding: np.ndarray, epoch_limit: Optional[int] = None) -> None:
         if epoch_limit is None:

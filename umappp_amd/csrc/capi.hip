@@ -1451,6 +1451,77 @@ int umx_layout_create_csr(int dtype, int64_t n, const int64_t* row_ptr, const in
 
 void umx_layout_destroy(umx_layout* L) { delete L; }
 
+int umx_layout_clone(const umx_layout* Lc, umx_layout** out) {
+    return guard([&] {
+        require(Lc != nullptr && out != nullptr, UMX_ERR_ARGUMENT, "NULL argument");
+        *out = nullptr;
+        // the canonical schedule is the CSR copy; bringing it up to date
+        // does not change the source's logical state
+        umx_layout* S = const_cast<umx_layout*>(Lc);
+        ensure_csr_current(S);
+        std::unique_ptr<umx_layout> L(new umx_layout());
+        L->s = make_stream();
+        L->dtype = S->dtype;
+        L->n = S->n;
+        L->nnz = S->nnz;
+        L->max_degree = S->max_degree;
+        L->total_epochs = S->total_epochs;
+        L->current_epoch = S->current_epoch;
+        L->nsr = S->nsr;
+        L->dim = S->dim;
+        L->mode = S->mode;
+        L->one_way = S->one_way;
+        L->phases = S->phases;
+        L->fires_per_item = S->fires_per_item;
+        L->edges_per_item = S->edges_per_item;
+        L->a = S->a;
+        L->b = S->b;
+        L->gamma = S->gamma;
+        L->alpha0 = S->alpha0;
+        L->seed = S->seed;
+        L->params = S->params;
+        L->have_emb = S->have_emb;
+        L->tile_cols = S->tile_cols;
+        L->waves = S->waves;
+        L->groups_req = S->groups_req;
+        L->graph_block = S->graph_block;
+        L->inst_off = S->inst_off;
+        L->inst_eoff = S->inst_eoff;
+        L->inst_epochs = S->inst_epochs;
+        L->inst_seeds = S->inst_seeds;
+        auto copy = [&](auto& dst, const auto& src) {
+            if (src.p == nullptr) return;
+            dst.alloc(src.n);
+            UMX_HIP(hipMemcpyAsync(dst.p, src.p, sizeof(*src.p) * src.n, hipMemcpyDeviceToDevice, S->s));
+        };
+        copy(L->rp, S->rp);
+        copy(L->tg, S->tg);
+        copy(L->ef, S->ef);
+        copy(L->nf, S->nf);
+        copy(L->gf, S->gf);
+        copy(L->mf, S->mf);
+        copy(L->mf2, S->mf2);
+        copy(L->ed, S->ed);
+        copy(L->nd, S->nd);
+        copy(L->gd, S->gd);
+        copy(L->md, S->md);
+        copy(L->md2, S->md2);
+        copy(L->tally, S->tally);
+        copy(L->tally_part, S->tally_part);
+        copy(L->d_inst_off, S->d_inst_off);
+        copy(L->of_row, S->of_row);
+        copy(L->d_inst_epochs, S->d_inst_epochs);
+        L->cursor.alloc(2);
+        UMX_HIP(hipMemsetAsync(L->cursor.p, 0, sizeof(unsigned long long) * 2, S->s));
+        L->epoch_ctr.alloc(1);
+        umx::set_epoch_counter(L->epoch_ctr.p, L->current_epoch, S->s);
+        UMX_HIP(hipStreamSynchronize(S->s));
+        // work items, the sliced-ELL copy, instance tables and captured
+        // blocks are built again on the first run (n_items = -1)
+        *out = L.release();
+    });
+}
+
 int umx_layout_info(const umx_layout* L, int64_t* n, int64_t* nnz, int* total_epochs, int* current_epoch) {
     return guard([&] {
         require(L != nullptr, UMX_ERR_ARGUMENT, "layout is NULL");
