@@ -431,7 +431,7 @@ class ReferenceLayout(threading.Thread):
 
 
 def cross_entropy(umx, torch, dev, y, heads, tails, coef, a, b, m=5, seed=4242):
-    """The UMAP cross-entropy of embedding y (device float n x 2) over all
+    """The UMAP cross-entropy of embedding y (device float n x D) over all
     edges (coefficient = fuzzy weight) and m negative samples per edge drawn
     by the counter sampler (the same draws for every embedding), per unit of
     weight: umx_loss_forward on the GPU."""
@@ -445,7 +445,7 @@ def cross_entropy(umx, torch, dev, y, heads, tails, coef, a, b, m=5, seed=4242):
     ab = torch.tensor([a, b], dtype=torch.float64, device=dev)
     out = torch.zeros(1, dtype=torch.float64, device=dev)
     yc = y.contiguous()
-    rc = fn(yc.shape[0], 2, yc.data_ptr(), heads.numel(), heads.data_ptr(), tails.data_ptr(), None, coef.data_ptr(),
+    rc = fn(yc.shape[0], yc.shape[1], yc.data_ptr(), heads.numel(), heads.data_ptr(), tails.data_ptr(), None, coef.data_ptr(),
             None, m, ab.data_ptr(), 1.0, seed, 0, out.data_ptr(), torch.cuda.current_stream(dev).cuda_stream)
     if rc != 0:
         raise RuntimeError(umx.lib().umx_last_error().decode())
@@ -868,6 +868,8 @@ def main_autograd(args, umx, torch, dev):
     log(f"200 epochs: plain {plain_s * 1000:.1f} ms, autograd-driven {auto_s * 1000:.1f} ms ({U} edge-updates)")
     pres_plain = preservation(kidx, plain, 15, torch)
     pres_auto = preservation(kidx, ya, 15, torch)
+    ce_plain = cross_entropy(umx, torch, dev, plain, heads, tails, w, a, b)
+    ce_auto = cross_entropy(umx, torch, dev, ya, heads, tails, w, a, b)
 
     # roofline of the backward pass: per edge, the head/tail indices and the
     # weight (12 B), (2 + m) point gathers of D floats and (2 + m) * D double
@@ -924,6 +926,8 @@ def main_autograd(args, umx, torch, dev):
         "loss_edges_per_s": round(E / ((fwd_ms + bwd_ms) / 1000.0), 1),
         "layout_200_epochs_ms": {"plain": round(plain_s * 1000, 3), "autograd": round(auto_s * 1000, 3)},
         "knn15_preservation": {"plain": round(pres_plain, 4), "autograd": round(pres_auto, 4)},
+        "cross_entropy": {"plain": round(ce_plain, 6), "autograd": round(ce_auto, 6),
+                          "measure": "all edges, 5 counter-sampler negatives each, per unit of weight"},
     }
     print(json.dumps(out), flush=True)
 

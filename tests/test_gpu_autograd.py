@@ -343,3 +343,64 @@ def test_autograd_layout_quality(torch_dev, umx):
     print(f"autograd layout vs plain: preservation {qa:.4f} vs {qp:.4f}, trustworthiness {ta:.4f} vs {tp:.4f}")
     assert np.all(np.isfinite(auto))
     assert abs(qa - qp) <= 0.01 and abs(ta - tp) <= 0.01
+
+
+def test_autograd_layout_at_full_c5_size(torch_dev, umx):
+    """Config C5 at full size (200,000 x 50 mixture, k = 30, D = 3): 200 epochs
+    driven through the autograd loss (one loss and gradient step per phase,
+    as the plain kernel's phases) against the plain layout kernel from the
+    same start and schedule: 15-NN preservation within 0.01 of it, and a
+    final UMAP cross-entropy (all edges, 5 counter-sampler negatives each)
+    no more than 5% above it.  The bound is one-sided: the autograd step is
+    the loss's full gradient, which also moves each negative sample away
+    from its head (the reference's update moves only the head,
+    optimize_layout.hpp:160-176), so it may end lower."""
+    torch, dev = torch_dev
+    from umappp_amd.autograd import AutogradLayout, edge_heads, umap_loss
+
+    n, k, D, T = 200_000, 30, 3, 200
+    x, _ = mixture(5, n, 50, 30)
+    idx, dist = umx.find_knn(x, k)
+    g = umx.Graph(umx.NeighborList.from_knn(idx, dist))
+    g.neighbor_similarities()
+    g.combine(1.0)
+    a, b = umx.find_ab(1.0, 0.1)
+    lay = umx.Layout(graph=g, num_epochs=T, negative_sample_rate=5.0)
+    ed = lay.epoch_data()
+    emb0 = np.random.default_rng(5).uniform(-10, 10, size=(n, D)).astype(np.float32)
+    lay.set_params(D, a, b, 1.0, 1.0, 99)
+    lay.set_embedding(emb0)
+    lay.run(T)
+    plain = torch.from_numpy(lay.get_embedding()).to(dev)
+    al = AutogradLayout(ed.cumulative_num_edges, ed.edge_targets, ed.epochs_per_sample, T, 5.0, a, b, seed=99)
+    auto = torch.from_numpy(emb0.copy()).to(dev)
+    al.run(auto)
+    assert np.array_equal(np.asarray(al.fires), lay.tallies()[:, 0])
+    assert bool(torch.isfinite(auto).all())
+
+    def preservation(y):
+        yc = y.double()
+        sq = (yc * yc).sum(1)
+        q = torch.arange(0, n, 97, device=dev)
+        d2 = sq[q, None] + sq[None, :] - 2.0 * (yc[q] @ yc.T)
+        d2[torch.arange(q.numel(), device=dev), q] = float("inf")
+        low = torch.topk(d2, 15, largest=False).indices.cpu().numpy()
+        hi = idx[q.cpu().numpy(), :15]
+        return float(np.mean([len(set(low[r]) & set(hi[r])) for r in range(low.shape[0])])) / 15.0
+
+    nl = g.export()
+    heads = edge_heads(torch.from_numpy(nl.row_ptr)).to(dev)
+    tails = torch.from_numpy(nl.idx).to(dev)
+    w = torch.from_numpy(nl.val.astype(np.float32)).to(dev)
+    at = torch.tensor(a, dtype=torch.float64, device=dev)
+    bt = torch.tensor(b, dtype=torch.float64, device=dev)
+
+    def cross_entropy(y):
+        return float(umap_loss(y, at, bt, heads, tails, coef=w, m=5, seed=4242, step=0)) / float(w.double().sum())
+
+    pp, pa = preservation(plain), preservation(auto)
+    cp, ca = cross_entropy(plain), cross_entropy(auto)
+    print(f"C5 autograd-driven layout vs plain: 15-NN preservation {pa:.4f} vs {pp:.4f}, "
+          f"cross-entropy {ca:.5f} vs {cp:.5f}")
+    assert abs(pa - pp) <= 0.01
+    assert ca <= 1.05 * cp
