@@ -14,7 +14,8 @@
 // loads of a tile in flight together) and advances the schedule of the
 // firing slots at once: the fire test, the number of negative samples and
 // both updated epochs depend on the schedule alone, so they are computed in
-// this pass and stored straight away, and the stores complete while the
+// this pass (for every column, in straight-line code, so that the columns'
+// IEEE divisions overlap) and stored straight away, and the stores complete while the
 // slots are walked (the vector memory counter retires in order, so stores
 // issued just before the next tile's loads would hold those loads back).
 // Each lane keeps its slots' targets and sample counts in LDS (it reads back
@@ -218,7 +219,9 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
         const int32_t aux = m < v.n_items ? v.item_aux[m] : kAuxPadding;
         const bool has_item = aux != kAuxPadding;
         bool live = has_item;  // the item's instance runs this epoch
-        const int width = v.chunk_width[c];
+        // the chunk's width is the same for the whole wavefront: read as a
+        // uniform value, the per-column guards below are scalar branches
+        const int width = __builtin_amdgcn_readfirstlane(v.chunk_width[c]);
 #if UMX_LONG_PRIO
         // Wide chunks (the pieces of heavy rows) are the longest chains of
         // sequential steps of a launch, and wavefronts of short chunks share
@@ -279,15 +282,13 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
             }
         };
 
-        for (int t0 = 0; t0 < width; t0 += TW) {
+        // The schedule of the tile of columns t0 .. t0 + TW - 1, coalesced,
+        // all columns in flight together (columns past the chunk's end are
+        // given values that never fire).
+        T xs[TW], ys[TW], es[TW];
+        int32_t ts[TW];
+        auto load_tile = [&](int t0) {
             const int tw = width - t0 < TW ? width - t0 : TW;
-            // coalesced fetch of the tile (all columns in flight together);
-            // the schedule of the firing slots is advanced here and stored at
-            // once (it depends on the schedule only), so the stores complete
-            // while the slots are walked
-            unsigned int mask = 0;
-            T xs[TW], ys[TW], es[TW];
-            int32_t ts[TW];
 #pragma unroll
             for (int k = 0; k < TW; ++k) {
                 if (k < tw) {
@@ -296,22 +297,43 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
                     ys[k] = ld_stream(v.sell_nn + slot);
                     es[k] = ld_stream(v.sell_eps + slot);
                     ts[k] = ld_stream(v.sell_tg + slot);
+                } else {
+                    xs[k] = ep + 1;
+                    ys[k] = ep;
+                    es[k] = 1;
+                    ts[k] = 0;
                 }
             }
+        };
+        if (width > 0) load_tile(0);
+        for (int t0 = 0; t0 < width; t0 += TW) {
+            // The fire test, sample count and both updated epochs of every
+            // column in straight-line code (a column that does not fire has
+            // them computed and dropped), so the columns' division chains
+            // interleave instead of running one column at a time.
+            unsigned int mask = 0;
+            T nxv[TW], nnv[TW];
 #pragma unroll
             for (int k = 0; k < TW; ++k) {
-                if (k < tw) {
-                    const bool fire = live && !(xs[k] > ep);
-                    tile.tg[k][lane] = ts[k];
-                    if (fire) {
-                        const int64_t slot = base + static_cast<int64_t>(t0 + k) * kWave + lane;
-                        const T epns = es[k] / v.nsr;
-                        const int nneg = static_cast<int>((ep - ys[k]) / epns);
-                        tile.nneg[k][lane] = nneg;
-                        st_stream(v.sell_nx + slot, xs[k] + es[k]);
-                        st_stream(v.sell_nn + slot, ys[k] + static_cast<T>(nneg) * epns);
-                        mask |= 1u << k;
-                    }
+                const bool fire = live && !(xs[k] > ep);
+                const T epns = es[k] / v.nsr;
+                const T q = (ep - ys[k]) / epns;
+                const int nneg = static_cast<int>(fire ? q : static_cast<T>(0));
+                tile.tg[k][lane] = ts[k];
+                tile.nneg[k][lane] = nneg;
+                nxv[k] = xs[k] + es[k];
+                nnv[k] = ys[k] + static_cast<T>(nneg) * epns;
+                mask |= (fire ? 1u : 0u) << k;
+            }
+            // The firing slots' schedule is stored at once (it depends on the
+            // schedule only), so the stores complete while the slots are
+            // walked.
+#pragma unroll
+            for (int k = 0; k < TW; ++k) {
+                if (mask & (1u << k)) {
+                    const int64_t slot = base + static_cast<int64_t>(t0 + k) * kWave + lane;
+                    st_stream(v.sell_nx + slot, nxv[k]);
+                    st_stream(v.sell_nn + slot, nnv[k]);
                 }
             }
             while (mask) {
@@ -388,6 +410,7 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
                     apply_batch();
                 }
             }
+            if (t0 + TW < width) load_tile(t0 + TW);
         }
         // every item writes, moved or not (the output buffer must hold the
         // whole state): a whole row its new position, a piece of a split row
