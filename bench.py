@@ -212,7 +212,7 @@ def main():
             if args.phases:
                 lay.set_phases(args.phases)
             if args.edges_per_item or args.fires_per_item:
-                lay.set_work_split(args.fires_per_item or 32.0, args.edges_per_item or 48)
+                lay.set_work_split(args.fires_per_item or 32.0, args.edges_per_item or 32)
         t_d = time.perf_counter()
         lay.set_params(2, a, b, 1.0, 1.0, seed)
         lay.set_embedding(device_ptr=emb0.data_ptr())

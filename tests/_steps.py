@@ -35,23 +35,31 @@ def phase_of(index, phases):
     return ((np.asarray(index, dtype=np.int64) >> 4) & (phases - 1)).astype(np.int64)
 
 
-def check_pull_inputs(trace, before, after, dim, dtype, phases):
+def check_pull_inputs(trace, launch_phase, before, after, dim, dtype, phases, last_phase=None):
     """One pull-mode epoch's reads: every head's first step starts from its
     position at the epoch's start; a head step (kind 0) or negative sample
     (kind 1) reads the other point as this epoch left it if that point is of
-    an earlier phase than the head ('after'), else as it was at the epoch's
-    start ('before'); a reverse step (kind 2) directly follows the head step
-    g of the same edge (kind 0): the head from y_i + g, the other from
-    y_t - g (where the pair's first event leaves the tail)."""
+    an earlier phase than the launch that took the step, else as it was at
+    the epoch's start ('before'); a reverse step (kind 2) directly follows
+    the head step g of the same edge (kind 0): the head from y_i + g, the
+    other from y_t - g (where the pair's first event leaves the tail).  A
+    point whose row is cut into pieces moves in every phase that runs one of
+    its pieces (last_phase: the last such phase of each point; whole rows:
+    their own phase): read by a later launch than its last phase it is at
+    its final position ('after'); read in between, at a state of the epoch
+    the trace does not pin, it is not compared."""
     kinds = trace[:, 0].astype(np.int64)
     heads = trace[:, 1].astype(np.int64)
     others = trace[:, 2].astype(np.int64)
     h = trace[:, 3:3 + dim].astype(dtype)
     o = trace[:, 3 + dim:3 + 2 * dim].astype(dtype)
     g = trace[:, 3 + 2 * dim:].astype(dtype)
-    fresh = phase_of(others, phases) < phase_of(heads, phases)
+    launch_phase = np.asarray(launch_phase, dtype=np.int64)
+    fresh = phase_of(others, phases) < launch_phase
     read = np.where(fresh[:, None], after[others], before[others])
-    fwd = kinds != 2
+    last = phase_of(others, phases) if last_phase is None else np.asarray(last_phase)[others]
+    pinned = ~fresh | (last < launch_phase)
+    fwd = (kinds != 2) & pinned
     assert np.array_equal(o[fwd], read[fwd])
     _, first = np.unique(heads, return_index=True)
     assert np.array_equal(h[first], before[heads[first]])
@@ -67,7 +75,9 @@ def check_pull_inputs(trace, before, after, dim, dtype, phases):
     m0 = k0[order[at]]
     assert np.array_equal(key0[at], heads[k2] * n + others[k2])
     assert np.array_equal(h[k2], h[m0] + g[m0])
-    assert np.array_equal(o[k2], read[k2] - g[m0])
+    p2 = pinned[k2]
+    assert np.array_equal(o[k2][p2], (read[k2] - g[m0])[p2])
+    return float(np.mean(pinned)) if pinned.size else 1.0
 
 
 def check_reverse_steps(trace, dim, dtype, a, b, gamma, alpha, rtol, atol):

@@ -229,7 +229,7 @@ def test_production_per_step(umx, mode, dtype, dim, tile, waves, split, phases):
     for limit in range(1, 7):
         before = lay.get_embedding()
         lay.run(limit)
-        tr = lay.get_trace()
+        tr, tph = lay.get_trace(with_phase=True)
         orc.run(eo, A, B, epoch_limit=limit)
         seq = orc.traced()
         orc.trace(600_000)
@@ -244,7 +244,17 @@ def test_production_per_step(umx, mode, dtype, dim, tile, waves, split, phases):
             rev = trip[kinds == 2][:, [1, 2]]
             assert np.array_equal(key(rev), key(att[:, [2, 1]]))
             check_reverse_steps(tr[kinds == 2], dim, dtype, A, B, 1.0, alpha, 1e-5, 1e-6)
-            check_pull_inputs(tr, before, lay.get_embedding(), dim, dtype, lay.phases()[0])
+            P = lay.phases()[0]
+            _, _, irow, isplit, iph, _ = lay.work_items(with_phase=True)
+            last = phase_of(np.arange(before.shape[0]), P)
+            np.maximum.at(last, irow, iph)
+            share = check_pull_inputs(tr, tph, before, lay.get_embedding(), dim, dtype, P, last)
+            assert tr.shape[0] == 0 or share > 0.3
+            # a head's steps outside its own phase are those of a split row's
+            # pieces, which run one phase after another from the row's phase
+            whole = np.ones(before.shape[0], bool)
+            whole[irow[isplit]] = False
+            assert np.array_equal(tph[whole[heads]], phase_of(heads[whole[heads]], P))
         check_steps(tr[kinds != 2], [int((kinds != 2).sum())], dim, dtype, A, B, 1.0, T, limit - 1, 1e-5, 1e-6)
         moved = np.zeros(before.shape, np.float64)
         d = tr[:, 3 + 2 * dim:]
@@ -347,9 +357,20 @@ def test_work_items_cover_every_edge_once(umx, fires, cap):
     crp, cidx, cw = oracle.combine_neighbor_sets(rp, idx, wd, 1.0)
     lay = umx.Layout(csr=umx.NeighborList(crp, cidx, cw.astype(np.float32)), num_epochs=500, negative_sample_rate=5.0)
     lay.set_work_split(fires, cap)
-    b, ln, row, split, ph = lay.work_items(with_phase=True)
+    b, ln, row, split, ph, from_out = lay.work_items(with_phase=True)
     P, _ = lay.phases()
-    assert np.array_equal(ph, phase_of(row, P))  # each row's items run in its phase
+    # a whole row runs in its own phase; piece q (in edge order) of a row cut
+    # into c pieces runs in phase (p0 + q) mod P, and starts from the head as
+    # an earlier phase left it when a piece of its row ran in an earlier phase
+    order = np.lexsort((b, row))
+    rs, ps = row[order], ph[order]
+    first = np.r_[True, rs[1:] != rs[:-1]]
+    q = np.arange(rs.size) - np.maximum.accumulate(np.where(first, np.arange(rs.size), 0))
+    c = np.bincount(rs)[rs]
+    p0 = phase_of(rs, P)
+    assert np.array_equal(ps, (p0 + q) % P)
+    earliest = np.where(p0 + c - 1 >= P, 0, p0)
+    assert np.array_equal(from_out[order], ps > earliest)
     ed = lay.epoch_data()
     rpe = ed.cumulative_num_edges
     assert np.all(ln >= 1) and np.all(ln <= cap)

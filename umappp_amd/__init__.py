@@ -492,14 +492,14 @@ class Layout:
         _check(lib().umx_layout_get_mode(self._h, C.byref(m)))
         return m.value
 
-    def set_work_split(self, fires_per_item: float = 32.0, edges_per_item: int = 48):
+    def set_work_split(self, fires_per_item: float = 32.0, edges_per_item: int = 32):
         _check(lib().umx_layout_set_work_split(self._h, float(fires_per_item), int(edges_per_item)))
         self._split = (float(fires_per_item), int(edges_per_item))
 
     def work_split(self):
         """(fires_per_item, edges_per_item) last set on this layout (the
         library's defaults otherwise)."""
-        return getattr(self, "_split", (32.0, 48))
+        return getattr(self, "_split", (32.0, 32))
 
     def set_tile(self, tile_cols: int = 0):
         """LDS tile of the pull kernel: 4, 8, 16 or 32 columns, or 0 to choose
@@ -508,8 +508,9 @@ class Layout:
 
     def work_items(self, with_phase: bool = False):
         """(begin, length, row, split) arrays of the production kernels' work
-        items in run order, padding left out; with ``with_phase`` also each
-        item's phase."""
+        items in run order, padding left out (split: the row has several
+        items); with ``with_phase`` also each item's phase and whether it
+        starts from the head as an earlier phase of the epoch left it."""
         cnt = C.c_int64()
         _check(lib().umx_layout_work_items(self._h, None, None, None, 0, C.byref(cnt)))
         m = cnt.value
@@ -521,8 +522,10 @@ class Layout:
         P, off = self.phases()
         ph = np.repeat(np.arange(P), np.diff(off))
         real = ln > 0
-        out = (b[real], ln[real], (ru[real] & 0x7FFFFFFF).astype(np.int64), (ru[real] >> 31).astype(bool))
-        return out + (ph[real],) if with_phase else out
+        rows = (ru[real] & 0x7FFFFFFF).astype(np.int64)
+        split = np.bincount(rows)[rows] > 1
+        out = (b[real], ln[real], rows, split)
+        return out + (ph[real], (ru[real] >> 31).astype(bool)) if with_phase else out
 
     def set_phases(self, phases: int):
         """Phases of a pull-mode epoch: 1, 2, 4 or 8 (default 4)."""
@@ -540,18 +543,22 @@ class Layout:
         _check(lib().umx_layout_set_trace(self._h, int(max_records)))
         self._trace_cap = int(max_records)
 
-    def get_trace(self, partial: bool = False) -> np.ndarray:
+    def get_trace(self, partial: bool = False, with_phase: bool = False):
         """Recorded steps since the last call, as an (m, 3 + 3*num_dim) array:
         kind, head, other, head before, other before, head move.  More steps
         than the capacity is an error unless ``partial``, which returns the
-        steps that were recorded (the first ones to claim a record)."""
+        steps that were recorded (the first ones to claim a record).  With
+        ``with_phase`` also each step's launch phase (pull mode; 0 otherwise)."""
         width = 3 + 3 * self.num_dim
         out = np.zeros((self._trace_cap, width))
         total = C.c_int64()
         _check(lib().umx_layout_get_trace(self._h, _ptr(out), self._trace_cap, C.byref(total)))
         if total.value > self._trace_cap and not partial:
             raise UmxError(-6, f"trace overflow: {total.value} steps, capacity {self._trace_cap}")
-        return out[: min(total.value, self._trace_cap)]
+        out = out[: min(total.value, self._trace_cap)]
+        code = out[:, 0].astype(np.int64)
+        out[:, 0] = code % 4
+        return (out, code // 4) if with_phase else out
 
     def instances(self):
         """(offsets[n_inst + 1], num_epochs[n_inst]) of the layout's instances."""

@@ -695,12 +695,12 @@ struct umx_layout {
     // split rows [phase_split[p], phase_split[p + 1])
     int phases = 4;
     std::vector<int64_t> phase_items, phase_split;
-    // Only heavy and long rows are split: a row's steps should see each
-    // other sequentially (items of a split row start from one snapshot of
-    // the head and their summed moves overshoot), but every phase of an epoch
-    // waits for its longest item.
+    // Only heavy and long rows are split: every phase of an epoch waits for
+    // its longest item, and the pieces of a split row run one per phase in
+    // turn, each from where the previous one left the head (only a row of
+    // more pieces than phases sums several pieces from one start).
     double fires_per_item = 32.0;
-    int edges_per_item = 48;
+    int edges_per_item = 32;
     double a = 0, b = 0, gamma = 1, alpha0 = 1;
     uint64_t seed = 0;
     bool params = false, have_emb = false;
@@ -937,10 +937,12 @@ void build_items_t(umx_layout* L, const T* eps) {
     const int P = L->phases;
     L->phase_items.assign(static_cast<size_t>(P) + 1, 0);
     L->phase_split.assign(static_cast<size_t>(P) + 1, 0);
+    DevBuf<int64_t> row_pieces;
+    umx::count_row_pieces<T>(L->n, L->rp.p, eps, L->fires_per_item, L->edges_per_item, row_pieces, L->s);
     if (P == 1) {
-        umx::build_work_items<T>(L->n, L->rp.p, eps, L->fires_per_item, L->edges_per_item, L->instances(), pl, 1, 0,
-                                 L->item_begin, L->item_len, L->item_row, L->item_inst, L->item_aux, L->split,
-                                 L->n_items, L->s);
+        umx::build_work_items<T>(L->n, L->rp.p, eps, L->fires_per_item, L->edges_per_item, row_pieces.p,
+                                 L->instances(), pl, 1, 0, L->item_begin, L->item_len, L->item_row, L->item_inst,
+                                 L->item_aux, L->split, L->n_items, L->s);
         L->phase_items[1] = L->n_items;
         L->phase_split[1] = L->split.rows;
     } else {
@@ -957,8 +959,8 @@ void build_items_t(umx_layout* L, const T* eps) {
         for (int p = 0; p < P; ++p) {
             parts.emplace_back(new Part());
             Part& q = *parts.back();
-            umx::build_work_items<T>(L->n, L->rp.p, eps, L->fires_per_item, L->edges_per_item, L->instances(), pl, P,
-                                     p, q.beg, q.len, q.row, q.inst, q.aux, q.sp, q.n, L->s);
+            umx::build_work_items<T>(L->n, L->rp.p, eps, L->fires_per_item, L->edges_per_item, row_pieces.p,
+                                     L->instances(), pl, P, p, q.beg, q.len, q.row, q.inst, q.aux, q.sp, q.n, L->s);
             items += q.n;
             rows += q.sp.rows;
             slots += q.sp.slots;

@@ -127,13 +127,14 @@ __device__ inline void tally_add(unsigned long long* part, int64_t slot, unsigne
 }
 
 // Reserves one trace record; nullptr when tracing is off or the buffer is full.
+// The record's first value is kind + 4 * the launch's phase.
 template <typename T>
 __device__ inline double* trace_slot(const LayoutView<T>& v, int kind, int64_t head, int64_t other) {
     if (!v.trace) return nullptr;
     const unsigned long long slot = atomicAdd(v.trace_len, 1ULL);
     if (slot >= static_cast<unsigned long long>(v.trace_cap)) return nullptr;
     double* rec = v.trace + slot * (3 + 3 * static_cast<long long>(v.dim));
-    rec[0] = kind;
+    rec[0] = kind + 4 * v.phase;
     rec[1] = static_cast<double>(head);
     rec[2] = static_cast<double>(other);
     return rec;

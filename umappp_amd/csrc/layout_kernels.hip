@@ -297,10 +297,14 @@ __global__ void __launch_bounds__(kBlock) k_layout_pull_generic(LayoutView<T> v,
         auto at = [&](int64_t j) -> const T* {
             return (phase_of(static_cast<uint64_t>(j), v.phases) < v.phase ? dst : src) + j * D;
         };
-        const int64_t i = static_cast<int64_t>(static_cast<uint32_t>(v.item_row[m]) & 0x7fffffffu);
+        const int32_t tagged = v.item_row[m];
+        const int64_t i = static_cast<int64_t>(static_cast<uint32_t>(tagged) & 0x7fffffffu);
         const int64_t s = v.item_begin[m], e = s + v.item_len[m];
         T* h = aux >= 0 ? v.split_delta + static_cast<int64_t>(aux) * D : dst + i * D;
-        for (int d = 0; d < D; ++d) h[d] = src[i * D + d];
+        // a piece whose row was moved by an earlier phase of this epoch
+        // starts where that phase's fold left the head
+        const T* base = tagged < 0 ? dst : src;
+        for (int d = 0; d < D; ++d) h[d] = base[i * D + d];
         const uint32_t n32 = static_cast<uint32_t>(v.n);
         for (int64_t j = s; j < e; ++j) {
             const T nx = v.nxt[j];
@@ -357,13 +361,17 @@ __global__ void __launch_bounds__(kBlock) k_layout_pull_generic(LayoutView<T> v,
             v.nxt_neg[j] = nn + static_cast<T>(nneg) * epns;
         }
         if (aux >= 0)
-            for (int d = 0; d < D; ++d) h[d] -= src[i * D + d];
+            for (int d = 0; d < D; ++d) h[d] -= base[i * D + d];
     }
     tally_add(v.tally_part, m / kWave, fires, draws, self_hits);
 }
 
-// The fold of split rows after a double-buffered pull epoch (one lane per
-// split row; see launch_fold_split).
+// The fold of split rows after a phase of a double-buffered pull epoch (one
+// lane per split row with pieces in the phase; see launch_fold_split): the
+// head's new position is its position at the phase's start (the input
+// buffer, or the output buffer when a fold of an earlier phase of the epoch
+// already moved it: the top bit of split_row) plus the phase's piece moves
+// in edge order.
 template <typename T, int D>
 __global__ void __launch_bounds__(kBlock) k_fold_split(LayoutView<T> v, int epoch_arg) {
     const int64_t r = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
@@ -371,10 +379,11 @@ __global__ void __launch_bounds__(kBlock) k_fold_split(LayoutView<T> v, int epoc
     const int epn = epoch_arg >= 0 ? epoch_arg : v.epoch_ctr[0] + (-1 - epoch_arg);
     const T* src = (epn & 1) ? v.emb_alt : v.emb;
     T* dst = (epn & 1) ? v.emb : v.emb_alt;
-    const int64_t i = v.split_row[r];
+    const int32_t tagged = v.split_row[r];
+    const int64_t i = static_cast<int64_t>(static_cast<uint32_t>(tagged) & 0x7fffffffu);
     const int64_t first = v.split_first[r];
     const int cnt = v.split_count[r];
-    Vec<T, D> h = load_point<T, D>(src, i);
+    Vec<T, D> h = load_point<T, D>(tagged < 0 ? dst : src, i);
     for (int q = 0; q < cnt; ++q) {
         const Vec<T, D> d = load_point<T, D>(v.split_delta, first + q);
 #pragma unroll
@@ -391,11 +400,13 @@ __global__ void __launch_bounds__(kBlock) k_fold_split_generic(LayoutView<T> v, 
     const int epn = epoch_arg >= 0 ? epoch_arg : v.epoch_ctr[0] + (-1 - epoch_arg);
     const T* src = (epn & 1) ? v.emb_alt : v.emb;
     T* dst = (epn & 1) ? v.emb : v.emb_alt;
-    const int64_t i = v.split_row[r];
+    const int32_t tagged = v.split_row[r];
+    const int64_t i = static_cast<int64_t>(static_cast<uint32_t>(tagged) & 0x7fffffffu);
     const int64_t first = v.split_first[r];
     const int cnt = v.split_count[r];
+    const T* base = tagged < 0 ? dst : src;
     for (int c = 0; c < D; ++c) {
-        T h = src[i * D + c];
+        T h = base[i * D + c];
         for (int q = 0; q < cnt; ++q) h += v.split_delta[(first + q) * D + c];
         dst[i * D + c] = h;
     }
@@ -427,12 +438,58 @@ __device__ inline void split_weights(const T* __restrict__ eps, int64_t j0, int6
     for (int u = 0; u < kSplitBatch; ++u) w[u] = 1.0 / static_cast<double>(e[u]);
 }
 
+// Where a row's pieces run: piece q of a row whose observation is in phase
+// p0 runs in phase (p0 + q) mod phases, so the pieces of a split row run one
+// after another through the epoch's phases (only a row of more pieces than
+// phases has several in one phase).  A piece starts from the head as the
+// epoch's earlier phases left it when a piece of its row ran in an earlier
+// phase (tagged in the top bit of its row index), else from the epoch's
+// input.
+struct Stagger {
+    int p0;         // the row's phase
+    int phases;     // a power of two
+    int phase;      // the phase whose pieces are written
+    int64_t total;  // the row's pieces
+};
+
+__host__ __device__ inline int64_t pieces_in_phase(int64_t total, int p0, int phases, int phase) {
+    const int64_t r = static_cast<int64_t>((phase - p0) & (phases - 1));
+    return r < total ? (total - 1 - r) / phases + 1 : 0;
+}
+
+// Whether the pieces of a row (phase p0, 'total' pieces) that run in 'phase'
+// start from the epoch's output buffer: some piece ran in an earlier phase.
+__host__ __device__ inline bool starts_from_output(int64_t total, int p0, int phases, int phase) {
+    const int first = p0 + total - 1 >= phases ? 0 : p0;  // the earliest phase with a piece
+    return phase > first;
+}
+
 template <typename T, bool WRITE>
 __device__ inline int64_t split_row(const int64_t* __restrict__ rp, const T* __restrict__ eps, int64_t i,
-                                    double fires_split, int edges_cap, int64_t m, int64_t* __restrict__ ibeg,
-                                    int32_t* __restrict__ ilen, int32_t* __restrict__ irow, bool split_tag) {
+                                    double fires_split, int edges_cap, const Stagger& st, int64_t m,
+                                    int64_t* __restrict__ ibeg, int32_t* __restrict__ ilen,
+                                    int32_t* __restrict__ irow) {
     const int64_t r0 = rp[i], r1 = rp[i + 1];
     if (r1 == r0) return 0;
+    const bool from_out = WRITE && starts_from_output(st.total, st.p0, st.phases, st.phase);
+    const int32_t tag = static_cast<int32_t>(static_cast<uint32_t>(i) | (from_out ? 0x80000000u : 0u));
+    // piece q (in edge order) is written when it runs in st.phase
+    int64_t q = 0, written = 0;
+    auto piece = [&](int64_t start, int64_t end) {
+        if (WRITE && ((st.p0 + q) & (st.phases - 1)) == st.phase) {
+            ibeg[m + written] = start;
+            ilen[m + written] = static_cast<int32_t>(end - start);
+            irow[m + written] = tag;
+            ++written;
+        }
+        ++q;
+    };
+    // every epochs_per_sample is at least 1, so a row of at most
+    // fires_split entries expects at most that many firings: one piece
+    if (r1 - r0 <= edges_cap && static_cast<double>(r1 - r0) <= fires_split) {
+        piece(r0, r1);
+        return WRITE ? written : q;
+    }
     double f = 0;
     for (int64_t j0 = r0; j0 < r1; j0 += kSplitBatch) {
         double w[kSplitBatch];
@@ -442,18 +499,13 @@ __device__ inline int64_t split_row(const int64_t* __restrict__ rp, const T* __r
             if (j0 + u < r1) f += w[u];
     }
     const double c = f > fires_split ? ceil(f / fires_split) : 1.0;
-    const int32_t tag = static_cast<int32_t>(static_cast<uint32_t>(i) | (split_tag ? 0x80000000u : 0u));
     if (c == 1.0 && r1 - r0 <= edges_cap) {
         // no cut can occur below (q < c never holds; no piece reaches the cap)
-        if (WRITE) {
-            ibeg[m] = r0;
-            ilen[m] = static_cast<int32_t>(r1 - r0);
-            irow[m] = tag;
-        }
-        return 1;
+        piece(r0, r1);
+        return WRITE ? written : q;
     }
-    int64_t items = 0, start = r0;
-    double acc = 0, q = 1;
+    int64_t start = r0;
+    double acc = 0, cq = 1;
     for (int64_t j0 = r0; j0 < r1; j0 += kSplitBatch) {
         double wb[kSplitBatch];
         split_weights(eps, j0, r1, wb);
@@ -462,61 +514,62 @@ __device__ inline int64_t split_row(const int64_t* __restrict__ rp, const T* __r
             const int64_t j = j0 + u;
             if (j < r1) {  // (a guard, not a break: the loop unrolls fully)
                 const double w = wb[u];
-                const bool cut = j > start && ((q < c && acc + w > q * f / c) || j - start >= edges_cap);
+                const bool cut = j > start && ((cq < c && acc + w > cq * f / c) || j - start >= edges_cap);
                 if (cut) {
-                    if (WRITE) {
-                        ibeg[m + items] = start;
-                        ilen[m + items] = static_cast<int32_t>(j - start);
-                        irow[m + items] = tag;
-                    }
-                    ++items;
-                    if (acc + w > q * f / c) q += 1;
+                    piece(start, j);
+                    if (acc + w > cq * f / c) cq += 1;
                     start = j;
                 }
                 acc += w;
             }
         }
     }
-    if (WRITE) {
-        ibeg[m + items] = start;
-        ilen[m + items] = static_cast<int32_t>(r1 - start);
-        irow[m + items] = tag;
-    }
-    return items + 1;
+    piece(start, r1);
+    return WRITE ? written : q;
 }
 
-// Whether row i belongs to phase 'phase' (its index within its instance
-// decides).
-__device__ inline bool in_phase(int64_t i, const int64_t* __restrict__ pt_off, const int32_t* __restrict__ of_row,
-                                int phases, int phase) {
-    if (phases <= 1) return true;
+// The phase of row i (its index within its instance decides).
+__device__ inline int row_phase(int64_t i, const int64_t* __restrict__ pt_off, const int32_t* __restrict__ of_row,
+                                int phases) {
+    if (phases <= 1) return 0;
     const int64_t local = i - (of_row ? pt_off[of_row[i]] : pt_off[0]);
-    return phase_of(static_cast<uint64_t>(local), phases) == phase;
+    return phase_of(static_cast<uint64_t>(local), phases);
 }
 
+// Pieces of every row (phase-independent), once per build.
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_count_items(int64_t n, const int64_t* __restrict__ rp,
-                                                      const T* __restrict__ eps, double fires_split, int edges_cap,
+__global__ void __launch_bounds__(kBlock) k_row_pieces(int64_t n, const int64_t* __restrict__ rp,
+                                                     const T* __restrict__ eps, double fires_split, int edges_cap,
+                                                     int64_t* __restrict__ total) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const Stagger none{0, 1, 0, 0};
+    total[i] = split_row<T, false>(rp, eps, i, fires_split, edges_cap, none, 0, nullptr, nullptr, nullptr);
+}
+
+// Items of each row in phase 'phase'.
+__global__ void __launch_bounds__(kBlock) k_count_items(int64_t n, const int64_t* __restrict__ total,
                                                       const int64_t* __restrict__ pt_off,
                                                       const int32_t* __restrict__ of_row, int phases, int phase,
                                                       int64_t* __restrict__ count) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;
-    count[i] = in_phase(i, pt_off, of_row, phases, phase)
-                   ? split_row<T, false>(rp, eps, i, fires_split, edges_cap, 0, nullptr, nullptr, nullptr, false)
-                   : 0;
+    count[i] = pieces_in_phase(total[i], row_phase(i, pt_off, of_row, phases), phases, phase);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_fill_items(int64_t n, const int64_t* __restrict__ rp,
                                                      const T* __restrict__ eps, double fires_split, int edges_cap,
+                                                     const int64_t* __restrict__ total,
+                                                     const int64_t* __restrict__ pt_off,
+                                                     const int32_t* __restrict__ of_row, int phases, int phase,
                                                      const int64_t* __restrict__ offs, int64_t* __restrict__ ibeg,
                                                      int32_t* __restrict__ ilen, int32_t* __restrict__ irow) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;
-    const int64_t c = offs[i + 1] - offs[i];
-    if (c == 0) return;
-    split_row<T, true>(rp, eps, i, fires_split, edges_cap, offs[i], ibeg, ilen, irow, c > 1);
+    if (offs[i + 1] == offs[i]) return;
+    const Stagger st{row_phase(i, pt_off, of_row, phases), phases, phase, total[i]};
+    split_row<T, true>(rp, eps, i, fires_split, edges_cap, st, offs[i], ibeg, ilen, irow);
 }
 
 // Longest items first: a stable counting sort of the items on a 128-level
@@ -643,20 +696,25 @@ __global__ void __launch_bounds__(kBlock) k_clear_items(int64_t count, int64_t* 
 }
 
 // Split rows (rows cut into more than one item).  Pass 1, per row: its
-// pieces (0 unless split) and whether it is split.
+// pieces in the phase (0 unless the row is split) and whether it has any.
 __global__ void __launch_bounds__(kBlock) k_split_count(int64_t n, const int64_t* __restrict__ offs,
+                                                      const int64_t* __restrict__ total,
                                                       int64_t* __restrict__ pieces, int64_t* __restrict__ is_split) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;
     const int64_t c = offs[i + 1] - offs[i];
-    pieces[i] = c > 1 ? c : 0;
-    is_split[i] = c > 1 ? 1 : 0;
+    const bool split = total[i] > 1 && c > 0;
+    pieces[i] = split ? c : 0;
+    is_split[i] = split ? 1 : 0;
 }
 
 // Pass 2, per row (after exclusive scans of pass 1): the aux value of each
-// of its items before the length sort (the pieces of a row are consecutive
-// there, in edge order) and, for a split row, its entry in the split list.
+// of its items in the phase before the length sort (the pieces of a row are
+// consecutive there, in edge order) and, for a split row, its entry in the
+// split list (its tagged row index, as its items carry it).
 __global__ void __launch_bounds__(kBlock) k_split_fill(int64_t n, const int64_t* __restrict__ offs,
+                                                     const int64_t* __restrict__ total,
+                                                     const int32_t* __restrict__ irow,
                                                      const int64_t* __restrict__ slot_off,
                                                      const int64_t* __restrict__ row_off, int32_t* __restrict__ aux,
                                                      int32_t* __restrict__ srow, int64_t* __restrict__ sfirst,
@@ -664,13 +722,14 @@ __global__ void __launch_bounds__(kBlock) k_split_fill(int64_t n, const int64_t*
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;
     const int64_t p0 = offs[i], c = offs[i + 1] - p0;
-    if (c == 1) {
+    if (c == 0) return;
+    if (total[i] == 1) {
         aux[p0] = kAuxWholeRow;
-    } else if (c > 1) {
+    } else {
         const int64_t s0 = slot_off[i];
         for (int64_t q = 0; q < c; ++q) aux[p0 + q] = static_cast<int32_t>(s0 + q);
         const int64_t r = row_off[i];
-        srow[r] = static_cast<int32_t>(i);
+        srow[r] = irow[p0];
         sfirst[r] = s0;
         scount[r] = static_cast<int32_t>(c);
     }
@@ -744,9 +803,19 @@ void launch_layout_serial(const LayoutView<T>& v, const EpochArgs& ea, const int
 }
 
 template <typename T>
+void count_row_pieces(int64_t n, const int64_t* rp, const T* eps, double fires_split, int edges_cap,
+                      DevBuf<int64_t>& row_pieces, hipStream_t s) {
+    if (!(fires_split > 0) || edges_cap < 1) throw Error(-1, "work split parameters must be positive");
+    row_pieces.alloc(static_cast<size_t>(n > 0 ? n : 1));
+    const unsigned grid = static_cast<unsigned>((n + kBlock - 1) / kBlock > 0 ? (n + kBlock - 1) / kBlock : 1);
+    if (n > 0) k_row_pieces<T><<<grid, kBlock, 0, s>>>(n, rp, eps, fires_split, edges_cap, row_pieces.p);
+    UMX_HIP(hipGetLastError());
+}
+
+template <typename T>
 void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_split, int edges_cap,
-                      const Instances& inst, const ItemPlacement& place, int phases, int phase,
-                      DevBuf<int64_t>& item_begin, DevBuf<int32_t>& item_len, DevBuf<int32_t>& item_row,
+                      const int64_t* row_pieces, const Instances& inst, const ItemPlacement& place, int phases,
+                      int phase, DevBuf<int64_t>& item_begin, DevBuf<int32_t>& item_len, DevBuf<int32_t>& item_row,
                       DevBuf<int32_t>& item_inst, DevBuf<int32_t>& item_aux, SplitPlan& split, int64_t& n_items,
                       hipStream_t s) {
     if (!(fires_split > 0) || edges_cap < 1) throw Error(-1, "work split parameters must be positive");
@@ -760,9 +829,8 @@ void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_s
     const unsigned grid = static_cast<unsigned>((n + kBlock - 1) / kBlock > 0 ? (n + kBlock - 1) / kBlock : 1);
     if (phases < 1 || phases > kMaxPhases || (phases & (phases - 1)) != 0 || phase < 0 || phase >= phases)
         throw Error(-1, "phases must be a power of two in [1, 8] and phase in [0, phases)");
-    if (n > 0)
-        k_count_items<T><<<grid, kBlock, 0, s>>>(n, rp, eps, fires_split, edges_cap, inst.pt_off,
-                                                 inst.n_inst > 1 ? inst.of_row : nullptr, phases, phase, count.p);
+    const int32_t* of_row_p = inst.n_inst > 1 ? inst.of_row : nullptr;
+    if (n > 0) k_count_items<<<grid, kBlock, 0, s>>>(n, row_pieces, inst.pt_off, of_row_p, phases, phase, count.p);
     UMX_HIP(hipGetLastError());
     DevBuf<int64_t> unused;
     exclusive_scan_i64(count.p, offs.p, n, unused, s);
@@ -777,7 +845,8 @@ void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_s
     il.alloc(M);
     ir.alloc(M);
     if (n > 0)
-        k_fill_items<T><<<grid, kBlock, 0, s>>>(n, rp, eps, fires_split, edges_cap, offs.p, ib.p, il.p, ir.p);
+        k_fill_items<T><<<grid, kBlock, 0, s>>>(n, rp, eps, fires_split, edges_cap, row_pieces, inst.pt_off,
+                                                of_row_p, phases, phase, offs.p, ib.p, il.p, ir.p);
     UMX_HIP(hipGetLastError());
 
     // split rows: piece slots numbered row by row in edge order
@@ -789,7 +858,7 @@ void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_s
         flags.alloc(static_cast<size_t>(n > 0 ? n : 1));
         slot_off.alloc(static_cast<size_t>(n + 1));
         row_off.alloc(static_cast<size_t>(n + 1));
-        if (n > 0) k_split_count<<<grid, kBlock, 0, s>>>(n, offs.p, pieces.p, flags.p);
+        if (n > 0) k_split_count<<<grid, kBlock, 0, s>>>(n, offs.p, row_pieces, pieces.p, flags.p);
         UMX_HIP(hipGetLastError());
         exclusive_scan_i64(pieces.p, slot_off.p, n, unused, s);
         exclusive_scan_i64(flags.p, row_off.p, n, unused, s);
@@ -804,8 +873,8 @@ void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_s
         split.first.alloc(static_cast<size_t>(split.rows));
         split.count.alloc(static_cast<size_t>(split.rows));
         if (n > 0)
-            k_split_fill<<<grid, kBlock, 0, s>>>(n, offs.p, slot_off.p, row_off.p, ia.p, split.row.p, split.first.p,
-                                                 split.count.p);
+            k_split_fill<<<grid, kBlock, 0, s>>>(n, offs.p, row_pieces, ir.p, slot_off.p, row_off.p, ia.p,
+                                                 split.row.p, split.first.p, split.count.p);
         UMX_HIP(hipGetLastError());
         UMX_HIP(hipStreamSynchronize(s));
     }
@@ -942,11 +1011,17 @@ void launch_fold_split(const LayoutView<T>& v, int epoch, hipStream_t s) {
     UMX_HIP(hipGetLastError());
 }
 
-template void build_work_items<float>(int64_t, const int64_t*, const float*, double, int, const Instances&,
+template void count_row_pieces<float>(int64_t, const int64_t*, const float*, double, int, DevBuf<int64_t>&,
+                                      hipStream_t);
+template void build_work_items<float>(int64_t, const int64_t*, const float*, double, int, const int64_t*,
+                                      const Instances&,
                                       const ItemPlacement&, int, int, DevBuf<int64_t>&, DevBuf<int32_t>&,
                                       DevBuf<int32_t>&, DevBuf<int32_t>&, DevBuf<int32_t>&, SplitPlan&, int64_t&,
                                       hipStream_t);
-template void build_work_items<double>(int64_t, const int64_t*, const double*, double, int, const Instances&,
+template void count_row_pieces<double>(int64_t, const int64_t*, const double*, double, int, DevBuf<int64_t>&,
+                                       hipStream_t);
+template void build_work_items<double>(int64_t, const int64_t*, const double*, double, int, const int64_t*,
+                                       const Instances&,
                                        const ItemPlacement&, int, int, DevBuf<int64_t>&, DevBuf<int32_t>&,
                                        DevBuf<int32_t>&, DevBuf<int32_t>&, DevBuf<int32_t>&, SplitPlan&, int64_t&,
                                        hipStream_t);

@@ -35,10 +35,15 @@
 // runs in phases, one launch each (observations dealt to phases in blocks of
 // 16 indices, see phase_of): phase p reads the points of earlier phases
 // from the output buffer, as this epoch left them, and every other point
-// (its own head included) from the input buffer.  Every item writes, moved
-// or not: a whole row its head's new position, a piece of a split row its
-// move into its own slot, which a fold after the phase adds to the head in
-// edge order.  No lane reads what another lane of its launch writes and
+// from the input buffer.  Every item writes, moved or not: a whole row its
+// head's new position, a piece of a split row its move into its own slot,
+// which a fold after the phase adds to the head in edge order.  The pieces
+// of a split row run one per phase in turn, starting at the row's own phase
+// (piece q in phase (p0 + q) mod phases), each from the head as the earlier
+// phases' folds left it (the first from the input buffer), so the row's
+// steps follow one another as in one long item; only a row of more pieces
+// than phases has several pieces, summed from one start, in a phase.  No
+// lane reads what another lane of its launch writes and
 // nothing is accumulated with atomics, so the result of an epoch does not
 // depend on the order the wavefronts run in, the tile width, the wavefronts
 // per workgroup or the placement: the same inputs and number of phases give
@@ -250,10 +255,13 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
             n32 = ic.n;
             ekey = mix64(ic.seed + kGold * (static_cast<uint64_t>(epn) + 1));
             alpha = live ? v.alpha_tbl[static_cast<int64_t>(q) * v.alpha_stride + epn] : static_cast<T>(0);
-            i = static_cast<int64_t>(static_cast<uint32_t>(v.item_row[m]) & 0x7fffffffu);
+            const int32_t tagged = v.item_row[m];
+            i = static_cast<int64_t>(static_cast<uint32_t>(tagged) & 0x7fffffffu);
             i32 = static_cast<int32_t>(i);
             e0 = v.item_begin[m];
-            h0 = load_point<T, D>(src, i);
+            // a piece of a split row whose row an earlier phase of this
+            // epoch moved (top bit) starts where that phase's fold left it
+            h0 = load_point<T, D>(tagged < 0 ? dst : src, i);
             h = h0;
         }
         const T two_ab = -2 * v.a * v.b;

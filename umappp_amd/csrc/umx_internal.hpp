@@ -251,8 +251,9 @@ struct LayoutView {
     unsigned long long* tally_part;
     // Work items of the production kernels: item m covers edges
     // [item_begin[m], item_begin[m] + item_len[m]) of row
-    // item_row[m] & 0x7fffffff; the top bit marks rows split over several
-    // items.  Items are ordered longest first.
+    // item_row[m] & 0x7fffffff; the top bit marks a piece of a split row
+    // whose row was moved by an earlier phase of the epoch (it starts from
+    // the output buffer).  Items are ordered longest first.
     const int64_t* item_begin;
     const int32_t* item_len;
     const int32_t* item_row;
@@ -260,8 +261,9 @@ struct LayoutView {
     // item_aux[m]: kAuxPadding for padding items, kAuxWholeRow for an item
     // that is a whole row, else the piece's slot in split_delta (D values:
     // the piece's move of its head, summed onto the head by a fold after
-    // the epoch).  Split row r is observation split_row[r], whose pieces
-    // hold slots split_first[r] .. + split_count[r] - 1 in edge order.
+    // the phase).  Split row r is observation split_row[r] & 0x7fffffff
+    // (top bit: as in item_row), whose pieces in the phase hold slots
+    // split_first[r] .. + split_count[r] - 1 in edge order.
     const int32_t* item_aux;
     T* split_delta;
     const int32_t* split_row;
@@ -367,13 +369,23 @@ struct SplitPlan {
 };
 
 //
-// Only the rows of phase 'phase' of 'phases' get items; their count is
-// padded to whole workgroups (waves * 64 items), so that a phase's items
-// are whole chunks and whole workgroups.
+// Pieces: row_pieces (from count_row_pieces, once per build) holds every
+// row's number of items.  Phase 'phase' of 'phases' gets the whole rows of
+// its observations and, of a row in phase p0 cut into pieces, piece q when
+// (p0 + q) mod phases == phase, so a split row's pieces run one after
+// another through the epoch's phases; such a piece's row index has its top
+// bit set when a piece of the same row runs in an earlier phase (it starts
+// from the head as that phase's fold left it), and so has the row's entry in
+// the split list.  The phase's items are padded to whole workgroups
+// (waves * 64 items), so that a phase's items are whole chunks and whole
+// workgroups.
+template <typename T>
+void count_row_pieces(int64_t n, const int64_t* rp, const T* eps, double fires_split, int edges_cap,
+                      DevBuf<int64_t>& row_pieces, hipStream_t s);
 template <typename T>
 void build_work_items(int64_t n, const int64_t* rp, const T* eps, double fires_split, int edges_cap,
-                      const Instances& inst, const ItemPlacement& place, int phases, int phase,
-                      DevBuf<int64_t>& item_begin, DevBuf<int32_t>& item_len, DevBuf<int32_t>& item_row,
+                      const int64_t* row_pieces, const Instances& inst, const ItemPlacement& place, int phases,
+                      int phase, DevBuf<int64_t>& item_begin, DevBuf<int32_t>& item_len, DevBuf<int32_t>& item_row,
                       DevBuf<int32_t>& item_inst, DevBuf<int32_t>& item_aux, SplitPlan& split, int64_t& n_items,
                       hipStream_t s);
 
@@ -387,10 +399,12 @@ void offset_all(int64_t* a, int64_t count, int64_t by, hipStream_t s);
 template <typename T>
 bool has_reverse_edges(int64_t n, const int64_t* rp, const int32_t* tg, const T* eps, hipStream_t s);
 
-// After an epoch of a double-buffered pull kernel: observation split_row[r]
-// of every split row r gets its state at the start of the epoch plus the
-// moves of its pieces, added in edge order, written to the epoch's output
-// buffer.  'epoch' as for launch_layout_sell.
+// After a phase of a double-buffered pull epoch: observation
+// split_row[r] & 0x7fffffff of every split row r with pieces in the phase
+// gets its state at the start of the phase (the epoch's input buffer, or its
+// output buffer when the top bit of split_row[r] is set) plus the moves of
+// those pieces, added in edge order, written to the epoch's output buffer.
+// 'epoch' as for launch_layout_sell.
 template <typename T>
 void launch_fold_split(const LayoutView<T>& v, int epoch, hipStream_t s);
 
