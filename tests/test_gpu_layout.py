@@ -267,6 +267,62 @@ def test_production_per_step(umx, mode, dtype, dim, tile, waves, split, phases):
         assert np.max(np.abs(after - (before.astype(np.float64) + moved))) <= tol
 
 
+@pytest.mark.parametrize("phases", [2, 4, 8])
+def test_split_pieces_run_in_sequence(umx, phases):
+    """Pull mode, config C1 graph, a work split that cuts most rows into
+    pieces (threshold 8 / P expected firings).  A row cut into at most P
+    pieces runs piece q in phase
+    (p0 + q) mod P; its first piece of the epoch starts from the epoch's
+    input, every later one from the head as the fold after the phase before
+    left it: the piece's first traced step starts at the previous state plus
+    the previous piece's move (its last traced position and move, minus its
+    start), added in float32 as the fold adds it; the epoch's output holds the
+    last fold.  Checked over several epochs for every such row."""
+    _, (rp, idx, dist) = c1_knn()
+    wd, _, _ = oracle.neighbor_similarities(rp, dist)
+    crp, cidx, cw = oracle.combine_neighbor_sets(rp, idx, wd, 1.0)
+    lay, _ = _pair(umx, crp, cidx, cw.astype(np.float32), 500, 5.0, 7, np.float32)
+    emb = np.random.default_rng(3).uniform(-10, 10, size=(2000, 2)).astype(np.float32)
+    lay.set_params(2, A, B, 1.0, 1.0, 7)
+    lay.set_work_split(8.0 / phases)  # most rows in 2..P pieces
+    lay.set_phases(phases)
+    lay.set_embedding(emb)
+    lay.set_trace(600_000)
+    _, _, row, _, ph, from_out = lay.work_items(with_phase=True)
+    count = np.bincount(row, minlength=2000)
+    staged = np.nonzero((count > 1) & (count <= phases))[0]
+    assert staged.size > 100
+    order = np.argsort(row, kind="stable")
+    first = np.searchsorted(row[order], np.arange(2000))
+    checked = 0
+    for limit in range(1, 6):
+        before = lay.get_embedding()
+        lay.run(limit)
+        tr, tph = lay.get_trace(with_phase=True)
+        after = lay.get_embedding()
+        heads = tr[:, 1].astype(np.int64)
+        key = heads * 16 + tph
+        srt = np.argsort(key, kind="stable")  # a lane's records stay in their order
+        ks = key[srt]
+        for i in staged:
+            its = order[first[i]:first[i] + count[i]]
+            pph = np.sort(ph[its])
+            assert np.array_equal(np.sort(from_out[its]), np.sort(pph > pph[0]))
+            state = before[i].astype(np.float32)
+            for p in pph:  # the row's pieces in the epoch's order
+                lo, hi = np.searchsorted(ks, [i * 16 + p, i * 16 + p + 1])
+                if hi > lo:
+                    recs = tr[srt[lo:hi]]
+                    h0 = recs[0, 3:5].astype(np.float32)
+                    assert np.array_equal(h0, state), (limit, i, p)
+                    end = recs[-1, 3:5].astype(np.float32) + recs[-1, 7:9].astype(np.float32)
+                    state = state + (end - h0)
+                    checked += 1
+            assert np.array_equal(after[i], state), (limit, i)
+    assert checked > 500
+    print(f"{phases} phases: {staged.size} rows cut into 2..{phases} pieces, {checked} pieces checked")
+
+
 _MIX = {}
 
 
