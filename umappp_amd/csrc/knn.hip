@@ -250,7 +250,7 @@ __global__ void __launch_bounds__(kQ) k_knn_qlds(int64_t n, int dim, const float
 // and walks all reference points, 128 at a time staged in LDS; each MFMA
 // tile is 16 points (the A operand) by 16 queries, and lane l holds the
 // products of query l % 16 with 4 of the tile's points, so each query's
-// points are split over 4 lanes.  Each lane keeps the 32 smallest
+// points are split over 4 lanes.  Each lane keeps the 24 smallest
 // approximate distances of its points, sorted, and drops a point once it is
 // above its window (its own 17th smallest + 2 Delta: the query itself may be
 // among its points and enters the list like any other, and at least 16 of
@@ -267,8 +267,9 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMfmaQ = 64;      // queries per workgroup (16 per wavefront)
 constexpr int kMfmaRB = 128;    // reference points per LDS block
-constexpr int kMfmaList = 32;   // candidates a lane keeps
+constexpr int kMfmaList = 24;   // candidates a lane keeps (> kMfmaK; 32 made the 1M x 50 search 3% slower)
 constexpr int kMfmaK = 16;      // largest k of the path (the lane window's rank)
+static_assert(kMfmaList > kMfmaK, "a lane's list must hold its window's rank");
 constexpr float kUnit = 5.9604644775390625e-08f;  // 2^-24
 
 // Row stride (floats) of the LDS block: the A operand of one MFMA step reads
