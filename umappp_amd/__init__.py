@@ -528,7 +528,7 @@ class Layout:
         return out + (ph[real], (ru[real] >> 31).astype(bool)) if with_phase else out
 
     def set_phases(self, phases: int):
-        """Phases of a pull-mode epoch: 1, 2, 4 or 8 (default 4)."""
+        """Phases of a pull-mode epoch: 1, 2, 4 or 8 (default 2)."""
         _check(lib().umx_layout_set_phases(self._h, int(phases)))
 
     def phases(self):

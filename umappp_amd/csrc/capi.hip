@@ -693,7 +693,7 @@ struct umx_layout {
     // phases of the pull kernels' epochs (a power of two, 1 to 8): phase p's
     // items are [phase_items[p], phase_items[p + 1]) (whole workgroups), its
     // split rows [phase_split[p], phase_split[p + 1])
-    int phases = 4;
+    int phases = 2;
     std::vector<int64_t> phase_items, phase_split;
     // Only heavy and long rows are split: every phase of an epoch waits for
     // its longest item, and the pieces of a split row run one per phase in
