@@ -668,13 +668,15 @@ struct umx_layout {
     hipStream_t s = nullptr;
     DevBuf<int64_t> rp;
     DevBuf<int32_t> tg;
-    // eps, next, next_neg and the two embedding buffers (float, double).
-    // The embedding at epoch c is in buffer c & 1 (mf / md for even c,
-    // mf2 / md2 for odd c): a double-buffered epoch e reads buffer e & 1 and
-    // writes the other; kernels that update in place (serial replay, push
-    // mode) leave both buffers equal after their run.
-    DevBuf<float> ef, nf, gf, mf, mf2;
-    DevBuf<double> ed, nd, gd, md, md2;
+    // eps, next, next_neg and the two embedding buffers (float, double),
+    // held one after the other in one allocation (mf / md: buffer 0, then
+    // buffer 1 n * dim values further), so that the epoch kernel can address
+    // both from one base.  The embedding at epoch c is in buffer c & 1: a
+    // double-buffered epoch e reads buffer e & 1 and writes the other;
+    // kernels that update in place (serial replay, push mode) leave both
+    // buffers equal after their run.
+    DevBuf<float> ef, nf, gf, mf;
+    DevBuf<double> ed, nd, gd, md;
     DevBuf<unsigned long long> tally;
     DevBuf<unsigned long long> tally_part;  // ring of per-epoch partial tallies (zero between runs)
     DevBuf<unsigned long long> cursor;  // serial replay cursor + overrun flag
@@ -754,9 +756,11 @@ struct umx_layout {
     }
     // embedding buffer 0 or 1
     void* emb_buf(int which) const {
-        if (dtype == UMX_F32) return which ? static_cast<void*>(mf2.p) : static_cast<void*>(mf.p);
-        return which ? static_cast<void*>(md2.p) : static_cast<void*>(md.p);
+        const size_t off = which ? emb_count() : 0;
+        if (dtype == UMX_F32) return static_cast<void*>(mf.p + off);
+        return static_cast<void*>(md.p + off);
     }
+    size_t emb_count() const { return static_cast<size_t>(n) * static_cast<size_t>(dim); }
     // the buffer holding the embedding at the current epoch
     void* emb() const { return emb_buf(current_epoch & 1); }
     size_t emb_bytes() const { return dsize(dtype) * static_cast<size_t>(n) * static_cast<size_t>(dim); }
@@ -787,11 +791,11 @@ DevBuf<double>& split_delta_buf<double>(umx_layout* L) {
 }
 
 // Kernel view.  'emb' and 'emb_alt' are the two embedding buffers for the
-// double-buffered pull kernels (buffer 0 and buffer 1); kernels that update
-// in place get the current buffer as 'emb' (see in_place_view).
+// double-buffered pull kernels (buffer 0 and buffer 1, one allocation);
+// kernels that update in place get the current buffer as 'emb' (see
+// in_place_view).
 template <typename T>
-umx::LayoutView<T> view_of(umx_layout* L, DevBuf<T>& eps, DevBuf<T>& nxt, DevBuf<T>& nng, DevBuf<T>& emb,
-                           DevBuf<T>& emb_alt) {
+umx::LayoutView<T> view_of(umx_layout* L, DevBuf<T>& eps, DevBuf<T>& nxt, DevBuf<T>& nng, T* emb, T* emb_alt) {
     umx::LayoutView<T> v{};  // value-initialised: padding is zero, so views compare bytewise
     v.n = L->n;
     v.nnz = L->nnz;
@@ -800,8 +804,8 @@ umx::LayoutView<T> view_of(umx_layout* L, DevBuf<T>& eps, DevBuf<T>& nxt, DevBuf
     v.eps = eps.p;
     v.nxt = nxt.p;
     v.nxt_neg = nng.p;
-    v.emb = emb.p;
-    v.emb_alt = emb_alt.p;
+    v.emb = emb;
+    v.emb_alt = emb_alt;
     v.phases = 1;
     v.phase = 0;
     v.dim = L->dim;
@@ -843,16 +847,19 @@ umx::LayoutView<T> view_of(umx_layout* L, DevBuf<T>& eps, DevBuf<T>& nxt, DevBuf
     return v;
 }
 
-umx::LayoutView<float> pull_view_f(umx_layout* L) { return view_of<float>(L, L->ef, L->nf, L->gf, L->mf, L->mf2); }
-umx::LayoutView<double> pull_view_d(umx_layout* L) { return view_of<double>(L, L->ed, L->nd, L->gd, L->md, L->md2); }
+umx::LayoutView<float> pull_view_f(umx_layout* L) {
+    return view_of<float>(L, L->ef, L->nf, L->gf, L->mf.p, L->mf.p + L->emb_count());
+}
+umx::LayoutView<double> pull_view_d(umx_layout* L) {
+    return view_of<double>(L, L->ed, L->nd, L->gd, L->md.p, L->md.p + L->emb_count());
+}
 
 // View for the kernels that update the embedding in place (serial replay,
 // push mode): the buffer of the current epoch.
 template <typename T>
-umx::LayoutView<T> in_place_view(umx_layout* L, DevBuf<T>& eps, DevBuf<T>& nxt, DevBuf<T>& nng, DevBuf<T>& b0,
-                                 DevBuf<T>& b1) {
-    umx::LayoutView<T> v = view_of<T>(L, eps, nxt, nng, b0, b1);
-    v.emb = (L->current_epoch & 1) ? b1.p : b0.p;
+umx::LayoutView<T> in_place_view(umx_layout* L, DevBuf<T>& eps, DevBuf<T>& nxt, DevBuf<T>& nng, DevBuf<T>& bufs) {
+    umx::LayoutView<T> v = view_of<T>(L, eps, nxt, nng, bufs.p, bufs.p + L->emb_count());
+    v.emb = (L->current_epoch & 1) ? bufs.p + L->emb_count() : bufs.p;
     v.emb_alt = nullptr;
     return v;
 }
@@ -1502,12 +1509,10 @@ int umx_layout_clone(const umx_layout* Lc, umx_layout** out) {
         copy(L->nf, S->nf);
         copy(L->gf, S->gf);
         copy(L->mf, S->mf);
-        copy(L->mf2, S->mf2);
         copy(L->ed, S->ed);
         copy(L->nd, S->nd);
         copy(L->gd, S->gd);
         copy(L->md, S->md);
-        copy(L->md2, S->md2);
         copy(L->tally, S->tally);
         copy(L->tally_part, S->tally_part);
         copy(L->d_inst_off, S->d_inst_off);
@@ -1566,12 +1571,11 @@ int umx_layout_set_params(umx_layout* L, int num_dim, double a, double b, double
         if (L->dim != num_dim) {
             L->have_emb = false;
             const size_t count = static_cast<size_t>(L->n) * static_cast<size_t>(num_dim);
+            // the two buffers, one after the other
             if (L->dtype == UMX_F32) {
-                L->mf.alloc(count);
-                L->mf2.alloc(count);
+                L->mf.alloc(2 * count);
             } else {
-                L->md.alloc(count);
-                L->md2.alloc(count);
+                L->md.alloc(2 * count);
             }
         }
         L->dim = num_dim;
@@ -1640,7 +1644,7 @@ int umx_layout_run(umx_layout* L, int epoch_limit) {
                 if (L->dtype == UMX_F32) {
                     ea.alpha = epoch_alpha<float>(L->alpha0, ep, L->total_epochs);
                     if (push)
-                        umx::launch_layout_epoch<float>(in_place_view<float>(L, L->ef, L->nf, L->gf, L->mf, L->mf2), ea,
+                        umx::launch_layout_epoch<float>(in_place_view<float>(L, L->ef, L->nf, L->gf, L->mf), ea,
                                                         L->s);
                     else
                         for (int p = 0; p < L->phases; ++p)
@@ -1648,7 +1652,7 @@ int umx_layout_run(umx_layout* L, int epoch_limit) {
                 } else {
                     ea.alpha = epoch_alpha<double>(L->alpha0, ep, L->total_epochs);
                     if (push)
-                        umx::launch_layout_epoch<double>(in_place_view<double>(L, L->ed, L->nd, L->gd, L->md, L->md2),
+                        umx::launch_layout_epoch<double>(in_place_view<double>(L, L->ed, L->nd, L->gd, L->md),
                                                          ea, L->s);
                     else
                         for (int p = 0; p < L->phases; ++p)
@@ -1691,11 +1695,11 @@ int umx_layout_run_serial(umx_layout* L, int epoch_limit, const int32_t* samples
             ea.ekey = umx::counter_epoch_key(L->seed, ep);
             if (L->dtype == UMX_F32) {
                 ea.alpha = epoch_alpha<float>(L->alpha0, ep, L->total_epochs);
-                umx::launch_layout_serial<float>(in_place_view<float>(L, L->ef, L->nf, L->gf, L->mf, L->mf2), ea, sp,
+                umx::launch_layout_serial<float>(in_place_view<float>(L, L->ef, L->nf, L->gf, L->mf), ea, sp,
                                                  count, L->cursor.p, L->s);
             } else {
                 ea.alpha = epoch_alpha<double>(L->alpha0, ep, L->total_epochs);
-                umx::launch_layout_serial<double>(in_place_view<double>(L, L->ed, L->nd, L->gd, L->md, L->md2), ea,
+                umx::launch_layout_serial<double>(in_place_view<double>(L, L->ed, L->nd, L->gd, L->md), ea,
                                                   sp, count, L->cursor.p, L->s);
             }
         }

@@ -273,6 +273,26 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
             const uint32_t local = static_cast<uint32_t>(j) - static_cast<uint32_t>(pt_off);
             return phase_of(local, v.phases) < v.phase ? dst : src;
         };
+        // The same read with 32-bit byte offsets (SMALL: the host checked that
+        // both buffers, which lie one after the other in one allocation, end
+        // within 4 GiB of the first): buffer offset plus the point's offset
+        // from one base, and the phase as one bit-field extract.
+        const char* const emb_base = reinterpret_cast<const char*>(v.emb);
+        const uint32_t alt_off = static_cast<uint32_t>(reinterpret_cast<const char*>(v.emb_alt) - emb_base);
+        const uint32_t src_off = (epn & 1) ? alt_off : 0u;
+        const uint32_t dst_off = alt_off - src_off;
+        const uint32_t phase_bits = static_cast<uint32_t>(31 - __clz(v.phases));
+        auto gather = [&](int64_t j) -> Vec<T, D> {
+            if constexpr (SMALL) {
+                const uint32_t local = static_cast<uint32_t>(j) - static_cast<uint32_t>(pt_off);
+                const uint32_t ph = __builtin_amdgcn_ubfe(local, static_cast<uint32_t>(kPhaseBlockShift), phase_bits);
+                const uint32_t off = (ph < static_cast<uint32_t>(v.phase) ? dst_off : src_off) +
+                                     static_cast<uint32_t>(j) * static_cast<uint32_t>(D * sizeof(T));
+                return load_point<T, D>(reinterpret_cast<const T*>(emb_base + static_cast<uint64_t>(off)), 0);
+            } else {
+                return load_point<T, D>(from(j), j);
+            }
+        };
         // Draws p0 .. p0 + kNegFetch - 1 of an edge and their points, as one
         // batch of unconditional loads (slots past nneg read the head's own
         // point and are marked -1); conditional loads would each be followed
@@ -286,7 +306,7 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
                 const bool use = p0 + u < nneg;
                 kk[u] = use ? d : -1;
                 const int64_t pt = use ? static_cast<int64_t>(d) : i;
-                q[u] = load_point<T, D>(from(pt), pt);
+                q[u] = gather(pt);
             }
         };
 
@@ -354,7 +374,7 @@ __global__ void __launch_bounds__(256) k_layout_sell(LayoutView<T> v, int epoch_
                 const int nneg = tile.nneg[k][lane];
                 draws += static_cast<uint32_t>(nneg > 0 ? nneg : 0);
                 const uint64_t ej = edge_key(ekey, static_cast<uint64_t>(j));
-                Vec<T, D> r = load_point<T, D>(from(t), t);
+                Vec<T, D> r = gather(t);
                 int32_t kk[kNegFetch];
                 Vec<T, D> q[kNegFetch];
                 fetch(ej, 0, nneg, kk, q);
@@ -537,13 +557,18 @@ void launch_sell_tw(const LayoutView<T>& v, int epoch, int waves, hipStream_t s)
     const size_t lds = sizeof(SellTile<T, TW>) * static_cast<size_t>(waves);
     if (lds > static_cast<size_t>(kSellLdsMax))
         throw Error(-1, "LDS tile times waves per workgroup exceeds 64 KiB; use a smaller tile or fewer waves");
-    // the sampler's arithmetic: 32-bit when every instance has at most 2^24
-    // observations (the draw is the same function of key, draw number and n
-    // either way; the choice is made here, once, so that the kernel carries
-    // one of the two)
-    const bool small = v.max_inst_n <= kSmallDraw;
+    // 32-bit arithmetic when every instance has at most 2^24 observations
+    // and both embedding buffers (one allocation, buffer 1 right after
+    // buffer 0) end within 4 GiB of the first: the sampler's draw and the
+    // points' addresses are the same either way; the choice is made here,
+    // once, so that the kernel carries one of the two forms)
+    const uint64_t buf_bytes = static_cast<uint64_t>(v.n) * static_cast<uint64_t>(v.dim) * sizeof(T);
+    const bool small = v.max_inst_n <= kSmallDraw && v.emb_alt == v.emb + static_cast<int64_t>(v.n) * v.dim &&
+                       2 * buf_bytes <= 0xffffffffull;
     if (v.trace) {
-        if (!small) throw Error(-1, "the update trace needs instances of at most 2^24 observations");
+        if (!small)
+            throw Error(-1, "the update trace needs instances of at most 2^24 observations and at most 4 GiB of "
+                            "embedding buffers");
         launch_sell_dim<T, TW, true, true>(v, epoch, grid, block, lds, s);
     } else if (small) {
         launch_sell_dim<T, TW, false, true>(v, epoch, grid, block, lds, s);
