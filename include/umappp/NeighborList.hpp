@@ -1,5 +1,5 @@
 // Drop-in for umappp/NeighborList.hpp (reference NeighborList.hpp:28-29).
-// The reference aliases knncolle::NeighborList; this build has no knncolle
+// The reference aliases knncolle::NeighborList; this library has no knncolle
 // dependency and spells the same type out: per observation, a vector of
 // (neighbour index, distance or weight) pairs, sorted by increasing distance,
 // unique, never containing the observation itself.

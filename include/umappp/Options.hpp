@@ -1,7 +1,7 @@
 // Drop-in for umappp/Options.hpp (reference Options.hpp:27-218).  Same fields,
 // defaults and meanings.  The reference's
 // `irlba::Options<Eigen::VectorXd> initialize_spectral_irlba_options` field
-// becomes the IrlbaOptions stand-in below: this build depends on neither
+// becomes the IrlbaOptions stand-in below: this library depends on neither
 // irlba nor Eigen.
 #ifndef UMAPPP_AMD_OPTIONS_HPP
 #define UMAPPP_AMD_OPTIONS_HPP
