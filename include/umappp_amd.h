@@ -275,8 +275,9 @@ int umx_layout_launch_shape(umx_layout* L, int* tile_cols, int* waves, int* grou
 int umx_layout_set_work_split(umx_layout* L, double fires_per_item, int edges_per_item);
 /* LDS tile of the pull kernel: schedule columns staged per wavefront at a
  * time (4, 8, 16 or 32), or 0 (the default) to choose from the number of
- * 64-item chunks: 32 up to 4096 chunks, 16 up to 8192, 8 above, the widest
- * that fits the 64 KiB LDS budget with the wavefronts per workgroup. */
+ * 64-item chunks: 32 up to 4096 chunks, 16 above (float64: 16 up to 8192,
+ * 8 above), the widest that fits the 64 KiB LDS budget with the wavefronts
+ * per workgroup. */
 int umx_layout_set_tile(umx_layout* L, int tile_cols);
 /* Launch shape of the production kernel: 'waves' wavefronts (1, 2 or 4)
  * per workgroup, and 'groups' placement groups: with groups > 1 the

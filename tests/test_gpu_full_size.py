@@ -8,8 +8,8 @@
 * the fuzzy union's structure (row lengths, indices, order) bit for bit,
   its weights within 1e-6 relative;
 * the epoch schedule of the GPU's graph bit for bit, and after a captured
-  block of 10 epochs of the production kernel (more than 8192 chunks, so its
-  automatic 8-column tile) epoch_of_next_sample,
+  block of 10 epochs of the production kernel (more than 4096 chunks, so its
+  automatic 16-column tile) epoch_of_next_sample,
   epoch_of_next_negative_sample and the per-epoch tallies bit for bit against
   the oracle's counter-sampler run;
 * every step among the first 2,000,000 trace records of epoch 10 equal to
@@ -92,7 +92,8 @@ def test_headline_1m_graph_schedule_and_steps(umx):
     lay.set_embedding(emb0)
     lay.set_graph_block(10)
     begin, _, _, _ = lay.work_items()
-    assert (begin.size + 63) // 64 > 8192  # the automatic tile is 8 columns
+    assert (begin.size + 63) // 64 > 4096  # the automatic tile is 16 columns
+    assert lay.launch_shape()[0] == 16
     lay.run(10)
     orc = oracle.Layout(ep, T, NSR, rng_kind=oracle.RNG_COUNTER, seed=SEED, dtype=np.float32)
     eo = emb0.copy()
