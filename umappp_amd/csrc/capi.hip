@@ -1263,14 +1263,15 @@ void collect_events(umx_layout* L, int count) {
 // does not fill the GPU many times over) that chain sets the epoch time and
 // wide tiles shorten it (MI355X, 200 epochs: 32 columns 20%, 17% and 17%
 // faster than 8 at 32, 1.6k and 3.2k chunks); with many chunks the wide
-// tile's LDS costs occupancy (15.8k chunks: 8 columns 528 us per epoch, 16
-// 551 us, 32 799 us).  The widest that fits the LDS budget is taken.
+// tile's registers cost occupancy.  f32 at 19.4k chunks (a 1M-point epoch,
+// two phases): 4 columns 509 us, 8 505.5 us, 16 500.5 us; 16 columns take
+// 138 VGPRs (3 wavefronts per SIMD, against 89 and 5 at 8), and the
+// gather-bound epoch still ends sooner.  f64's 16-column kernel takes 235
+// VGPRs: 8 columns above 8192 chunks.  The widest that fits the LDS budget
+// is taken.
 int effective_tile(const umx_layout* L) {
     if (L->tile_cols > 0) return L->tile_cols;
     const int64_t chunks = (std::max<int64_t>(L->n_items, 0) + umx::kWave - 1) / umx::kWave;
-    // f32: 16 columns above 4096 chunks (138 VGPRs, 3 wavefronts per SIMD,
-    // against 89 and 5 at 8 columns) run a gather-bound 1M-point epoch 1%
-    // faster.  f64's 16-column kernel takes 235 VGPRs: 8 above 8192 chunks.
     const int64_t wide_max = L->dtype == UMX_F32 ? std::numeric_limits<int64_t>::max() : 8192;
     int t = chunks <= 4096 ? 32 : (chunks <= wide_max ? 16 : 8);
     while (t > 4 && umx::sell_lds_bytes(dsize(L->dtype), t, L->waves) > static_cast<size_t>(umx::kSellLdsMax)) t /= 2;
