@@ -373,7 +373,7 @@ __global__ void __launch_bounds__(256, 2) k_knn_mfma(int64_t n, int dim, const f
     // values of a block each thread stages: the block's rows are one
     // contiguous run of kMfmaRB * dim floats, read flat
     constexpr int kPer = (kMfmaRB * S4 + 255) / 256;
-    __shared__ float rt[kMfmaRB * RS];
+    __shared__ float rt[kMfmaRB * RS + kWave];  // + a spare word per lane for stores past the block
     __shared__ float rn[kMfmaRB];
     __shared__ float mus[4 * 16];
     const int w = threadIdx.x >> 6, l = threadIdx.x & (kWave - 1);
@@ -404,7 +404,12 @@ __global__ void __launch_bounds__(256, 2) k_knn_mfma(int64_t n, int dim, const f
     float drop = INFINITY, win = INFINITY;
     // Each block's values are loaded into registers while the block before
     // is computed (all loads of a block in flight together), then centred
-    // into LDS.
+    // into LDS.  Slot u of a thread holds value threadIdx.x + 256 u of the
+    // block.  Loads and stores take no branch: a load reads a clamped address
+    // and selects, and a lane whose value lies past the block stores into its
+    // own spare word.  (With a guard on each, every load and store was a
+    // branch of its own, the centring waited for each mean in turn, and
+    // staging a block took as long as its 104 MFMAs per wavefront.)
     const int blk_vals = kMfmaRB * dim;
     const int step_r = 256 / dim, step_d = 256 - (256 / dim) * dim;
     float pre[kPer];
@@ -412,12 +417,24 @@ __global__ void __launch_bounds__(256, 2) k_knn_mfma(int64_t n, int dim, const f
     auto load_block = [&](int64_t r0) {
         const float* xb = x + r0 * dim;
         const int avail = static_cast<int>((n - r0 < kMfmaRB ? n - r0 : kMfmaRB) * dim);
+        if ((n - r0) * dim >= static_cast<int64_t>(kPer) * 256) {
+            // every slot inside x: plain loads at fixed offsets (values past
+            // the block are never stored)
 #pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const int e = static_cast<int>(threadIdx.x) + u * 256;
-            pre[u] = e < avail ? xb[e] : 0.0f;
+            for (int u = 0; u < kPer; ++u) pre[u] = xb[threadIdx.x + u * 256];
+        } else {
+#pragma unroll
+            for (int u = 0; u < kPer; ++u) {
+                const int e = static_cast<int>(threadIdx.x) + u * 256;
+                const bool ok = e < avail;
+                const float v = xb[ok ? e : 0];  // row r0 exists: r0 < n
+                pre[u] = ok ? v : 0.0f;
+            }
         }
-        pre_n = (threadIdx.x < kMfmaRB && r0 + threadIdx.x < n) ? nrm[r0 + threadIdx.x] : INFINITY;
+        const int64_t rj = r0 + static_cast<int64_t>(threadIdx.x);
+        const bool okn = threadIdx.x < kMfmaRB && rj < n;
+        const float vn = nrm[okn ? rj : r0];
+        pre_n = okn ? vn : INFINITY;
     };
     load_block(0);
     for (int64_t r0 = 0; r0 < n; r0 += kMfmaRB) {
@@ -428,7 +445,8 @@ __global__ void __launch_bounds__(256, 2) k_knn_mfma(int64_t n, int dim, const f
             int ri = static_cast<int>(threadIdx.x) / dim, d = static_cast<int>(threadIdx.x) - ri * dim;
 #pragma unroll
             for (int u = 0; u < kPer; ++u) {
-                if (static_cast<int>(threadIdx.x) + u * 256 < blk_vals) rt[ri * RS + d] = pre[u] - mus[d];
+                const bool ok = static_cast<int>(threadIdx.x) + u * 256 < blk_vals;
+                rt[ok ? ri * RS + d : kMfmaRB * RS + l] = pre[u] - mus[d];
                 ri += step_r;
                 d += step_d;
                 if (d >= dim) {
