@@ -490,27 +490,35 @@ __global__ void __launch_bounds__(256, 2) k_knn_mfma(int64_t n, int dim, const f
             }
             if (amin <= win) {
 #pragma unroll
-                for (int b = 0; b < 16; ++b) {
-                    const float ap = aps[b];
-                    if (ap <= win && ap < INFINITY) {
-                        if (ap <= lv[kMfmaList - 1]) {
-                            float cv = ap;
-                            int32_t ci = static_cast<int32_t>(r0 + tg + (b >> 2) * 16 + 4 * g + (b & 3));
+                for (int t = 0; t < 4; ++t) {
+                    // a tile's 4 values with nothing at or below the window
+                    // are passed over with one test (the window only shrinks)
+                    const float tm = fminf(fminf(aps[4 * t], aps[4 * t + 1]), fminf(aps[4 * t + 2], aps[4 * t + 3]));
+                    if (!(tm <= win)) continue;
 #pragma unroll
-                            for (int m = 0; m < kMfmaList; ++m) {
-                                const bool lt = cv < lv[m];
-                                const float tv = lv[m];
-                                const int32_t ti = li[m];
-                                lv[m] = lt ? cv : tv;
-                                li[m] = lt ? ci : ti;
-                                cv = lt ? tv : cv;
-                                ci = lt ? ti : ci;
+                    for (int v = 0; v < 4; ++v) {
+                        const int b = 4 * t + v;
+                        const float ap = aps[b];
+                        if (ap <= win && ap < INFINITY) {
+                            if (ap <= lv[kMfmaList - 1]) {
+                                float cv = ap;
+                                int32_t ci = static_cast<int32_t>(r0 + tg + t * 16 + 4 * g + v);
+#pragma unroll
+                                for (int m = 0; m < kMfmaList; ++m) {
+                                    const bool lt = cv < lv[m];
+                                    const float tv = lv[m];
+                                    const int32_t ti = li[m];
+                                    lv[m] = lt ? cv : tv;
+                                    li[m] = lt ? ci : ti;
+                                    cv = lt ? tv : cv;
+                                    ci = lt ? ti : ci;
+                                }
+                                drop = fminf(drop, cv);  // the value pushed out (+inf while the list fills)
+                            } else {
+                                drop = fminf(drop, ap);
                             }
-                            drop = fminf(drop, cv);  // the value pushed out (+inf while the list fills)
-                        } else {
-                            drop = fminf(drop, ap);
+                            win = lv[kMfmaK] + margin2;
                         }
-                        win = lv[kMfmaK] + margin2;
                     }
                 }
             }
