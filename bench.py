@@ -498,7 +498,7 @@ def quality_gate(umx, torch, dev, x, labels, kidx, g, lay, emb0, emb_gpu, ref, n
     y_ref = torch.from_numpy(ref.emb).to(dev)
     m_ref = metrics(y_ref)
 
-    def judged(m):
+    def gated(m):
         d = {"knn15_preservation": abs(m["knn15_preservation"] - m_ref["knn15_preservation"]),
              "knn15_preservation_rel": abs(m["knn15_preservation"] - m_ref["knn15_preservation"])
              / m_ref["knn15_preservation"],
@@ -508,7 +508,7 @@ def quality_gate(umx, torch, dev, x, labels, kidx, g, lay, emb0, emb_gpu, ref, n
               and d["knn15_preservation_rel"] <= PRES_REL_TOL and d["cross_entropy_rel"] <= CE_REL_TOL)
         return dict(m, diff={k_: round(v, 6) for k_, v in d.items()}, passes=bool(ok))
 
-    gpu = judged(metrics(emb_gpu))
+    gpu = gated(metrics(emb_gpu))
     lay10 = umx.Layout(graph=g, num_epochs=T, negative_sample_rate=nsr)
     lay10.set_phases(lay.phases()[0])
     lay10.set_work_split(*lay.work_split())
@@ -517,13 +517,13 @@ def quality_gate(umx, torch, dev, x, labels, kidx, g, lay, emb0, emb_gpu, ref, n
     lay10.run(10)
     y10 = torch.empty_like(emb0)
     lay10.get_embedding(device_ptr=y10.data_ptr())
-    ctl10 = judged(metrics(y10))
+    ctl10 = gated(metrics(y10))
     perm = np.arange(n)
     prng = np.random.default_rng(12)
     for c in np.unique(labels):
         ii = np.nonzero(labels == c)[0]
         perm[ii] = prng.permutation(ii)
-    ctl_sh = judged(metrics(emb_gpu[torch.from_numpy(perm).to(dev)]))
+    ctl_sh = gated(metrics(emb_gpu[torch.from_numpy(perm).to(dev)]))
     res = {"reference": "the reference's serial layout (its loop and mt19937_64 sampler, float path, restated by "
                         "the oracle) from the same kNN lists, initial embedding and seed, on one host core",
            "reference_layout_s": round(ref.seconds, 1),
@@ -535,7 +535,7 @@ def quality_gate(umx, torch, dev, x, labels, kidx, g, lay, emb0, emb_gpu, ref, n
            "negative_controls": {"gpu_stopped_at_epoch_10": ctl10, "gpu_shuffled_within_clusters": ctl_sh},
            "gate_passes": bool(gpu["passes"] and not ctl10["passes"] and not ctl_sh["passes"])}
     if extra is not None:
-        res["extra"] = judged(metrics(extra))
+        res["extra"] = gated(metrics(extra))
     log(f"quality gate: gpu {gpu} reference {m_ref}; controls pass: {ctl10['passes']}, {ctl_sh['passes']}")
     return res
 
