@@ -500,3 +500,26 @@ def test_final_quality_c1_matches_reference_path(umx, dtype):
           f"trustworthiness cpu {t_cpu:.4f} gpu {t_gpu:.4f}")
     assert abs(p_cpu - p_gpu) <= 0.01
     assert abs(t_cpu - t_gpu) <= 0.01
+
+
+@pytest.mark.parametrize("n,dtype,tile", [(20_000, np.float32, 32), (300_000, np.float32, 16),
+                                          (300_000, np.float64, 16), (600_000, np.float64, 8)])
+def test_automatic_tile_choice(umx, n, dtype, tile):
+    """The automatic LDS tile of the production kernel: 32 columns up to 4096
+    64-item chunks, then 16 for float32; for float64 16 up to 8192 chunks and
+    8 above.  Ring graphs (each point's 30 neighbours at offsets +-1..15,
+    weight 1 / (1 + |offset|), symmetric) keep every row one item, so the
+    chunk count is about n / 64, far from either threshold."""
+    offs = np.concatenate([np.arange(-15, 0), np.arange(1, 16)])
+    base = 1.0 / (1.0 + np.abs(offs))
+    cols = (np.arange(n, dtype=np.int64)[:, None] + offs[None, :]) % n
+    order = np.argsort(cols, axis=1)
+    cols = np.take_along_axis(cols, order, axis=1).astype(np.int32)
+    w = np.take_along_axis(np.broadcast_to(base, cols.shape), order, axis=1).astype(dtype)
+    rp = np.arange(n + 1, dtype=np.int64) * offs.size
+    lay = umx.Layout(csr=umx.NeighborList(rp, cols.ravel(), w.ravel()), num_epochs=200, negative_sample_rate=5.0)
+    lay.set_params(2, A, B, 1.0, 1.0, 1)
+    got, _, _, chunks, _ = lay.launch_shape()
+    lo, hi = {32: (0, 4096), 16: (4096, 8192 if dtype == np.float64 else 1 << 40), 8: (8192, 1 << 40)}[tile]
+    assert lo < chunks <= hi, (chunks, lo, hi)
+    assert got == tile
